@@ -1,0 +1,170 @@
+/*
+ * tups_gpu.h — C ABI of the MI355X vertex hot path of the TUPS parquet solver.
+ *
+ * The reference (annakauch/TUPS, Fortran) runs one parquet iteration from
+ * SRC/loop.f90:29-249.  Its hot path is five routines that take no array
+ * arguments and operate on module variables of parquet_ini / parquet_EQMatrix:
+ *
+ *   BSE(ite)                 SRC/parquet_BSE.f90:19          -> tups_bse
+ *   calculate_expSqR/expqSR  SRC/parquet_PhiR.f90:360,400    -> tups_create (tables)
+ *   calc_PhiR(ite)           SRC/parquet_PhiR.f90:35         -> tups_phir
+ *   calc_M_*()               SRC/parquet_EQMatrix.f90:21,126,198,274 -> tups_build_m
+ *   calculate_PAE()          SRC/parquet_EQOuter.f90:16      -> tups_pae
+ *   G_r = F_r - G_r          SRC/loop.f90:219-222            -> tups_gamma_update
+ *
+ * This library keeps those arrays resident in HBM behind an opaque context.
+ * The Fortran side (tups_amd/fortran/tups_gpu_iface.f90) binds every entry
+ * point below with ISO_C_BINDING; the drop-in module replacing the reference
+ * routines is tups_amd/fortran/tups_dropin.f90.
+ *
+ * Conventions
+ *  - All arrays are the reference's Fortran column-major layouts:
+ *      vertex  X(Nz, Nz, Nq)           complex(8)   (G_r_LL, F_r_LL)
+ *      chi     chi(Nl, Nl, Nf, Nq)     complex(8)   compact l-blocks of the
+ *              block-diagonal bubble chi_LL(l1 nu, l2 nu) (SRC/parquet_chi.f90:500-521)
+ *      PhiR    PhiR(Nz*Nz, Nw, NR)     complex(8)   (SRC/parquet_ini.f90:263)
+ *      Lambda  L(Nf, Nf, Nw)           complex(8)   (SRC/parquet_util.f90:302-305)
+ *      M       M(Nl,Nl,Nl,Nl,NIBZ,NR)  complex(8)   stored per IBZ momentum; the
+ *              reference's per-q copy M(...,q,R) equals M(...,momentum(q),R)
+ *    complex(8) is passed as double* pointing at interleaved (re, im).
+ *  - Integers are 32-bit (the reference uses default integer / MKL lp64).
+ *  - Every entry point returns 0 on success and a negative TUPS_E* code on
+ *    failure; tups_last_error() returns a message for the calling thread.
+ *  - One host thread drives one context; calls are stream-ordered on the
+ *    context's HIP stream and return without synchronising unless stated.
+ */
+#ifndef TUPS_GPU_H
+#define TUPS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TUPS_OK 0
+#define TUPS_EINVAL -1    /* bad argument / shape (the reference stops, SRC/preprocessing.f90:95-98) */
+#define TUPS_EHIP -2      /* HIP runtime error */
+#define TUPS_ENOMEM -3    /* device allocation failed */
+#define TUPS_ESTATE -4    /* call order violated (e.g. PAE before M) */
+#define TUPS_ENONFINITE -5/* non-finite value found by tups_check_finite */
+
+/* channel numbering of the reference: d, m, s, t (SRC/parquet_BSE.f90:69-128) */
+#define TUPS_CH_D 0
+#define TUPS_CH_M 1
+#define TUPS_CH_S 2
+#define TUPS_CH_T 3
+
+/* resident arrays addressed by tups_upload / tups_download */
+#define TUPS_GAMMA 0   /* G_r_LL (Nz,Nz,Nq) per channel; holds Phi_r after tups_bse */
+#define TUPS_F 1       /* F_r_LL (Nz,Nz,Nq) per channel */
+#define TUPS_CHI 2     /* chi(Nl,Nl,Nf,Nq); channel 0 = ph bubble, 1 = pp bubble */
+#define TUPS_PHIR 3    /* PhiR_r (Nz*Nz,Nw,NR) per channel */
+#define TUPS_M 4       /* M(Nl,Nl,Nl,Nl,NIBZ,NR); channel 0..3 = phbar_to_ph,
+                          pp_to_ph, ph_to_pp, phbar_to_pp */
+#define TUPS_LAMBDA 5  /* L_r (Nf,Nf,Nw) per channel */
+
+/* BSE modes */
+#define TUPS_BSE_LADDER_FP64 0  /* the reference's damped ladder, FP64 MFMA */
+
+typedef struct tups_dims {
+  int32_t nx;        /* Nx = Ny (even), SRC/parquet_ini.f90:304-305 */
+  int32_t nf;        /* Nf fermionic box (even) */
+  int32_t nl;        /* Nl form factors: 1, 5 or 9 (sym_op_L, SRC/parquet_util.f90:756) */
+  int32_t nr;        /* NR real-space shells; 0 = reference choice {1,4,6} */
+  int32_t nq;        /* q points held; 0 = all of IBZ x Nf/2 (one rank holds everything) */
+  int32_t reserved;
+  double beta;       /* inverse temperature */
+  double f_damping;  /* BSE damping f, prms line 11 */
+} tups_dims;
+
+typedef struct tups_ctx tups_ctx;
+
+/* derived sizes returned by tups_query: */
+typedef struct tups_sizes {
+  int32_t nx, ny, nf, nl, nz, nr, nw, nibz, nq, nc, full_ibz;
+  int32_t reserved;
+  double alpha; /* 1/(beta*Nx*Ny), SRC/parquet_BSE.f90:72 */
+} tups_sizes;
+
+const char* tups_last_error(void);
+
+/* Host-only (no device): the one-time tables for `dims`, in the reference's
+ * layouts FF(Nx,Ny,Nl), FF_inv(8,Nx,Ny,Nl), signs(Nl), reps(Nl),
+ * expSqR(NIBZ,NR,8) and expqSR(Nx*Ny,NR,8) complex (interleaved); any output
+ * pointer may be NULL.  Replaces FF_array/init_FF_inv
+ * (SRC/parquet_formfactors.f90:11-90,545-569) and calculate_expSqR/expqSR
+ * (SRC/parquet_PhiR.f90:360-435). */
+int tups_build_tables(const tups_dims* dims, double* ff, double* ff_inv, int32_t* signs,
+                      int32_t* reps, double* exp_sqr, double* exp_qsr);
+
+/* --- context: replaces set_parameters/Initialization's allocations and the
+ *     one-time tables of SRC/preprocessing.f90:19-133, SRC/parquet_PhiR.f90:360-435 */
+int tups_create(const tups_dims* dims, tups_ctx** out);
+int tups_destroy(tups_ctx* ctx);
+int tups_query(const tups_ctx* ctx, tups_sizes* out);
+/* host tables computed by the library (FF, FF_inv, signs, reps, expSqR, expqSR,
+ * Index_bosonic_IBZ) — a caller that computed its own may overwrite them: */
+int tups_get_tables(const tups_ctx* ctx, double* ff, double* ff_inv, int32_t* signs,
+                    int32_t* reps, double* exp_sqr, double* exp_qsr);
+int tups_set_tables(tups_ctx* ctx, const double* ff, const double* ff_inv,
+                    const int32_t* signs, const int32_t* reps, const double* exp_sqr,
+                    const double* exp_qsr);
+int tups_get_q_map(const tups_ctx* ctx, int32_t* qx, int32_t* qy, int32_t* qw);
+/* synthetic q batches (BSE only): explicit (qx,qy,qw) per slot, 1-based */
+int tups_set_q_map(tups_ctx* ctx, const int32_t* qx, const int32_t* qy, const int32_t* qw);
+
+/* --- host <-> HBM transfers (synchronous w.r.t. the host buffer) */
+int tups_upload(tups_ctx* ctx, int32_t which, int32_t channel, const double* host);
+int tups_download(tups_ctx* ctx, int32_t which, int32_t channel, double* host);
+/* pack a dense (Nz,Nz,nqs) bubble block for slots [q0, q0+nqs) into TUPS_CHI */
+int tups_upload_chi_dense(tups_ctx* ctx, int32_t pp, int32_t q0, int32_t nqs,
+                          const double* dense);
+
+/* --- the hot path (stream-ordered) */
+/* BSE(ite): Gamma_r <- Phi_r for r = d,m,s,t; SRC/parquet_BSE.f90:62-132 */
+int tups_bse(tups_ctx* ctx, int32_t mode);
+/* calc_PhiR(ite): PhiR_r from Gamma_r (holding Phi_r); SRC/parquet_PhiR.f90:35-109 */
+int tups_phir(tups_ctx* ctx);
+/* calc_M_*: the four transformation tensors; SRC/parquet_EQMatrix.f90:21-360 */
+int tups_build_m(tups_ctx* ctx);
+/* calculate_PAE: F_r <- Phi_r + Lambda_r*Nc + crossing; SRC/parquet_EQOuter.f90:16-118 */
+int tups_pae(tups_ctx* ctx);
+/* G_r = F_r - G_r; SRC/loop.f90:219-222 */
+int tups_gamma_update(tups_ctx* ctx);
+
+/* --- stream control and timing */
+int tups_sync(tups_ctx* ctx);
+/* records event `slot` (0..15) on the context stream */
+int tups_event_record(tups_ctx* ctx, int32_t slot);
+/* milliseconds between two recorded events (synchronises on `b`) */
+int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
+/* per-kernel timing: when on, every kernel launch is bracketed by HIP events
+ * on the context stream and synchronised (serialises launches; for profiling
+ * runs only).  tups_set_timing resets the counters. */
+#define TUPS_K_BSE_MIX 0      /* P = alpha Gamma chi, Q = alpha chi Gamma (Nl > 1) */
+#define TUPS_K_BSE_LADDER 1   /* FP64 MFMA ladder GEMM + mixing epilogue */
+#define TUPS_K_PHIR 2
+#define TUPS_K_BUILD_M 3
+#define TUPS_K_PAE 4
+#define TUPS_K_GAMMA_UPDATE 5
+#define TUPS_K_COUNT 8
+int tups_set_timing(tups_ctx* ctx, int32_t on);
+int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);
+/* scans the resident array for NaN/Inf; returns TUPS_ENONFINITE if any */
+int tups_check_finite(tups_ctx* ctx, int32_t which, int32_t channel);
+
+/* --- synthetic inputs on the device (benchmarks): Gamma_r = U*I + X_r with
+ * X_r complex Gaussian, Re/Im iid N(0, sigma^2/2), from a counter-based
+ * splitmix64 stream (seed, array, channel, element).  which = TUPS_GAMMA or TUPS_F. */
+int tups_synth_vertex(tups_ctx* ctx, int32_t which, int32_t channel, uint64_t seed,
+                      double diag, double sigma);
+/* copies resident array (which_src, ch) onto (which_dst, ch) */
+int tups_copy(tups_ctx* ctx, int32_t which_dst, int32_t which_src, int32_t channel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TUPS_GPU_H */

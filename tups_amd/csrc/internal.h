@@ -1,0 +1,125 @@
+// Internal declarations shared by the TUPS MI355X library sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tups_gpu.h"
+
+namespace tups {
+
+constexpr int kNs = 8;       // C4v operations, SRC/parquet_ini.f90:26
+constexpr int kMaxNR = 8;    // register budget of the PhiR / crossing kernels
+constexpr int kEvents = 16;
+
+void set_error(const std::string& msg);
+
+#define TUPS_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      ::tups::set_error(std::string(#call) + ": " + hipGetErrorString(e_));        \
+      return TUPS_EHIP;                                                             \
+    }                                                                               \
+  } while (0)
+
+#define TUPS_CHECK(cond, code, msg)  \
+  do {                               \
+    if (!(cond)) {                   \
+      ::tups::set_error(msg);        \
+      return (code);                 \
+    }                                \
+  } while (0)
+
+// One-time host tables (the reference computes these on the host as well).
+struct Tables {
+  int nx = 0, nf = 0, nl = 0, nr = 0, nw = 0, nibz = 0, nc = 0, nx_ibz = 0;
+  std::vector<int> ibz_x, ibz_y;        // IBZ momenta (1-based), index m
+  std::vector<uint8_t> sym_list;        // [m*8 + s] star operations of IBZ point m
+  std::vector<double> ff;               // FF(kx,ky,l) Fortran order (Nx,Ny,Nl)
+  std::vector<double> ff_inv;           // FF_inv(s,kx,ky,l) Fortran order (Ns,Nx,Ny,Nl)
+  std::vector<int> signs, reps;         // per l
+  std::vector<int> lsym, lfac;          // [s*nl + l] (0-based l)
+  std::vector<double> exp_sqr;          // expSqR(NIBZ,NR,Ns) complex, interleaved
+  std::vector<double> exp_qsr;          // expqSR(Nc,NR,Ns) complex, interleaved
+  std::vector<int> qx, qy, qw;          // per q slot, 1-based
+};
+
+int nl_supported(int nl);
+int nr_default(int nl);
+int build_tables(int nx, int nf, int nl, int nr, Tables& t);
+int rebuild_sym_l(Tables& t);
+void build_q_map_full(Tables& t);
+
+// PhiR source table: for every (l1,l2,m) at most four distinct permuted sources.
+struct PhirTable {
+  std::vector<int> cnt;       // [(l1*nl+l2)*nibz + m]
+  std::vector<int> src;       // [... *4 + slot] = l1s + nl*l2s
+  std::vector<double> coef;   // [((... *4 + slot)*nr + R)] complex interleaved, includes 1/Nx
+};
+void build_phir_table(const Tables& t, PhirTable& out);
+
+}  // namespace tups
+
+struct tups_ctx {
+  tups_sizes sz{};
+  double beta = 0, f_damping = 0;
+  tups::Tables tab;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[tups::kEvents] = {};
+  int device = 0;
+
+  // resident arrays (interleaved complex as double2)
+  double2* gamma[4] = {};
+  double2* fvert[4] = {};
+  double2* spare[4] = {};   // ping-pong partner of gamma (Phi output)
+  double2* chi[2] = {};
+  double2* phir[4] = {};
+  double2* mten[4] = {};
+  double2* lam[4] = {};
+
+  // device copies of the tables
+  double* d_ffk = nullptr;      // [l*nc + k], k = ky*nx + kx
+  double* d_ffik = nullptr;     // [(s*nl + l)*nc + k]
+  double2* d_eqsr = nullptr;    // [(s*nr + R)*nc + k]
+  int* d_qm = nullptr;          // IBZ momentum slot of each q
+  int* d_qw = nullptr;          // bosonic frequency of each q (1-based)
+  int* d_ibz = nullptr;         // IBZ momenta: x[0..nibz), then y[0..nibz) (1-based)
+  int* d_signs = nullptr;
+  int* d_ph_cnt = nullptr;
+  int* d_ph_src = nullptr;
+  double2* d_ph_coef = nullptr;
+  int* d_flag = nullptr;
+
+  double2* work = nullptr;      // BSE operand workspace (Nl > 1)
+  size_t work_elems = 0;
+  int bse_qbatch = 0;
+
+  bool timing = false;
+  hipEvent_t tev[2] = {};
+  double t_ms[TUPS_K_COUNT] = {};
+  long long t_n[TUPS_K_COUNT] = {};
+
+  bool tables_dirty = true;
+  bool m_built = false;
+  bool phir_valid = false;
+};
+
+namespace tups {
+// kernels launchers (defined in the .hip sources)
+int launch_bse_ladder(tups_ctx* c);
+int launch_phir(tups_ctx* c);
+int launch_build_m(tups_ctx* c);
+int launch_pae(tups_ctx* c);
+int launch_gamma_update(tups_ctx* c);
+int launch_synth(tups_ctx* c, double2* dst, size_t n, int nz, uint64_t seed, uint64_t stream_id,
+                 double diag, double sigma);
+int launch_check_finite(tups_ctx* c, const double2* p, size_t n);
+int upload_tables(tups_ctx* c);
+// bracket one kernel launch when per-kernel timing is on
+int timer_begin(tups_ctx* c);
+int timer_end(tups_ctx* c, int kernel);
+}  // namespace tups

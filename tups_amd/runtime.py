@@ -1,0 +1,264 @@
+"""ctypes binding of ``libtups_gpu.so`` (the C ABI declared in include/tups_gpu.h).
+
+The class :class:`TupsContext` mirrors the reference's hot-path routines by
+name (``BSE``, ``calc_PhiR``, ``calc_M_*``, ``calculate_PAE``) and keeps their
+argument meaning: the routines take no arrays — they act on the vertex arrays
+held by the context, exactly as the Fortran routines act on the module
+variables of ``parquet_ini`` (SRC/parquet_ini.f90:128-137).  Arrays cross the
+boundary in the reference's Fortran layouts (numpy ``order='F'``).
+
+There is no CPU fallback: if the HIP library is missing or no GPU is visible
+the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
+
+GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
+K_BSE_MIX, K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE = range(6)
+CH_D, CH_M, CH_S, CH_T = range(4)
+BSE_LADDER_FP64 = 0
+M_NAMES = ("phbar_to_ph", "pp_to_ph", "ph_to_pp", "phbar_to_pp")
+
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE"}
+
+
+class TupsError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed ({ERRORS.get(code, code)}): {msg}")
+        self.code = code
+
+
+class Dims(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("nf", C.c_int32), ("nl", C.c_int32), ("nr", C.c_int32),
+                ("nq", C.c_int32), ("reserved", C.c_int32), ("beta", C.c_double),
+                ("f_damping", C.c_double)]
+
+
+class Sizes(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nx", "ny", "nf", "nl", "nz", "nr", "nw", "nibz", "nq",
+                                         "nc", "full_ibz", "reserved")] + [("alpha", C.c_double)]
+
+
+_P = C.POINTER
+_D = _P(C.c_double)
+_I = _P(C.c_int32)
+_V = C.c_void_p
+
+SIGNATURES = {
+    "tups_last_error": ([], C.c_char_p),
+    "tups_build_tables": ([_P(Dims), _D, _D, _I, _I, _D, _D], C.c_int),
+    "tups_create": ([_P(Dims), _P(_V)], C.c_int),
+    "tups_destroy": ([_V], C.c_int),
+    "tups_query": ([_V, _P(Sizes)], C.c_int),
+    "tups_get_tables": ([_V, _D, _D, _I, _I, _D, _D], C.c_int),
+    "tups_set_tables": ([_V, _D, _D, _I, _I, _D, _D], C.c_int),
+    "tups_get_q_map": ([_V, _I, _I, _I], C.c_int),
+    "tups_set_q_map": ([_V, _I, _I, _I], C.c_int),
+    "tups_upload": ([_V, C.c_int32, C.c_int32, _D], C.c_int),
+    "tups_download": ([_V, C.c_int32, C.c_int32, _D], C.c_int),
+    "tups_upload_chi_dense": ([_V, C.c_int32, C.c_int32, C.c_int32, _D], C.c_int),
+    "tups_bse": ([_V, C.c_int32], C.c_int),
+    "tups_phir": ([_V], C.c_int),
+    "tups_build_m": ([_V], C.c_int),
+    "tups_pae": ([_V], C.c_int),
+    "tups_gamma_update": ([_V], C.c_int),
+    "tups_sync": ([_V], C.c_int),
+    "tups_event_record": ([_V, C.c_int32], C.c_int),
+    "tups_event_elapsed": ([_V, C.c_int32, C.c_int32, _P(C.c_float)], C.c_int),
+    "tups_set_timing": ([_V, C.c_int32], C.c_int),
+    "tups_get_timing": ([_V, C.c_int32, _D, _P(C.c_int64)], C.c_int),
+    "tups_check_finite": ([_V, C.c_int32, C.c_int32], C.c_int),
+    "tups_synth_vertex": ([_V, C.c_int32, C.c_int32, C.c_uint64, C.c_double, C.c_double], C.c_int),
+    "tups_copy": ([_V, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+}
+
+_lib = None
+
+
+def load_library(path: Path | str | None = None) -> C.CDLL:
+    """Load the HIP library once; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build it with `python -m tups_amd.build` "
+                           "(there is no CPU fallback for the hot path)")
+    lib = C.CDLL(str(p))
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int, where: str):
+    if rc != 0:
+        raise TupsError(rc, where, load_library().tups_last_error().decode())
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(_I)
+
+
+def make_dims(nx, nf, nl, nr=0, nq=0, beta=2.0, f_damping=0.5) -> Dims:
+    return Dims(nx, nf, nl, nr, nq, 0, beta, f_damping)
+
+
+def build_tables(nx, nf, nl, nr=0):
+    """Host-only table build (no GPU needed): FF, FF_inv, signs, reps, expSqR, expqSR."""
+    lib = load_library()
+    d = make_dims(nx, nf, nl, nr)
+    nr_ = nr if nr > 0 else {1: 1, 5: 4, 9: 6}.get(nl, 1)
+    nxi = (nx + 2) // 2
+    nibz = nxi * (nxi + 1) // 2
+    ff = np.zeros((nx, nx, nl), order="F")
+    ffi = np.zeros((8, nx, nx, nl), order="F")
+    signs = np.zeros(nl, dtype=np.int32)
+    reps = np.zeros(nl, dtype=np.int32)
+    esq = np.zeros((nibz, nr_, 8), dtype=np.complex128, order="F")
+    eqs = np.zeros((nx * nx, nr_, 8), dtype=np.complex128, order="F")
+    _check(lib.tups_build_tables(C.byref(d), _dptr(ff), _dptr(ffi), _iptr(signs), _iptr(reps),
+                                 esq.ctypes.data_as(_D), eqs.ctypes.data_as(_D)), "tups_build_tables")
+    return ff, ffi, signs, reps, esq, eqs
+
+
+class TupsContext:
+    """HBM-resident state of one rank's share of the TUPS vertex hot path."""
+
+    def __init__(self, nx: int, nf: int, nl: int, nr: int = 0, nq: int = 0, beta: float = 2.0,
+                 f_damping: float = 0.5):
+        self.lib = load_library()
+        self._h = _V()
+        d = make_dims(nx, nf, nl, nr, nq, beta, f_damping)
+        _check(self.lib.tups_create(C.byref(d), C.byref(self._h)), "tups_create")
+        s = Sizes()
+        _check(self.lib.tups_query(self._h, C.byref(s)), "tups_query")
+        self.sizes = s
+        self.nx, self.nf, self.nl, self.nz = s.nx, s.nf, s.nl, s.nz
+        self.nr, self.nw, self.nibz, self.nq, self.nc = s.nr, s.nw, s.nibz, s.nq, s.nc
+        self.alpha = s.alpha
+        self.beta = beta
+        self.f_damping = f_damping
+
+    # --- lifetime -------------------------------------------------------------------
+    def close(self):
+        if self._h:
+            self.lib.tups_destroy(self._h)
+            self._h = _V()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- shapes ---------------------------------------------------------------------
+    def shape(self, which: int):
+        nz, nl, nf = self.nz, self.nl, self.nf
+        return {GAMMA: (nz, nz, self.nq), F: (nz, nz, self.nq), CHI: (nl, nl, nf, self.nq),
+                PHIR: (nz * nz, self.nw, self.nr), M: (nl, nl, nl, nl, self.nibz, self.nr),
+                LAMBDA: (nf, nf, self.nw)}[which]
+
+    # --- transfers ------------------------------------------------------------------
+    def upload(self, which: int, channel: int, arr: np.ndarray):
+        a = np.asfortranarray(arr, dtype=np.complex128)
+        if a.shape != self.shape(which):
+            raise ValueError(f"shape {a.shape} != {self.shape(which)}")
+        _check(self.lib.tups_upload(self._h, which, channel, a.ctypes.data_as(_D)), "tups_upload")
+
+    def download(self, which: int, channel: int) -> np.ndarray:
+        out = np.zeros(self.shape(which), dtype=np.complex128, order="F")
+        _check(self.lib.tups_download(self._h, which, channel, out.ctypes.data_as(_D)), "tups_download")
+        return out
+
+    def upload_chi_dense(self, pp: int, dense: np.ndarray, q0: int = 0):
+        a = np.asfortranarray(dense, dtype=np.complex128)
+        if a.shape[:2] != (self.nz, self.nz):
+            raise ValueError("dense bubble must be (Nz, Nz, nq)")
+        _check(self.lib.tups_upload_chi_dense(self._h, pp, q0, a.shape[2], a.ctypes.data_as(_D)),
+               "tups_upload_chi_dense")
+
+    def q_map(self):
+        qx = np.zeros(self.nq, dtype=np.int32)
+        qy = np.zeros(self.nq, dtype=np.int32)
+        qw = np.zeros(self.nq, dtype=np.int32)
+        _check(self.lib.tups_get_q_map(self._h, _iptr(qx), _iptr(qy), _iptr(qw)), "tups_get_q_map")
+        return qx, qy, qw
+
+    def set_q_map(self, qx, qy, qw):
+        qx, qy, qw = (np.ascontiguousarray(v, dtype=np.int32) for v in (qx, qy, qw))
+        _check(self.lib.tups_set_q_map(self._h, _iptr(qx), _iptr(qy), _iptr(qw)), "tups_set_q_map")
+
+    # --- the reference's routines -------------------------------------------------------
+    def BSE(self, ite: int = 0, mode: int = BSE_LADDER_FP64):
+        """BSE(ite), SRC/parquet_BSE.f90:19-135: G_r_LL <- Phi_r for r = d, m, s, t."""
+        _check(self.lib.tups_bse(self._h, mode), "tups_bse")
+
+    def calc_PhiR(self, ite: int = 0):
+        """calc_PhiR(ite), SRC/parquet_PhiR.f90:35-109."""
+        _check(self.lib.tups_phir(self._h), "tups_phir")
+
+    def calc_M(self):
+        """calc_M_phbar_to_ph / pp_to_ph / ph_to_pp / phbar_to_pp, SRC/parquet_EQMatrix.f90:21-360."""
+        _check(self.lib.tups_build_m(self._h), "tups_build_m")
+
+    calc_M_phbar_to_ph = calc_M_pp_to_ph = calc_M_ph_to_pp = calc_M_phbar_to_pp = calc_M
+
+    def calculate_PAE(self):
+        """calculate_PAE, SRC/parquet_EQOuter.f90:16-118."""
+        _check(self.lib.tups_pae(self._h), "tups_pae")
+
+    def gamma_update(self):
+        """G_r_LL = F_r_LL - G_r_LL, SRC/loop.f90:219-222."""
+        _check(self.lib.tups_gamma_update(self._h), "tups_gamma_update")
+
+    # --- stream / timing ----------------------------------------------------------------
+    def sync(self):
+        _check(self.lib.tups_sync(self._h), "tups_sync")
+
+    def record(self, slot: int):
+        _check(self.lib.tups_event_record(self._h, slot), "tups_event_record")
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = C.c_float()
+        _check(self.lib.tups_event_elapsed(self._h, a, b, C.byref(ms)), "tups_event_elapsed")
+        return float(ms.value)
+
+    def set_timing(self, on: bool):
+        """Per-kernel HIP-event timing on the context stream (serialises launches)."""
+        _check(self.lib.tups_set_timing(self._h, 1 if on else 0), "tups_set_timing")
+
+    def kernel_time(self, kernel: int):
+        """(total milliseconds, launches) accumulated for one kernel id."""
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(self.lib.tups_get_timing(self._h, kernel, C.byref(ms), C.byref(n)), "tups_get_timing")
+        return float(ms.value), int(n.value)
+
+    def check_finite(self, which: int, channel: int):
+        _check(self.lib.tups_check_finite(self._h, which, channel), "tups_check_finite")
+
+    def synth_vertex(self, which: int, channel: int, seed: int, diag: float, sigma: float):
+        _check(self.lib.tups_synth_vertex(self._h, which, channel, seed, diag, sigma), "tups_synth_vertex")
+
+    def copy(self, which_dst: int, which_src: int, channel: int):
+        _check(self.lib.tups_copy(self._h, which_dst, which_src, channel), "tups_copy")
