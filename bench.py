@@ -21,6 +21,7 @@ import sys
 import time
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 MFMA, AMD published
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E, AMD published
 
 
 def parse():
@@ -76,7 +77,7 @@ def main():
         sys.exit(2)
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from tups_amd import TupsContext
-    from tups_amd.runtime import K_BSE_LADDER, K_BSE_MIX
+    from tups_amd.runtime import K_BSE_LADDER
     from tups_amd.synth import batched_bse_inputs
 
     nx, nf, nl, U = 8, 64, 5, 2.0
@@ -86,6 +87,8 @@ def main():
     D, nq = ctx.nz, ctx.nq
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s D={D} nq={nq}", file=sys.stderr, flush=True)
 
+    variant, tune_ms = ctx.autotune()
+    print(f"[bench] autotuner picked {variant} ({tune_ms:.2f} ms/launch)", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ctx.BSE(0)
     ctx.sync()
@@ -100,22 +103,41 @@ def main():
     print(f"[bench] {args.steps} steps {total_ms:.1f} ms -> {qps:.1f} q-points/s", file=sys.stderr, flush=True)
 
     # live roofline of the dominant kernel: per-launch HIP events on the library stream
+    nprof = max(2, min(args.steps, 5))
     ctx.set_timing(True)
-    for _ in range(max(2, min(args.steps, 5))):
+    for _ in range(nprof):
         ctx.BSE(0)
     lad_ms, lad_n = ctx.kernel_time(K_BSE_LADDER)
-    mix_ms, mix_n = ctx.kernel_time(K_BSE_MIX)
     ctx.set_timing(False)
-    launches_per_step = lad_n / max(2, min(args.steps, 5))
-    q_per_launch = nq / launches_per_step
+    q_per_launch = nq * nprof / lad_n
     flops_per_launch = 16.0 * D ** 3 * 4 * q_per_launch  # 2 complex DxD GEMMs per (q, channel)
     avg_launch_s = lad_ms / lad_n / 1e3
     achieved = flops_per_launch / avg_launch_s / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "bse_ladder_kernel<false>", "avg_launch_ms": round(lad_ms / lad_n, 4),
-                "prologue_ms_per_step": round(mix_ms / max(1, launches_per_step) if mix_n else 0.0, 4)}
+                "kernel": f"bse_ladder_kernel variant {variant}", "avg_launch_ms": round(lad_ms / lad_n, 4),
+                "flops_per_launch": flops_per_launch}
     ctx.close()
+
+    # crossing (calculate_PAE) at Nl = 1 on the 16x16 / Nf = 64 box: the HBM-bound regime
+    from tups_amd.profile_workloads import crossing_bytes, crossing_context
+    from tups_amd.runtime import K_PAE
+
+    cx = crossing_context(16, 64, 1)
+    for _ in range(2):
+        cx.calculate_PAE()
+    cx.set_timing(True)
+    for _ in range(10):
+        cx.calculate_PAE()
+    pae_ms, pae_n = cx.kernel_time(K_PAE)
+    cx.set_timing(False)
+    cbytes = crossing_bytes(cx)
+    gbs = cbytes / (pae_ms / pae_n / 1e3) / 1e9
+    crossing = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "pae_kernel",
+                "avg_launch_ms": round(pae_ms / pae_n, 4), "bytes_per_launch": cbytes,
+                "workload": f"calculate_PAE Nx=16 Nf=64 Nl=1 NR=1, Nq={cx.nq} (all IBZ q), 4 channels"}
+    cx.close()
 
     cpu = None
     if not args.no_cpu:
@@ -138,6 +160,7 @@ def main():
                                f"Nq={nq} (64 momenta x 16 bosonic freq), FP64 path",
                    "nx": nx, "nf": nf, "nl": nl, "nq": nq, "D": D, "path": "fp64-mfma"},
         "roofline": roofline,
+        "crossing_roofline": crossing,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -135,6 +135,17 @@ int tups_pae(tups_ctx* ctx);
 /* G_r = F_r - G_r; SRC/loop.f90:219-222 */
 int tups_gamma_update(tups_ctx* ctx);
 
+/* Project autotuner: times the tile variants of `kernel` (TUPS_K_BSE_LADDER)
+ * on the context's resident inputs and keeps the fastest for later calls.
+ * Synchronous.  `variant` receives the chosen variant id, `ms` its time per
+ * launch; tups_variant_name() names a variant id. */
+int tups_autotune(tups_ctx* ctx, int32_t kernel, int32_t* variant, float* ms);
+const char* tups_variant_name(int32_t kernel, int32_t variant);
+/* forces a variant id (must match the context's Nl); -1 restores the default */
+int tups_set_variant(tups_ctx* ctx, int32_t kernel, int32_t variant);
+/* number of variant ids of `kernel` (ids 0 .. n-1; each serves one Nl) */
+int tups_variant_count(int32_t kernel);
+
 /* --- stream control and timing */
 int tups_sync(tups_ctx* ctx);
 /* records event `slot` (0..15) on the context stream */
@@ -144,12 +155,11 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 /* per-kernel timing: when on, every kernel launch is bracketed by HIP events
  * on the context stream and synchronised (serialises launches; for profiling
  * runs only).  tups_set_timing resets the counters. */
-#define TUPS_K_BSE_MIX 0      /* P = alpha Gamma chi, Q = alpha chi Gamma (Nl > 1) */
-#define TUPS_K_BSE_LADDER 1   /* FP64 MFMA ladder GEMM + mixing epilogue */
-#define TUPS_K_PHIR 2
-#define TUPS_K_BUILD_M 3
-#define TUPS_K_PAE 4
-#define TUPS_K_GAMMA_UPDATE 5
+#define TUPS_K_BSE_LADDER 0   /* FP64 MFMA ladder GEMM, chi mixing and damping fused */
+#define TUPS_K_PHIR 1
+#define TUPS_K_BUILD_M 2
+#define TUPS_K_PAE 3
+#define TUPS_K_GAMMA_UPDATE 4
 #define TUPS_K_COUNT 8
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);

@@ -14,7 +14,7 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def _declared_symbols():
     text = (ROOT / "include" / "tups_gpu.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int)\s+(tups_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*\s*|int\s+)(tups_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -197,3 +197,36 @@ def test_synth_vertex_statistics():
     v = off.ravel()
     assert abs(np.var(v.real) - 0.18) < 0.01 and abs(np.var(v.imag) - 0.18) < 0.01
     assert abs(np.mean(v)) < 0.01
+
+
+@pytest.mark.parametrize("nx,nf,nl,nq", [(8, 64, 1, 4), (4, 8, 1, 0), (8, 64, 5, 2), (4, 6, 5, 0),
+                                         (16, 64, 9, 2), (4, 8, 9, 0)])
+def test_every_bse_variant_and_autotune(nx, nf, nl, nq):
+    """Each tile variant (frequency chunk, LDS double buffering) and the autotuner's
+    pick reproduce the oracle; Nf=6 exercises a partial frequency chunk."""
+    from tups_amd import CHI, F, GAMMA
+
+    rng = np.random.default_rng(nf * 100 + nl)
+    with _ctx(nx=nx, nf=nf, nl=nl, nq=nq, beta=2.0, f_damping=0.5) as c:
+        lat = Lattice(nx, nf, nl)
+        gam = random_vertices(rng, c.nz, c.nq)
+        fv = random_vertices(rng, c.nz, c.nq)
+        chic = [random_chi_compact(rng, nl, nf, c.nq, 3.0) for _ in range(2)]
+        exp = B.bse_ladder(lat, gam, fv, [B.chi_compact_to_dense(lat, x) for x in chic], 2.0, 0.5)
+        for p in range(2):
+            c.upload(CHI, p, chic[p])
+        variants = c.variants()
+        assert variants
+        for v, name in variants + [(-1, "default"), (None, "autotuned")]:
+            _say(f"variant {name} nx={nx} nf={nf} nl={nl}")
+            for r in range(4):
+                c.upload(GAMMA, r, gam[r])
+                c.upload(F, r, fv[r])
+            if v is None:
+                name, ms = c.autotune()
+                assert ms > 0
+            else:
+                c.set_variant(v)
+            c.BSE(0)
+            for r in range(4):
+                assert rel_fro(c.download(GAMMA, r), exp[r]) < TOL, (name, r)

@@ -14,11 +14,17 @@
 // Kernel layout: one workgroup (4 waves) per 64x64 output tile of one (q, r)
 // matrix; each wave owns a 32x32 sub-tile = 2x2 blocks of 16x16 complex.  A
 // complex block uses four real MFMAs per k-step (re*re, -im*im, re*im, im*re)
-// on v_mfma_f64_16x16x4f64, accumulators kept separate for Re and Im.  Operand
-// tiles (16 k-columns) are staged through LDS as split Re/Im planes so each
-// MFMA operand is one aligned double per lane.  The mixing term is fused into
-// the epilogue.  For Nl = 1 the chi scaling of P and Q is applied while the
-// tile is loaded (chi is diagonal); for Nl > 1 a prologue kernel forms P and Q.
+// on v_mfma_f64_16x16x4f64, accumulators kept separate for Re and Im.
+//
+// The chi mixing is fused: the inner dimension is walked in chunks of G
+// fermionic frequencies times all Nl form factors, so a loader thread that owns
+// one (row, nu) or (nu, column) pair holds all Nl values that the Nl x Nl block
+// chi_nu couples and applies it in registers before the tile goes to LDS:
+//   part 0: A = Gamma(:, S),  B = alpha chi F (S, :)   (rows of F mixed)
+//   part 1: A = F alpha chi (:, S),  B = Gamma(S, :)   (columns of F mixed)
+// Operand tiles are stored as split Re/Im planes so each MFMA operand is one
+// double per lane.  Global loads of chunk c+1 are issued before the MFMAs of
+// chunk c (register prefetch), and the damped mixing is fused into the epilogue.
 #include "internal.h"
 
 namespace tups {
@@ -27,16 +33,13 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64;
 constexpr int BN = 64;
-constexpr int BK = 16;
 
 struct BseArgs {
   const double2* gam[4];   // Gamma_r, (D, D, nq)
   const double2* fv[4];    // F_r
   double2* out[4];         // Phi_r (must not alias gam)
   const double2* chi[2];   // (Nl, Nl, Nf, nq): ph, pp
-  const double2* ws;       // P, Q workspace for q in [q0, q0 + nqb): [(r*nqb + qb)*2 + {0,1}] * D*D
-  int D, nf, nl;
-  int q0, nqb, tiles_n;
+  int D, nf, nq, tiles_n;
   double alpha;
   double cf[4];            // c_r * f
   double omf;              // 1 - f
@@ -50,39 +53,29 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-template <bool kScaled>
-__global__ __launch_bounds__(256) void bse_ladder_kernel(BseArgs a) {
-  __shared__ double As[2][BK][BM];
-  __shared__ double Bs[2][BK][BN];
+template <int NL, int G, bool DB>
+__global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
+  constexpr int KC = NL * G;   // inner-dimension chunk: G frequencies x NL form factors
+  constexpr int PP = G / 4;    // (row|col, nu) pairs per thread: 64 * G / 256
+  constexpr int NB = DB ? 2 : 1;
+  static_assert(KC % 4 == 0, "chunk must be a multiple of the MFMA k");
+  __shared__ double As[NB][2][KC][BM];
+  __shared__ double Bs[NB][2][KC][BN];
 
   const int tiles = a.tiles_n * a.tiles_n;
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
   const int mat = logical / tiles;
   const int t = logical - mat * tiles;
-  const int r = mat / a.nqb;
-  const int qb = mat - r * a.nqb;
-  const int q = a.q0 + qb;
-  const int D = a.D;
+  const int r = mat / a.nq;
+  const int q = mat - r * a.nq;
+  const int D = a.D, nf = a.nf;
   const size_t DD = (size_t)D * D;
   const int row0 = (t / a.tiles_n) * BM;
   const int col0 = (t % a.tiles_n) * BN;
-
-  const double2* G = a.gam[r] + DD * q;
-  const double2* F = a.fv[r] + DD * q;
-  const double2* A1;
-  const double2* B2;
-  if (kScaled) {
-    A1 = G;
-    B2 = G;
-  } else {
-    A1 = a.ws + DD * ((size_t)(r * a.nqb + qb) * 2);
-    B2 = A1 + DD;
-  }
-  const double2* chiq = a.chi[r >> 1] + (size_t)a.nf * a.nl * a.nl * q;
+  const double2* Gm = a.gam[r] + DD * q;
+  const double2* Fm = a.fv[r] + DD * q;
+  const double2* chiq = a.chi[r >> 1] + (size_t)nf * NL * NL * q;
+  const double alpha = a.alpha;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -101,74 +94,134 @@ __global__ __launch_bounds__(256) void bse_ladder_kernel(BseArgs a) {
       acc_im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     }
 
-  // A-tile loader: 16 columns x 64 rows, thread -> (column ka, rows ia..ia+3)
-  const int ka = tid >> 4;
-  const int ia = (tid & 15) * 4;
-  // B-tile loader: 64 columns x 16 rows, thread -> (column jb, rows kb..kb+3)
-  const int jb = tid >> 2;
-  const int kb = (tid & 3) * 4;
+  const int nch = (nf + G - 1) / G;
+  const int niter = 2 * nch;
+  double2 ra[PP][NL], rb[PP][NL];
 
-  for (int part = 0; part < 2; ++part) {
-    const double2* A = part == 0 ? A1 : F;
-    const double2* B = part == 0 ? F : B2;
-    for (int k0 = 0; k0 < D; k0 += BK) {
-      {
-        const int gk = k0 + ka;
-        double2 sc = make_double2(0.0, 0.0);
-        if (kScaled && part == 0 && gk < D) {
-          const double2 c = chiq[gk];
-          sc = make_double2(a.alpha * c.x, a.alpha * c.y);
-        }
+  // raw global loads of chunk `it` (A pairs: (row i, nu); B pairs: (nu, column j))
+  auto load = [&](int it) {
+    const int part = it >= nch ? 1 : 0;
+    const int nu0 = (it - part * nch) * G;
+    const double2* Asrc = part ? Fm : Gm;
+    const double2* Bsrc = part ? Gm : Fm;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int gi = row0 + ia + j;
-          double2 v = make_double2(0.0, 0.0);
-          if (gi < D && gk < D) v = A[(size_t)gk * D + gi];
-          if (kScaled && part == 0) v = cmul(v, sc);
-          As[0][ka][ia + j] = v.x;
-          As[1][ka][ia + j] = v.y;
-        }
-      }
-      {
-        const int gj = col0 + jb;
+    for (int p = 0; p < PP; ++p) {
+      const int u = tid + 256 * p;
+      const int i = u & 63, g = u >> 6;
+      const int nu = nu0 + g, gi = row0 + i;
+      const bool ok = gi < D && nu < nf;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int gk = k0 + kb + j;
-          double2 v = make_double2(0.0, 0.0);
-          if (gj < D && gk < D) {
-            v = B[(size_t)gj * D + gk];
-            if (kScaled && part == 1) {
-              const double2 c = chiq[gk];
-              v = cmul(v, make_double2(a.alpha * c.x, a.alpha * c.y));
-            }
+      for (int l = 0; l < NL; ++l)
+        ra[p][l] = ok ? Asrc[(size_t)(l * nf + nu) * D + gi] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int p = 0; p < PP; ++p) {
+      const int u = tid + 256 * p;
+      const int g = u % G, j = u / G;
+      const int nu = nu0 + g, gj = col0 + j;
+      const bool ok = gj < D && nu < nf;
+#pragma unroll
+      for (int l = 0; l < NL; ++l)
+        rb[p][l] = ok ? Bsrc[(size_t)gj * D + l * nf + nu] : make_double2(0.0, 0.0);
+    }
+  };
+
+  // chi mixing in registers, then store chunk `it` to LDS (k index = l*G + g)
+  auto store = [&](int it) {
+    const int part = it >= nch ? 1 : 0;
+    const int sb = DB ? (it & 1) : 0;
+    const int nu0 = (it - part * nch) * G;
+#pragma unroll
+    for (int p = 0; p < PP; ++p) {
+      const int u = tid + 256 * p;
+      const int i = u & 63, g = u >> 6;
+      const int nu = nu0 + g < nf ? nu0 + g : nf - 1;
+      const double2* cb = chiq + (size_t)nu * NL * NL;
+#pragma unroll
+      for (int l2 = 0; l2 < NL; ++l2) {
+        double2 v = ra[p][l2];
+        if (part) {  // (F alpha chi)(i, (l2, nu)) = alpha sum_l3 F(i, (l3, nu)) chi_nu(l3, l2)
+          double sr = 0.0, si = 0.0;
+#pragma unroll
+          for (int l3 = 0; l3 < NL; ++l3) {
+            const double2 x = cb[l3 + NL * l2];
+            sr += ra[p][l3].x * x.x - ra[p][l3].y * x.y;
+            si += ra[p][l3].x * x.y + ra[p][l3].y * x.x;
           }
-          Bs[0][kb + j][jb] = v.x;
-          Bs[1][kb + j][jb] = v.y;
+          v = make_double2(alpha * sr, alpha * si);
         }
+        As[sb][0][l2 * G + g][i] = v.x;
+        As[sb][1][l2 * G + g][i] = v.y;
       }
+    }
+#pragma unroll
+    for (int p = 0; p < PP; ++p) {
+      const int u = tid + 256 * p;
+      const int g = u % G, j = u / G;
+      const int nu = nu0 + g < nf ? nu0 + g : nf - 1;
+      const double2* cb = chiq + (size_t)nu * NL * NL;
+#pragma unroll
+      for (int l1 = 0; l1 < NL; ++l1) {
+        double2 v = rb[p][l1];
+        if (!part) {  // (alpha chi F)((l1, nu), j) = alpha sum_l3 chi_nu(l1, l3) F((l3, nu), j)
+          double sr = 0.0, si = 0.0;
+#pragma unroll
+          for (int l3 = 0; l3 < NL; ++l3) {
+            const double2 x = cb[l1 + NL * l3];
+            sr += x.x * rb[p][l3].x - x.y * rb[p][l3].y;
+            si += x.x * rb[p][l3].y + x.y * rb[p][l3].x;
+          }
+          v = make_double2(alpha * sr, alpha * si);
+        }
+        Bs[sb][0][l1 * G + g][j] = v.x;
+        Bs[sb][1][l1 * G + g][j] = v.y;
+      }
+    }
+  };
+
+  auto mfma_chunk = [&](int buf) {
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int kr = ks * 4 + lk;
+      double ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = As[buf][0][kr][wr + i * 16 + li];
+        ai[i] = As[buf][1][kr][wr + i * 16 + li];
+        nai[i] = -ai[i];
+        br[i] = Bs[buf][0][kr][wc + i * 16 + li];
+        bi[i] = Bs[buf][1][kr][wc + i * 16 + li];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc_re[i][j], 0, 0, 0);
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], acc_re[i][j], 0, 0, 0);
+          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acc_im[i][j], 0, 0, 0);
+          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acc_im[i][j], 0, 0, 0);
+        }
+    }
+  };
+
+  load(0);
+  if (DB) {
+    // two LDS buffers, one barrier per chunk: the mixing and LDS stores of
+    // chunk it+1 follow the MFMAs of chunk it in the same instruction stream
+    store(0);
+    __syncthreads();
+    for (int it = 0; it < niter; ++it) {
+      if (it + 1 < niter) load(it + 1);
+      mfma_chunk(it & 1);
+      if (it + 1 < niter) store(it + 1);
       __syncthreads();
-#pragma unroll
-      for (int ks = 0; ks < BK / 4; ++ks) {
-        const int kr = ks * 4 + lk;
-        double ar[2], ai[2], nai[2], br[2], bi[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          ar[i] = As[0][kr][wr + i * 16 + li];
-          ai[i] = As[1][kr][wr + i * 16 + li];
-          nai[i] = -ai[i];
-          br[i] = Bs[0][kr][wc + i * 16 + li];
-          bi[i] = Bs[1][kr][wc + i * 16 + li];
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc_re[i][j], 0, 0, 0);
-            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], acc_re[i][j], 0, 0, 0);
-            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acc_im[i][j], 0, 0, 0);
-            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acc_im[i][j], 0, 0, 0);
-          }
-      }
+    }
+  } else {
+    for (int it = 0; it < niter; ++it) {
+      store(it);
+      __syncthreads();
+      if (it + 1 < niter) load(it + 1);
+      mfma_chunk(0);
       __syncthreads();
     }
   }
@@ -187,8 +240,8 @@ __global__ __launch_bounds__(256) void bse_ladder_kernel(BseArgs a) {
         const int gi = row0 + wr + i * 16 + lk + 4 * e;
         if (gi < D && gj < D) {
           const size_t idx = (size_t)gj * D + gi;
-          const double2 f = F[idx];
-          const double2 g = G[idx];
+          const double2 f = Fm[idx];
+          const double2 g = Gm[idx];
           O[idx] = make_double2(cf * acc_re[i][j][e] + a.omf * (f.x - g.x),
                                 cf * acc_im[i][j][e] + a.omf * (f.y - g.y));
         }
@@ -196,54 +249,35 @@ __global__ __launch_bounds__(256) void bse_ladder_kernel(BseArgs a) {
     }
 }
 
-// P = alpha*Gamma*chi, Q = alpha*chi*Gamma for Nl > 1 (block-diagonal chi).
-__global__ __launch_bounds__(256) void bse_mix_kernel(BseArgs a) {
-  const int D = a.D, nf = a.nf, nl = a.nl;
-  const size_t DD = (size_t)D * D;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= DD) return;
-  const int mat = blockIdx.y;
-  const int r = mat / a.nqb;
-  const int qb = mat - r * a.nqb;
-  const int q = a.q0 + qb;
-  const double2* G = a.gam[r] + DD * q;
-  const double2* chiq = a.chi[r >> 1] + (size_t)nf * nl * nl * q;
-  double2* P = const_cast<double2*>(a.ws) + DD * ((size_t)(r * a.nqb + qb) * 2);
-  double2* Q = P + DD;
-  const int i = (int)(e % D);
-  const int c = (int)(e / D);
-  // P(i, c) with c = (l2, nu): sum_l3 Gamma(i, (l3, nu)) chi_nu(l3, l2)
-  {
-    const int l2 = c / nf, nu = c - l2 * nf;
-    const double2* cb = chiq + (size_t)nu * nl * nl;
-    double sr = 0.0, si = 0.0;
-    for (int l3 = 0; l3 < nl; ++l3) {
-      const double2 g = G[(size_t)(l3 * nf + nu) * D + i];
-      const double2 x = cb[l3 + nl * l2];
-      sr += g.x * x.x - g.y * x.y;
-      si += g.x * x.y + g.y * x.x;
-    }
-    P[e] = make_double2(a.alpha * sr, a.alpha * si);
-  }
-  // Q(k, c) with k = i = (l1, nu): sum_l3 chi_nu(l1, l3) Gamma((l3, nu), c)
-  {
-    const int l1 = i / nf, nu = i - l1 * nf;
-    const double2* cb = chiq + (size_t)nu * nl * nl;
-    double sr = 0.0, si = 0.0;
-    for (int l3 = 0; l3 < nl; ++l3) {
-      const double2 x = cb[l1 + nl * l3];
-      const double2 g = G[(size_t)c * D + l3 * nf + nu];
-      sr += x.x * g.x - x.y * g.y;
-      si += x.x * g.y + x.y * g.x;
-    }
-    Q[e] = make_double2(a.alpha * sr, a.alpha * si);
-  }
+typedef void (*BseKernel)(BseArgs);
+
+struct BseVariant {
+  int nl;
+  BseKernel fn;
+  const char* name;
+};
+
+// tile variants per form-factor count: frequency chunk G and LDS double buffering
+static const BseVariant kBseVariants[] = {
+    {1, bse_ladder_kernel<1, 16, false>, "nl1_g16"},
+    {1, bse_ladder_kernel<1, 16, true>, "nl1_g16_db"},
+    {1, bse_ladder_kernel<1, 32, false>, "nl1_g32"},
+    {5, bse_ladder_kernel<5, 4, false>, "nl5_g4"},
+    {5, bse_ladder_kernel<5, 4, true>, "nl5_g4_db"},
+    {5, bse_ladder_kernel<5, 8, false>, "nl5_g8"},
+    {9, bse_ladder_kernel<9, 4, false>, "nl9_g4"},
+    {9, bse_ladder_kernel<9, 4, true>, "nl9_g4_db"},
+};
+constexpr int kNumBseVariants = sizeof(kBseVariants) / sizeof(kBseVariants[0]);
+
+static int default_variant(int nl) {
+  for (int v = 0; v < kNumBseVariants; ++v)
+    if (kBseVariants[v].nl == nl) return v;
+  return -1;
 }
 
-int launch_bse_ladder(tups_ctx* c) {
+static BseArgs bse_args(tups_ctx* c) {
   const tups_sizes& s = c->sz;
-  const int D = s.nz;
-  const size_t DD = (size_t)D * D;
   BseArgs a{};
   for (int r = 0; r < 4; ++r) {
     a.gam[r] = c->gamma[r];
@@ -252,39 +286,71 @@ int launch_bse_ladder(tups_ctx* c) {
   }
   a.chi[0] = c->chi[0];
   a.chi[1] = c->chi[1];
-  a.D = D;
+  a.D = s.nz;
   a.nf = s.nf;
-  a.nl = s.nl;
-  a.tiles_n = (D + BM - 1) / BM;
+  a.nq = s.nq;
+  a.tiles_n = (s.nz + BM - 1) / BM;
   a.alpha = s.alpha;
   const double cr[4] = {0.5, 0.5, -0.25, 0.25};
   for (int r = 0; r < 4; ++r) a.cf[r] = cr[r] * c->f_damping;
   a.omf = 1.0 - c->f_damping;
-  const int tiles = a.tiles_n * a.tiles_n;
-  const bool scaled = s.nl == 1;
-  const int qbatch = scaled ? s.nq : c->bse_qbatch;
-  for (int q0 = 0; q0 < s.nq; q0 += qbatch) {
-    a.q0 = q0;
-    a.nqb = std::min(qbatch, s.nq - q0);
-    a.ws = c->work;
-    if (!scaled) {
-      dim3 g((unsigned)((DD + 255) / 256), (unsigned)(4 * a.nqb));
-      if (int rc = timer_begin(c)) return rc;
-      hipLaunchKernelGGL(bse_mix_kernel, g, dim3(256), 0, c->stream, a);
-      TUPS_HIP(hipGetLastError());
-      if (int rc = timer_end(c, TUPS_K_BSE_MIX)) return rc;
-    }
-    const long long nblk = (long long)tiles * 4 * a.nqb;
-    TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "BSE grid too large");
-    if (int rc = timer_begin(c)) return rc;
-    if (scaled)
-      hipLaunchKernelGGL(bse_ladder_kernel<true>, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
-    else
-      hipLaunchKernelGGL(bse_ladder_kernel<false>, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
-    TUPS_HIP(hipGetLastError());
-    if (int rc = timer_end(c, TUPS_K_BSE_LADDER)) return rc;
-  }
+  return a;
+}
+
+static int launch_variant(tups_ctx* c, int v, const BseArgs& a) {
+  const long long nblk = (long long)a.tiles_n * a.tiles_n * 4 * a.nq;
+  TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "BSE grid too large");
+  hipLaunchKernelGGL(kBseVariants[v].fn, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+  TUPS_HIP(hipGetLastError());
   return TUPS_OK;
+}
+
+int launch_bse_ladder(tups_ctx* c) {
+  int v = c->bse_variant;
+  if (v < 0 || v >= kNumBseVariants || kBseVariants[v].nl != c->sz.nl) v = default_variant(c->sz.nl);
+  TUPS_CHECK(v >= 0, TUPS_EINVAL, "no BSE kernel for this Nl");
+  const BseArgs a = bse_args(c);
+  if (int rc = timer_begin(c)) return rc;
+  if (int rc = launch_variant(c, v, a)) return rc;
+  if (int rc = timer_end(c, TUPS_K_BSE_LADDER)) return rc;
+  return TUPS_OK;
+}
+
+// Times every variant for this context's shape on its resident inputs (the
+// outputs go to the ping-pong buffer and are discarded) and keeps the fastest.
+int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms) {
+  const BseArgs a = bse_args(c);
+  float best = 0.f;
+  int arg = -1;
+  for (int v = 0; v < kNumBseVariants; ++v) {
+    if (kBseVariants[v].nl != c->sz.nl) continue;
+    if (int rc = launch_variant(c, v, a)) return rc;  // warm-up
+    TUPS_HIP(hipEventRecord(c->tev[0], c->stream));
+    for (int i = 0; i < reps; ++i)
+      if (int rc = launch_variant(c, v, a)) return rc;
+    TUPS_HIP(hipEventRecord(c->tev[1], c->stream));
+    TUPS_HIP(hipEventSynchronize(c->tev[1]));
+    float ms = 0.f;
+    TUPS_HIP(hipEventElapsedTime(&ms, c->tev[0], c->tev[1]));
+    ms /= reps;
+    if (arg < 0 || ms < best) {
+      best = ms;
+      arg = v;
+    }
+  }
+  TUPS_CHECK(arg >= 0, TUPS_EINVAL, "no BSE kernel for this Nl");
+  c->bse_variant = arg;
+  if (chosen) *chosen = arg;
+  if (best_ms) *best_ms = best;
+  return TUPS_OK;
+}
+
+int bse_variant_count() { return kNumBseVariants; }
+
+int bse_variant_nl(int v) { return v >= 0 && v < kNumBseVariants ? kBseVariants[v].nl : -1; }
+
+const char* bse_variant_name(int v) {
+  return v >= 0 && v < kNumBseVariants ? kBseVariants[v].name : "";
 }
 
 }  // namespace tups

@@ -244,14 +244,6 @@ int tups_create(const tups_dims* d, tups_ctx** out) {
     if (!rc && s.full_ibz) rc = dalloc(&c->mten[r], array_elems(c, TUPS_M));
   }
   for (int p = 0; p < 2 && rc == 0; ++p) rc = dalloc(&c->chi[p], array_elems(c, TUPS_CHI));
-  if (!rc && s.nl > 1) {
-    // P, Q operands of the BSE for a batch of q (bounded to ~32 GiB)
-    const size_t per_q = (size_t)2 * 4 * s.nz * s.nz;
-    const size_t cap = ((size_t)32 << 30) / sizeof(double2);
-    c->bse_qbatch = (int)std::max<size_t>(1, std::min<size_t>(s.nq, cap / per_q));
-    c->work_elems = per_q * c->bse_qbatch;
-    rc = dalloc(&c->work, c->work_elems);
-  }
   if (!rc) {
     he = hipMalloc((void**)&c->d_flag, sizeof(int));
     if (he != hipSuccess) rc = TUPS_ENOMEM;
@@ -303,7 +295,6 @@ int tups_destroy(tups_ctx* c) {
   f(c->d_ph_src);
   f(c->d_ph_coef);
   f(c->d_flag);
-  f(c->work);
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   for (int i = 0; i < 2; ++i)
@@ -468,6 +459,35 @@ int tups_gamma_update(tups_ctx* c) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
   return launch_gamma_update(c);
 }
+
+int tups_autotune(tups_ctx* c, int32_t kernel, int32_t* variant, float* ms) {
+  TUPS_CHECK(c, TUPS_EINVAL, "null context");
+  TUPS_CHECK(kernel == TUPS_K_BSE_LADDER, TUPS_EINVAL, "only the BSE ladder has tile variants");
+  int rc = ensure_tables(c);
+  if (rc) return rc;
+  int v = -1;
+  float t = 0.f;
+  rc = autotune_bse(c, 2, &v, &t);
+  if (rc) return rc;
+  if (variant) *variant = v;
+  if (ms) *ms = t;
+  return TUPS_OK;
+}
+
+const char* tups_variant_name(int32_t kernel, int32_t variant) {
+  return kernel == TUPS_K_BSE_LADDER ? bse_variant_name(variant) : "";
+}
+
+int tups_set_variant(tups_ctx* c, int32_t kernel, int32_t variant) {
+  TUPS_CHECK(c, TUPS_EINVAL, "null context");
+  TUPS_CHECK(kernel == TUPS_K_BSE_LADDER, TUPS_EINVAL, "only the BSE ladder has tile variants");
+  TUPS_CHECK(variant == -1 || bse_variant_nl(variant) == c->sz.nl, TUPS_EINVAL,
+             "variant does not exist for this Nl");
+  c->bse_variant = variant;
+  return TUPS_OK;
+}
+
+int tups_variant_count(int32_t kernel) { return kernel == TUPS_K_BSE_LADDER ? bse_variant_count() : 0; }
 
 int tups_sync(tups_ctx* c) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");

@@ -94,10 +94,8 @@ struct tups_ctx {
   double2* d_ph_coef = nullptr;
   int* d_flag = nullptr;
 
-  double2* work = nullptr;      // BSE operand workspace (Nl > 1)
-  size_t work_elems = 0;
-  int bse_qbatch = 0;
 
+  int bse_variant = -1;          // kernel variant chosen by the autotuner (-1: default)
   bool timing = false;
   hipEvent_t tev[2] = {};
   double t_ms[TUPS_K_COUNT] = {};
@@ -111,6 +109,10 @@ struct tups_ctx {
 namespace tups {
 // kernels launchers (defined in the .hip sources)
 int launch_bse_ladder(tups_ctx* c);
+int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms);
+const char* bse_variant_name(int v);
+int bse_variant_count();
+int bse_variant_nl(int v);
 int launch_phir(tups_ctx* c);
 int launch_build_m(tups_ctx* c);
 int launch_pae(tups_ctx* c);

@@ -1,0 +1,275 @@
+!> ISO_C_BINDING interface to libtups_gpu (include/tups_gpu.h).
+!>
+!> Every C entry point is declared here with the reference's Fortran types:
+!> complex(8) arrays pass as complex(c_double_complex) assumed-size dummies
+!> (the reference's Fortran column-major layouts), integers are 32-bit like the
+!> reference's default integer (makefile:17, MKL lp64).  tups_check mirrors the
+!> reference's error convention: print the message and stop
+!> (SRC/MPI_mod.f90:13-16, SRC/preprocessing.f90:95-98).
+module tups_gpu_iface
+  use, intrinsic :: iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: TUPS_OK = 0, TUPS_EINVAL = -1, TUPS_EHIP = -2, &
+                               TUPS_ENOMEM = -3, TUPS_ESTATE = -4, TUPS_ENONFINITE = -5
+  integer(c_int32_t), parameter :: TUPS_CH_D = 0, TUPS_CH_M = 1, TUPS_CH_S = 2, TUPS_CH_T = 3
+  integer(c_int32_t), parameter :: TUPS_ARR_GAMMA = 0, TUPS_ARR_F = 1, TUPS_ARR_CHI = 2, TUPS_ARR_PHIR = 3, &
+                                   TUPS_ARR_M = 4, TUPS_ARR_LAMBDA = 5
+  integer(c_int32_t), parameter :: TUPS_BSE_LADDER_FP64 = 0
+  integer(c_int32_t), parameter :: TUPS_K_BSE_LADDER = 0, TUPS_K_PHIR = 1, TUPS_K_BUILD_M = 2, &
+                                   TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4
+
+  type, bind(C) :: tups_dims
+    integer(c_int32_t) :: nx, nf, nl, nr, nq, reserved
+    real(c_double) :: beta, f_damping
+  end type tups_dims
+
+  type, bind(C) :: tups_sizes
+    integer(c_int32_t) :: nx, ny, nf, nl, nz, nr, nw, nibz, nq, nc, full_ibz, reserved
+    real(c_double) :: alpha
+  end type tups_sizes
+
+  interface
+    function tups_last_error() bind(C, name="tups_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function tups_last_error
+
+    function tups_build_tables(dims, ff, ff_inv, signs, reps, exp_sqr, exp_qsr) &
+        bind(C, name="tups_build_tables") result(rc)
+      import :: tups_dims, c_double, c_int32_t, c_double_complex, c_int
+      type(tups_dims), intent(in) :: dims
+      real(c_double), intent(out) :: ff(*), ff_inv(*)
+      integer(c_int32_t), intent(out) :: signs(*), reps(*)
+      complex(c_double_complex), intent(out) :: exp_sqr(*), exp_qsr(*)
+      integer(c_int) :: rc
+    end function tups_build_tables
+
+    function tups_create(dims, ctx) bind(C, name="tups_create") result(rc)
+      import :: tups_dims, c_ptr, c_int
+      type(tups_dims), intent(in) :: dims
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: rc
+    end function tups_create
+
+    function tups_destroy(ctx) bind(C, name="tups_destroy") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function tups_destroy
+
+    function tups_query(ctx, sizes) bind(C, name="tups_query") result(rc)
+      import :: c_ptr, c_int, tups_sizes
+      type(c_ptr), value :: ctx
+      type(tups_sizes), intent(out) :: sizes
+      integer(c_int) :: rc
+    end function tups_query
+
+    function tups_get_tables(ctx, ff, ff_inv, signs, reps, exp_sqr, exp_qsr) &
+        bind(C, name="tups_get_tables") result(rc)
+      import :: c_ptr, c_double, c_int32_t, c_double_complex, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ff(*), ff_inv(*)
+      integer(c_int32_t), intent(out) :: signs(*), reps(*)
+      complex(c_double_complex), intent(out) :: exp_sqr(*), exp_qsr(*)
+      integer(c_int) :: rc
+    end function tups_get_tables
+
+    function tups_set_tables(ctx, ff, ff_inv, signs, reps, exp_sqr, exp_qsr) &
+        bind(C, name="tups_set_tables") result(rc)
+      import :: c_ptr, c_double, c_int32_t, c_double_complex, c_int
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ff(*), ff_inv(*)
+      integer(c_int32_t), intent(in) :: signs(*), reps(*)
+      complex(c_double_complex), intent(in) :: exp_sqr(*), exp_qsr(*)
+      integer(c_int) :: rc
+    end function tups_set_tables
+
+    function tups_get_q_map(ctx, qx, qy, qw) bind(C, name="tups_get_q_map") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: qx(*), qy(*), qw(*)
+      integer(c_int) :: rc
+    end function tups_get_q_map
+
+    function tups_set_q_map(ctx, qx, qy, qw) bind(C, name="tups_set_q_map") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: qx(*), qy(*), qw(*)
+      integer(c_int) :: rc
+    end function tups_set_q_map
+
+    function tups_upload(ctx, which, channel, host) bind(C, name="tups_upload") result(rc)
+      import :: c_ptr, c_int32_t, c_double_complex, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: which, channel
+      complex(c_double_complex), intent(in) :: host(*)
+      integer(c_int) :: rc
+    end function tups_upload
+
+    function tups_download(ctx, which, channel, host) bind(C, name="tups_download") result(rc)
+      import :: c_ptr, c_int32_t, c_double_complex, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: which, channel
+      complex(c_double_complex), intent(out) :: host(*)
+      integer(c_int) :: rc
+    end function tups_download
+
+    function tups_upload_chi_dense(ctx, pp, q0, nqs, dense) bind(C, name="tups_upload_chi_dense") result(rc)
+      import :: c_ptr, c_int32_t, c_double_complex, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: pp, q0, nqs
+      complex(c_double_complex), intent(in) :: dense(*)
+      integer(c_int) :: rc
+    end function tups_upload_chi_dense
+
+    function tups_bse(ctx, mode) bind(C, name="tups_bse") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode
+      integer(c_int) :: rc
+    end function tups_bse
+
+    function tups_phir(ctx) bind(C, name="tups_phir") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function tups_phir
+
+    function tups_build_m(ctx) bind(C, name="tups_build_m") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function tups_build_m
+
+    function tups_pae(ctx) bind(C, name="tups_pae") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function tups_pae
+
+    function tups_gamma_update(ctx) bind(C, name="tups_gamma_update") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function tups_gamma_update
+
+    function tups_autotune(ctx, kernel, variant, ms) bind(C, name="tups_autotune") result(rc)
+      import :: c_ptr, c_int32_t, c_float, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kernel
+      integer(c_int32_t), intent(out) :: variant
+      real(c_float), intent(out) :: ms
+      integer(c_int) :: rc
+    end function tups_autotune
+
+    function tups_variant_name(kernel, variant) bind(C, name="tups_variant_name") result(p)
+      import :: c_ptr, c_int32_t
+      integer(c_int32_t), value :: kernel, variant
+      type(c_ptr) :: p
+    end function tups_variant_name
+
+    function tups_set_variant(ctx, kernel, variant) bind(C, name="tups_set_variant") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kernel, variant
+      integer(c_int) :: rc
+    end function tups_set_variant
+
+    function tups_variant_count(kernel) bind(C, name="tups_variant_count") result(n)
+      import :: c_int32_t, c_int
+      integer(c_int32_t), value :: kernel
+      integer(c_int) :: n
+    end function tups_variant_count
+
+    function tups_sync(ctx) bind(C, name="tups_sync") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function tups_sync
+
+    function tups_event_record(ctx, slot) bind(C, name="tups_event_record") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: slot
+      integer(c_int) :: rc
+    end function tups_event_record
+
+    function tups_event_elapsed(ctx, a, b, ms) bind(C, name="tups_event_elapsed") result(rc)
+      import :: c_ptr, c_int32_t, c_float, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: a, b
+      real(c_float), intent(out) :: ms
+      integer(c_int) :: rc
+    end function tups_event_elapsed
+
+    function tups_set_timing(ctx, on) bind(C, name="tups_set_timing") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+      integer(c_int) :: rc
+    end function tups_set_timing
+
+    function tups_get_timing(ctx, kernel, total_ms, launches) bind(C, name="tups_get_timing") result(rc)
+      import :: c_ptr, c_int32_t, c_double, c_int64_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kernel
+      real(c_double), intent(out) :: total_ms
+      integer(c_int64_t), intent(out) :: launches
+      integer(c_int) :: rc
+    end function tups_get_timing
+
+    function tups_check_finite(ctx, which, channel) bind(C, name="tups_check_finite") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: which, channel
+      integer(c_int) :: rc
+    end function tups_check_finite
+
+    function tups_synth_vertex(ctx, which, channel, seed, diag, sigma) &
+        bind(C, name="tups_synth_vertex") result(rc)
+      import :: c_ptr, c_int32_t, c_int64_t, c_double, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: which, channel
+      integer(c_int64_t), value :: seed
+      real(c_double), value :: diag, sigma
+      integer(c_int) :: rc
+    end function tups_synth_vertex
+
+    function tups_copy(ctx, which_dst, which_src, channel) bind(C, name="tups_copy") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: which_dst, which_src, channel
+      integer(c_int) :: rc
+    end function tups_copy
+
+    function c_strlen(s) bind(C, name="strlen") result(n)
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+      integer(c_size_t) :: n
+    end function c_strlen
+  end interface
+
+contains
+
+  !> Stop with the library's message on a nonzero status.
+  subroutine tups_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    type(c_ptr) :: p
+    integer :: n, i
+    character(len=512) :: text
+    if (rc == TUPS_OK) return
+    p = tups_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, msg, [n])
+    text = ''
+    do i = 1, min(n, len(text))
+      text(i:i) = msg(i)
+    end do
+    write(*, *) 'ERROR - ', trim(where), ' failed (', rc, '): ', trim(text)
+    stop 1
+  end subroutine tups_check
+
+end module tups_gpu_iface

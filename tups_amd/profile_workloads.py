@@ -1,0 +1,104 @@
+"""Named workloads for rocprofv3 runs of the hot-path kernels.
+
+    python -m tups_amd.profile_workloads bse      --steps 3   # batched BSE, D=320, Nq=1024
+    python -m tups_amd.profile_workloads crossing --steps 5   # PAE crossing, Nl=1, Nx=16, Nf=64
+    python -m tups_amd.profile_workloads phir     --steps 5   # PhiR, Nl=1, Nx=16, Nf=64
+    python -m tups_amd.profile_workloads calib    --steps 3   # G = F - G streaming kernel
+
+`calib` runs a streaming kernel whose bytes are known exactly (per element:
+read 2 x 16 B, write 16 B) so hardware counters (FETCH_SIZE, WRITE_SIZE) can be
+scaled to bytes.  Each workload prints its step count and per-launch
+algorithmic bytes / flops.
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+
+import numpy as np
+
+from .runtime import GAMMA, LAMBDA, TupsContext
+from .synth import batched_bse_inputs
+
+
+def crossing_context(nx=16, nf=64, nl=1, seed=0):
+    """Full-IBZ context with random Phi (device generator), PhiR and M built on the device."""
+    c = TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5)
+    rng = np.random.default_rng(seed)
+    for r in range(4):
+        c.synth_vertex(GAMMA, r, seed + r, 0.0, 1.0)
+        lam = rng.standard_normal((nf, nf, nf // 2)) + 1j * rng.standard_normal((nf, nf, nf // 2))
+        c.upload(LAMBDA, r, lam)
+    c.calc_PhiR(0)
+    c.calc_M()
+    c.sync()
+    return c
+
+
+def crossing_bytes(c: TupsContext) -> float:
+    """Minimum HBM bytes of one calculate_PAE launch: Phi read + F write (4 channels),
+    PhiR, M and Lambda read once."""
+    nz2q = c.nz * c.nz * c.nq
+    phir = c.nz * c.nz * c.nw * c.nr
+    m = c.nl ** 4 * c.nibz * c.nr
+    lam = c.nf * c.nf * c.nw
+    return 16.0 * 4 * (2 * nz2q + phir + m + lam)
+
+
+def crossing_flops(c: TupsContext) -> float:
+    """Complex MACs of one launch: every (q, row, col) entry of both target pairs
+    takes up to two terms of Nl^2 NR products, each feeding two targets."""
+    nf, nw = c.nf, c.nw
+    terms = 0
+    for qw in range(1, nw + 1):
+        n1 = np.add.outer(np.arange(1, nf + 1), np.zeros(nf, int))
+        n2 = n1.T
+        d = n2 - n1
+        ph = ((d >= 0) & (d + 1 <= nw)) | ((d < 0) & (1 - d <= nw))
+        w = n1 + n2 + qw - nf - 1
+        wt = nf + 3 - qw - n1 - n2
+        pp = ((w >= 1) & (w <= nw)) | ((wt >= 2) & (wt <= nw))
+        w = qw - n1 - n2 + nf + 1
+        wt = n1 + n2 - qw - nf + 1
+        php = ((w >= 1) & (w <= nw)) | ((wt >= 2) & (wt <= nw))
+        terms += int(ph.sum() + pp.sum() + php.sum() + ph.sum())
+    per_term = c.nl ** 2 * c.nr * 2 * 8  # two targets, complex MAC = 8 flops
+    return float(terms) * c.nibz * c.nl ** 2 * per_term
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload", choices=["bse", "crossing", "phir", "calib"])
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=16)
+    ap.add_argument("--nf", type=int, default=64)
+    ap.add_argument("--nl", type=int, default=1)
+    args = ap.parse_args(argv)
+    if args.workload in ("bse", "calib"):
+        c = TupsContext(nx=8, nf=64, nl=5, nq=1024, beta=2.0, f_damping=0.5)
+        batched_bse_inputs(c)
+        if args.workload == "bse":
+            for _ in range(args.steps):
+                c.BSE(0)
+            print(f"bse: {args.steps} launches, flops/launch {16.0 * c.nz ** 3 * 4 * c.nq:.4e}")
+        else:
+            for _ in range(args.steps):
+                c.gamma_update()
+            n = c.nz * c.nz * c.nq * 4
+            print(f"calib: {args.steps} launches, fetch {32.0 * n:.6e} B, write {16.0 * n:.6e} B per launch")
+    else:
+        c = crossing_context(args.nx, args.nf, args.nl)
+        for _ in range(args.steps):
+            if args.workload == "crossing":
+                c.calculate_PAE()
+            else:
+                c.calc_PhiR(0)
+        print(f"{args.workload}: {args.steps} launches, Nq={c.nq} Nz={c.nz}, "
+              f"crossing bytes/launch {crossing_bytes(c):.6e}")
+    c.sync()
+    c.close()
+    sys.stdout.flush()
+
+
+if __name__ == "__main__":
+    main()

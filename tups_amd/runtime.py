@@ -20,7 +20,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
 
 GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
-K_BSE_MIX, K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE = range(6)
+K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE = range(5)
 CH_D, CH_M, CH_S, CH_T = range(4)
 BSE_LADDER_FP64 = 0
 M_NAMES = ("phbar_to_ph", "pp_to_ph", "ph_to_pp", "phbar_to_pp")
@@ -68,6 +68,10 @@ SIGNATURES = {
     "tups_build_m": ([_V], C.c_int),
     "tups_pae": ([_V], C.c_int),
     "tups_gamma_update": ([_V], C.c_int),
+    "tups_autotune": ([_V, C.c_int32, _I, _P(C.c_float)], C.c_int),
+    "tups_variant_name": ([C.c_int32, C.c_int32], C.c_char_p),
+    "tups_set_variant": ([_V, C.c_int32, C.c_int32], C.c_int),
+    "tups_variant_count": ([C.c_int32], C.c_int),
     "tups_sync": ([_V], C.c_int),
     "tups_event_record": ([_V, C.c_int32], C.c_int),
     "tups_event_elapsed": ([_V, C.c_int32, C.c_int32, _P(C.c_float)], C.c_int),
@@ -230,6 +234,25 @@ class TupsContext:
     def gamma_update(self):
         """G_r_LL = F_r_LL - G_r_LL, SRC/loop.f90:219-222."""
         _check(self.lib.tups_gamma_update(self._h), "tups_gamma_update")
+
+    def autotune(self, kernel: int = K_BSE_LADDER):
+        """Pick the fastest tile variant on the resident inputs; returns (name, ms per launch)."""
+        v = C.c_int32()
+        ms = C.c_float()
+        _check(self.lib.tups_autotune(self._h, kernel, C.byref(v), C.byref(ms)), "tups_autotune")
+        return self.lib.tups_variant_name(kernel, v.value).decode(), float(ms.value)
+
+    def variants(self, kernel: int = K_BSE_LADDER):
+        """Variant ids (and names) of `kernel` usable with this context's Nl."""
+        out = []
+        for v in range(self.lib.tups_variant_count(kernel)):
+            name = self.lib.tups_variant_name(kernel, v).decode()
+            if name.startswith(f"nl{self.nl}_"):
+                out.append((v, name))
+        return out
+
+    def set_variant(self, variant: int, kernel: int = K_BSE_LADDER):
+        _check(self.lib.tups_set_variant(self._h, kernel, variant), "tups_set_variant")
 
     # --- stream / timing ----------------------------------------------------------------
     def sync(self):
