@@ -160,6 +160,7 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 #define TUPS_K_BUILD_M 2
 #define TUPS_K_PAE 3
 #define TUPS_K_GAMMA_UPDATE 4
+#define TUPS_K_PAE_GATHER 5   /* Nl = 1: PhiR re-indexed for contiguous crossing reads */
 #define TUPS_K_COUNT 8
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);

@@ -18,13 +18,18 @@
 //
 // The chi mixing is fused: the inner dimension is walked in chunks of G
 // fermionic frequencies times all Nl form factors, so a loader thread that owns
-// one (row, nu) or (nu, column) pair holds all Nl values that the Nl x Nl block
-// chi_nu couples and applies it in registers before the tile goes to LDS:
-//   part 0: A = Gamma(:, S),  B = alpha chi F (S, :)   (rows of F mixed)
-//   part 1: A = F alpha chi (:, S),  B = Gamma(S, :)   (columns of F mixed)
-// Operand tiles are stored as split Re/Im planes so each MFMA operand is one
-// double per lane.  Global loads of chunk c+1 are issued before the MFMAs of
-// chunk c (register prefetch), and the damped mixing is fused into the epilogue.
+// one (row, nu) pair of the A operand holds all Nl values that the Nl x Nl
+// block chi_nu couples and applies it in registers before the tile goes to LDS:
+//   part 0: A = Gamma alpha chi (:, S),  B = F(S, :)
+//   part 1: A = F alpha chi (:, S),      B = Gamma(S, :)
+// ((Gamma chi) F = Gamma (chi F) and F (chi Gamma) = (F chi) Gamma.)  The
+// frequency of an A pair is uniform across a wave, so chi_nu comes through the
+// scalar cache.  On gfx950 no VALU instruction issues while an F64 MFMA runs on
+// the SIMD, so the loop keeps VALU work minimal: -Im*Im uses the MFMA negate
+// modifier.  Operand tiles are stored as split Re/Im planes so each MFMA
+// operand is one double per lane.  Global loads of chunk c+1 are issued before
+// the MFMAs of chunk c (register prefetch), and the damped mixing is fused
+// into the epilogue.
 #include "internal.h"
 
 namespace tups {
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
     }
   };
 
-  // chi mixing in registers, then store chunk `it` to LDS (k index = l*G + g)
+  // chi mixing of the A pairs in registers, then store chunk `it` to LDS (k index = l*G + g)
   auto store = [&](int it) {
     const int part = it >= nch ? 1 : 0;
     const int sb = DB ? (it & 1) : 0;
@@ -134,47 +139,33 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
 #pragma unroll
     for (int p = 0; p < PP; ++p) {
       const int u = tid + 256 * p;
-      const int i = u & 63, g = u >> 6;
+      const int i = u & 63;
+      // g = u >> 6 is the wave index (+ 4p): uniform, so chi_nu is a scalar load
+      const int g = __builtin_amdgcn_readfirstlane(u >> 6);
       const int nu = nu0 + g < nf ? nu0 + g : nf - 1;
       const double2* cb = chiq + (size_t)nu * NL * NL;
 #pragma unroll
       for (int l2 = 0; l2 < NL; ++l2) {
-        double2 v = ra[p][l2];
-        if (part) {  // (F alpha chi)(i, (l2, nu)) = alpha sum_l3 F(i, (l3, nu)) chi_nu(l3, l2)
-          double sr = 0.0, si = 0.0;
+        // (X alpha chi)(i, (l2, nu)) = alpha sum_l3 X(i, (l3, nu)) chi_nu(l3, l2)
+        double sr = 0.0, si = 0.0;
 #pragma unroll
-          for (int l3 = 0; l3 < NL; ++l3) {
-            const double2 x = cb[l3 + NL * l2];
-            sr += ra[p][l3].x * x.x - ra[p][l3].y * x.y;
-            si += ra[p][l3].x * x.y + ra[p][l3].y * x.x;
-          }
-          v = make_double2(alpha * sr, alpha * si);
+        for (int l3 = 0; l3 < NL; ++l3) {
+          const double2 x = cb[l3 + NL * l2];
+          sr += ra[p][l3].x * x.x - ra[p][l3].y * x.y;
+          si += ra[p][l3].x * x.y + ra[p][l3].y * x.x;
         }
-        As[sb][0][l2 * G + g][i] = v.x;
-        As[sb][1][l2 * G + g][i] = v.y;
+        As[sb][0][l2 * G + g][i] = alpha * sr;
+        As[sb][1][l2 * G + g][i] = alpha * si;
       }
     }
 #pragma unroll
     for (int p = 0; p < PP; ++p) {
       const int u = tid + 256 * p;
       const int g = u % G, j = u / G;
-      const int nu = nu0 + g < nf ? nu0 + g : nf - 1;
-      const double2* cb = chiq + (size_t)nu * NL * NL;
 #pragma unroll
       for (int l1 = 0; l1 < NL; ++l1) {
-        double2 v = rb[p][l1];
-        if (!part) {  // (alpha chi F)((l1, nu), j) = alpha sum_l3 chi_nu(l1, l3) F((l3, nu), j)
-          double sr = 0.0, si = 0.0;
-#pragma unroll
-          for (int l3 = 0; l3 < NL; ++l3) {
-            const double2 x = cb[l1 + NL * l3];
-            sr += x.x * rb[p][l3].x - x.y * rb[p][l3].y;
-            si += x.x * rb[p][l3].y + x.y * rb[p][l3].x;
-          }
-          v = make_double2(alpha * sr, alpha * si);
-        }
-        Bs[sb][0][l1 * G + g][j] = v.x;
-        Bs[sb][1][l1 * G + g][j] = v.y;
+        Bs[sb][0][l1 * G + g][j] = rb[p][l1].x;
+        Bs[sb][1][l1 * G + g][j] = rb[p][l1].y;
       }
     }
   };
@@ -183,12 +174,11 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
 #pragma unroll
     for (int ks = 0; ks < KC / 4; ++ks) {
       const int kr = ks * 4 + lk;
-      double ar[2], ai[2], nai[2], br[2], bi[2];
+      double ar[2], ai[2], br[2], bi[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         ar[i] = As[buf][0][kr][wr + i * 16 + li];
         ai[i] = As[buf][1][kr][wr + i * 16 + li];
-        nai[i] = -ai[i];
         br[i] = Bs[buf][0][kr][wc + i * 16 + li];
         bi[i] = Bs[buf][1][kr][wc + i * 16 + li];
       }
@@ -197,7 +187,8 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc_re[i][j], 0, 0, 0);
-          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], acc_re[i][j], 0, 0, 0);
+          // blgp bit 0 negates the A operand of an F64 MFMA: acc -= Im(A) Im(B)
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acc_re[i][j], 0, 0, 1);
           acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acc_im[i][j], 0, 0, 0);
           acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acc_im[i][j], 0, 0, 0);
         }

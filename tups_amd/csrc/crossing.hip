@@ -57,137 +57,281 @@ __device__ __forceinline__ void st_stream(double2* p, double2 v) {
   __builtin_nontemporal_store(w, reinterpret_cast<dv2*>(p));
 }
 
-// acc_a += sum M * (ca0*X0 + ca1*X1); acc_b += sum M * (cb0*X0 + cb1*X1)
+// One crossing term, branch-free: `wt` is 0 for an entry the term does not
+// reach (its indices are then any in-range value), `tr` selects the
+// time-reversed form conj(x) * signs(l3) signs(l4).
+struct Term {
+  int w2, nua, nub;
+  double wt;
+  bool tr;
+};
+
+__device__ __forceinline__ Term make_term(bool plain_ok, bool tr_ok, int w2p, int nap, int nbp, int w2t,
+                                          int nat, int nbt) {
+  Term t;
+  t.tr = !plain_ok && tr_ok;
+  t.wt = (plain_ok || tr_ok) ? 1.0 : 0.0;
+  t.w2 = plain_ok ? w2p : (tr_ok ? w2t : 1);
+  t.nua = plain_ok ? nap : (tr_ok ? nat : 1);
+  t.nub = plain_ok ? nbp : (tr_ok ? nbt : 1);
+  return t;
+}
+
+// acc_a += wt sum M * (ca0*X0 + ca1*X1); acc_b += wt sum M * (cb0*X0 + cb1*X1)
 template <int NL>
 __device__ __forceinline__ void crossing_term(const PaeArgs& a, const double2* M, int s0, int s1,
                                               double ca0, double ca1, double cb0, double cb1,
-                                              int l1, int l2, int m, int w2, int nua, int nub,
-                                              bool tr, double2& acc_a, double2& acc_b) {
+                                              int l1, int l2, int m, const Term& t,
+                                              double2& acc_a, double2& acc_b) {
   const int nf = a.nf, nz = nf * NL;
   const size_t nz2 = (size_t)nz * nz;
   const double2* p0 = a.phir[s0];
   const double2* p1 = a.phir[s1];
   constexpr int NL4 = NL * NL * NL * NL;
+  const double im_sign = t.tr ? -1.0 : 1.0;
   for (int R = 0; R < a.nr; ++R) {
-    const size_t pbase = nz2 * ((size_t)(w2 - 1) + (size_t)a.nw * R);
+    const size_t pbase = nz2 * ((size_t)(t.w2 - 1) + (size_t)a.nw * R);
     const double2* mb = M + (size_t)NL4 * ((size_t)m + (size_t)a.nibz * R) + l1 + NL * l2;
-#pragma unroll
+    double2 ya_sum = make_double2(0.0, 0.0), yb_sum = make_double2(0.0, 0.0);
+#pragma unroll 1
     for (int l4 = 0; l4 < NL; ++l4) {
-      const size_t colo = pbase + (size_t)nz * (l4 * nf + nub - 1) + (nua - 1);
+      const size_t colo = pbase + (size_t)nz * (l4 * nf + t.nub - 1) + (t.nua - 1);
 #pragma unroll
       for (int l3 = 0; l3 < NL; ++l3) {
         const size_t pi = colo + (size_t)(l3 * nf);
-        double2 x0 = p0[pi];
-        double2 x1 = p1[pi];
-        if (tr) {
-          const double sg = (double)(a.signs[l3] * a.signs[l4]);
-          x0 = make_double2(sg * x0.x, -sg * x0.y);
-          x1 = make_double2(sg * x1.x, -sg * x1.y);
-        }
+        const double2 x0 = p0[pi];
+        const double2 x1 = p1[pi];
+        double sg = t.wt;
+        if (NL > 1) sg *= t.tr ? (double)(a.signs[l3] * a.signs[l4]) : 1.0;
         const double2 mv = mb[NL * NL * (l3 + NL * l4)];
-        const double2 ya = make_double2(ca0 * x0.x + ca1 * x1.x, ca0 * x0.y + ca1 * x1.y);
-        const double2 yb = make_double2(cb0 * x0.x + cb1 * x1.x, cb0 * x0.y + cb1 * x1.y);
-        acc_a.x += mv.x * ya.x - mv.y * ya.y;
-        acc_a.y += mv.x * ya.y + mv.y * ya.x;
-        acc_b.x += mv.x * yb.x - mv.y * yb.y;
-        acc_b.y += mv.x * yb.y + mv.y * yb.x;
+        // y = sg * (c0 x0 + c1 x1) with conj for time reversal
+        const double2 ya = make_double2(sg * (ca0 * x0.x + ca1 * x1.x), sg * im_sign * (ca0 * x0.y + ca1 * x1.y));
+        const double2 yb = make_double2(sg * (cb0 * x0.x + cb1 * x1.x), sg * im_sign * (cb0 * x0.y + cb1 * x1.y));
+        ya_sum.x += mv.x * ya.x - mv.y * ya.y;
+        ya_sum.y += mv.x * ya.y + mv.y * ya.x;
+        yb_sum.x += mv.x * yb.x - mv.y * yb.y;
+        yb_sum.y += mv.x * yb.y + mv.y * yb.x;
       }
     }
+    acc_a.x += ya_sum.x;
+    acc_a.y += ya_sum.y;
+    acc_b.x += yb_sum.x;
+    acc_b.y += yb_sum.y;
   }
 }
 
-// grid: x -> 64 rows, y -> 4 columns, z -> (target pair, q) with the bosonic
-// frequency of q slowest so that concurrently running blocks gather from the
-// same PhiR(., ., w2) subset (which depends on qw only) and keep it in L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a
+  // contiguous logical range (here: a contiguous range of bosonic frequencies)
+  const int xcd = bid & 7, idx = bid >> 3;
+  const int per = nblk >> 3, rem = nblk & 7;
+  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+// 1-D grid remapped per XCD; logical block = ((qw-major q) * 2 + pair) * tiles + tile,
+// tile = 64 rows x (4 * CPT) columns of one (q, pair); a thread owns CPT columns
+// of one row so that each wave keeps several streaming loads in flight.
+constexpr int CPT = 4;
+
 template <int NL>
 __global__ __launch_bounds__(256) void pae_kernel(PaeArgs a) {
   const int nf = a.nf, nz = nf * NL, nw = a.nw;
   const size_t nz2 = (size_t)nz * nz;
-  const int row = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int col = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (row >= nz || col >= nz) return;
-  const int pair = blockIdx.z & 1;  // 0: targets d,m ; 1: targets s,t
-  const int jq = blockIdx.z >> 1;
+  const int tx = (nz + 63) >> 6, ty = (nz + 4 * CPT - 1) / (4 * CPT);
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = logical % (tx * ty);
+  const int zq = logical / (tx * ty);
+  const int row = (tile % tx) * 64 + (threadIdx.x & 63);
+  const int col0 = (tile / tx) * (4 * CPT) + (threadIdx.x >> 6);
+  if (row >= nz) return;
+  const int pair = zq & 1;  // 0: targets d,m ; 1: targets s,t
+  const int jq = zq >> 1;
   const int w0 = jq / a.nibz;
   const int m = jq - w0 * a.nibz;
   const int q = m * nw + w0;        // Index_bosonic_IBZ order: momentum-major
   const int qw = w0 + 1;
   const int l1 = row / nf, nu1 = row - l1 * nf + 1;
-  const int l2 = col / nf, nu2 = col - l2 * nf + 1;
   const int ta = pair == 0 ? 0 : 2, tb = ta + 1;
-  const size_t idx = (size_t)row + (size_t)nz * col + nz2 * q;
 
-  // streaming operands first: Phi (read once) and the Lambda block
-  double2 fa = ld_stream(a.phi[ta] + idx);
-  double2 fb = ld_stream(a.phi[tb] + idx);
-  double2 la = make_double2(0.0, 0.0), lb = make_double2(0.0, 0.0);
-  if (l1 == 0 && l2 == 0) {
-    const size_t li = (size_t)(nu1 - 1) + (size_t)nf * ((size_t)(nu2 - 1) + (size_t)nf * (qw - 1));
-    la = a.lam[ta][li];
-    lb = a.lam[tb][li];
+  // streaming operands of all CPT columns first
+  double2 fa[CPT], fb[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int col = col0 + 4 * c;
+    const size_t idx = (size_t)row + (size_t)nz * (col < nz ? col : 0) + nz2 * q;
+    fa[c] = ld_stream(a.phi[ta] + idx);
+    fb[c] = ld_stream(a.phi[tb] + idx);
   }
-
-  double2 acc_a = make_double2(0.0, 0.0), acc_b = make_double2(0.0, 0.0);
-  if (pair == 0) {
-    // phbar -> ph from d (-1/2, -1/2) and m (-3/2, +1/2)
-    {
-      const int dn = nu2 - nu1;
-      if (dn >= 0 && dn + 1 <= nw) {
-        const int w2 = dn + 1;
-        crossing_term<NL>(a, a.M[0], 0, 1, -0.5, -1.5, -0.5, 0.5, l1, l2, m, w2, nu1,
-                          clampnu(nu1 + qw - 1, nf), false, acc_a, acc_b);
-      } else if (dn < 0 && 1 - dn <= nw) {
-        const int w2 = 1 - dn;
-        crossing_term<NL>(a, a.M[0], 0, 1, -0.5, -1.5, -0.5, 0.5, l1, l2, m, w2, nf + 1 - nu1,
-                          clampnu(nf + 1 - (nu1 + qw - 1), nf), true, acc_a, acc_b);
-      }
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int col = col0 + 4 * c;
+    if (col >= nz) continue;
+    const int l2 = col / nf, nu2 = col - l2 * nf + 1;
+    double2 la = make_double2(0.0, 0.0), lb = make_double2(0.0, 0.0);
+    if (l1 == 0 && l2 == 0) {
+      const size_t li = (size_t)(nu1 - 1) + (size_t)nf * ((size_t)(nu2 - 1) + (size_t)nf * (qw - 1));
+      la = a.lam[ta][li];
+      lb = a.lam[tb][li];
     }
-    // pp -> ph from s (+1/2, -1/2) and t (+3/2, +1/2)
-    {
-      const int w2 = nu1 + nu2 + qw - nf - 1;
-      const int w2t = nf + 3 - qw - nu1 - nu2;
-      if (w2 >= 1 && w2 <= nw) {
-        crossing_term<NL>(a, a.M[1], 2, 3, 0.5, 1.5, -0.5, 0.5, l1, l2, m, w2, nu1, nu2, false,
-                          acc_a, acc_b);
-      } else if (w2t >= 2 && w2t <= nw) {
-        crossing_term<NL>(a, a.M[1], 2, 3, 0.5, 1.5, -0.5, 0.5, l1, l2, m, w2t, nf + 1 - nu1,
-                          nu1 + qw + w2t - 2, true, acc_a, acc_b);
-      }
+    const int dn = nu2 - nu1;
+    // phbar-type term (same frequency geometry for both target pairs)
+    const bool pb_plain = dn >= 0 && dn + 1 <= nw;
+    const bool pb_tr = dn < 0 && 1 - dn <= nw;
+    double2 acc_a = make_double2(0.0, 0.0), acc_b = make_double2(0.0, 0.0);
+    if (pair == 0) {
+      // phbar -> ph from d (-1/2, -1/2) and m (-3/2, +1/2)
+      const Term tb1 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(nu1 + qw - 1, nf), 1 - dn,
+                                 nf + 1 - nu1, clampnu(nf + 1 - (nu1 + qw - 1), nf));
+      // pp -> ph from s (+1/2, -1/2) and t (+3/2, +1/2)
+      const int w2 = nu1 + nu2 + qw - nf - 1, w2t = nf + 3 - qw - nu1 - nu2;
+      const Term tp = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
+                                nf + 1 - nu1, nu1 + qw + w2t - 2);
+      crossing_term<NL>(a, a.M[0], 0, 1, -0.5, -1.5, -0.5, 0.5, l1, l2, m, tb1, acc_a, acc_b);
+      crossing_term<NL>(a, a.M[1], 2, 3, 0.5, 1.5, -0.5, 0.5, l1, l2, m, tp, acc_a, acc_b);
+    } else {
+      // ph -> pp from d (+1/2, +1/2) and m (-3/2, +1/2)
+      const int w2 = qw - nu1 - nu2 + nf + 1, w2t = nu1 + nu2 - qw - nf + 1;
+      const Term th = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
+                                nf + 1 - nu1, nu1 - qw - w2t + 2);
+      // phbar -> pp from d (+1/2, -1/2) and m (-3/2, -1/2)
+      const Term tb2 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(qw - nu1 - dn + nf, nf), 1 - dn,
+                                 nf + 1 - nu1, clampnu(nu1 - qw - (1 - dn) + 2, nf));
+      crossing_term<NL>(a, a.M[2], 0, 1, 0.5, -1.5, 0.5, 0.5, l1, l2, m, th, acc_a, acc_b);
+      crossing_term<NL>(a, a.M[3], 0, 1, 0.5, -1.5, -0.5, -0.5, l1, l2, m, tb2, acc_a, acc_b);
     }
-  } else {
-    // ph -> pp from d (+1/2, +1/2) and m (-3/2, +1/2)
-    {
-      const int w2 = qw - nu1 - nu2 + nf + 1;
-      const int w2t = nu1 + nu2 - qw - nf + 1;
-      if (w2 >= 1 && w2 <= nw) {
-        crossing_term<NL>(a, a.M[2], 0, 1, 0.5, -1.5, 0.5, 0.5, l1, l2, m, w2, nu1, nu2, false,
-                          acc_a, acc_b);
-      } else if (w2t >= 2 && w2t <= nw) {
-        crossing_term<NL>(a, a.M[2], 0, 1, 0.5, -1.5, 0.5, 0.5, l1, l2, m, w2t, nf + 1 - nu1,
-                          nu1 - qw - w2t + 2, true, acc_a, acc_b);
-      }
-    }
-    // phbar -> pp from d (+1/2, -1/2) and m (-3/2, -1/2)
-    {
-      const int dn = nu2 - nu1;
-      if (dn >= 0 && dn + 1 <= nw) {
-        const int w2 = dn + 1;
-        crossing_term<NL>(a, a.M[3], 0, 1, 0.5, -1.5, -0.5, -0.5, l1, l2, m, w2, nu1,
-                          clampnu(qw - nu1 - w2 + nf + 1, nf), false, acc_a, acc_b);
-      } else if (dn < 0 && 1 - dn <= nw) {
-        const int w2 = 1 - dn;
-        crossing_term<NL>(a, a.M[3], 0, 1, 0.5, -1.5, -0.5, -0.5, l1, l2, m, w2, nf + 1 - nu1,
-                          clampnu(nu1 - qw - w2 + 2, nf), true, acc_a, acc_b);
-      }
-    }
+    double2 oa = fa[c], ob = fb[c];
+    oa.x += la.x * a.nc_aux + acc_a.x * a.inv_nx;
+    oa.y += la.y * a.nc_aux + acc_a.y * a.inv_nx;
+    ob.x += lb.x * a.nc_aux + acc_b.x * a.inv_nx;
+    ob.y += lb.y * a.nc_aux + acc_b.y * a.inv_nx;
+    const size_t idx = (size_t)row + (size_t)nz * col + nz2 * q;
+    st_stream(a.F[ta] + idx, oa);
+    st_stream(a.F[tb] + idx, ob);
   }
+}
 
-  fa.x += la.x * a.nc_aux + acc_a.x * a.inv_nx;
-  fa.y += la.y * a.nc_aux + acc_a.y * a.inv_nx;
-  fb.x += lb.x * a.nc_aux + acc_b.x * a.inv_nx;
-  fb.y += lb.y * a.nc_aux + acc_b.y * a.inv_nx;
-  st_stream(a.F[ta] + idx, fa);
-  st_stream(a.F[tb] + idx, fb);
+// ---- Nl = 1: gather pre-pass + streaming crossing ---------------------------------
+//
+// With one form factor each routine's term at target (nu1, nu2, q) depends on
+// q only through qw (the M tensor is a scalar per (Q, R)).  The pre-pass
+// re-indexes the source PhiR of every routine into
+//   Y[k][R][qw][nu2][nu1],  k = 2*routine + target-of-pair,
+// already combined over the source channels with the factor table, with the
+// time-reversed form (conj) selected and invalid entries zero.  The crossing
+// then reads Y contiguously: F = Phi + Lambda Nc + (1/Nx) sum_R M(Q,R) (Y_a + Y_b).
+
+__global__ __launch_bounds__(256) void pae_pregather_nl1(const double2* __restrict__ pd,
+                                                         const double2* __restrict__ pm,
+                                                         const double2* __restrict__ ps,
+                                                         const double2* __restrict__ pt,
+                                                         double2* __restrict__ Y, int nf, int nw,
+                                                         int nr) {
+  const size_t nf2 = (size_t)nf * nf;
+  const size_t per = nf2 * nw * nr;  // one k slice
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= per) return;
+  const int nu1 = (int)(gid % nf) + 1;
+  const int nu2 = (int)((gid / nf) % nf) + 1;
+  const int qw = (int)((gid / nf2) % nw) + 1;
+  const int R = (int)(gid / (nf2 * nw));
+  const size_t rb = nf2 * (size_t)nw * R;  // PhiR(nu_a, nu_b, w2, R) = rb + (w2-1)*nf2 + (nub-1)*nf + nua-1
+  auto at = [&](const Term& t) { return rb + (size_t)(t.w2 - 1) * nf2 + (size_t)(t.nub - 1) * nf + (t.nua - 1); };
+  auto comb = [&](const double2* p0, const double2* p1, const Term& t, double c0, double c1) {
+    const size_t i = at(t);
+    const double2 x0 = p0[i], x1 = p1[i];
+    const double im = t.tr ? -1.0 : 1.0;
+    return make_double2(t.wt * (c0 * x0.x + c1 * x1.x), t.wt * im * (c0 * x0.y + c1 * x1.y));
+  };
+  const int dn = nu2 - nu1;
+  const bool pb_plain = dn >= 0 && dn + 1 <= nw;
+  const bool pb_tr = dn < 0 && 1 - dn <= nw;
+  const Term t0 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(nu1 + qw - 1, nf), 1 - dn,
+                            nf + 1 - nu1, clampnu(nf + 1 - (nu1 + qw - 1), nf));
+  int w2 = nu1 + nu2 + qw - nf - 1, w2t = nf + 3 - qw - nu1 - nu2;
+  const Term t1 = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
+                            nf + 1 - nu1, nu1 + qw + w2t - 2);
+  w2 = qw - nu1 - nu2 + nf + 1;
+  w2t = nu1 + nu2 - qw - nf + 1;
+  const Term t2 = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
+                            nf + 1 - nu1, nu1 - qw - w2t + 2);
+  const Term t3 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(qw - nu1 - dn + nf, nf), 1 - dn,
+                            nf + 1 - nu1, clampnu(nu1 - qw - (1 - dn) + 2, nf));
+  const size_t o = gid;  // same (nu1, nu2, qw, R) ordering as the k slices
+  Y[0 * per + o] = comb(pd, pm, t0, -0.5, -1.5);  // phbar->ph  -> d
+  Y[1 * per + o] = comb(pd, pm, t0, -0.5, 0.5);   //            -> m
+  Y[2 * per + o] = comb(ps, pt, t1, 0.5, 1.5);    // pp->ph     -> d
+  Y[3 * per + o] = comb(ps, pt, t1, -0.5, 0.5);   //            -> m
+  Y[4 * per + o] = comb(pd, pm, t2, 0.5, -1.5);   // ph->pp     -> s
+  Y[5 * per + o] = comb(pd, pm, t2, 0.5, 0.5);    //            -> t
+  Y[6 * per + o] = comb(pd, pm, t3, 0.5, -1.5);   // phbar->pp  -> s
+  Y[7 * per + o] = comb(pd, pm, t3, -0.5, -0.5);  //            -> t
+}
+
+struct PaeNl1Args {
+  const double2* phi[4];
+  double2* F[4];
+  const double2* lam[4];
+  const double2* M[4];
+  const double2* Y;
+  int nf, nw, nibz, nr, nq;
+  double inv_nx, nc_aux;
+};
+
+// one thread per (nu1, nu2) entry of one (q, target pair), CPT columns per thread
+__global__ __launch_bounds__(256) void pae_stream_nl1(PaeNl1Args a) {
+  const int nf = a.nf, nw = a.nw;
+  const size_t nf2 = (size_t)nf * nf;
+  const int tx = (nf + 63) >> 6, ty = (nf + 4 * CPT - 1) / (4 * CPT);
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = logical % (tx * ty);
+  const int zq = logical / (tx * ty);
+  const int nu1 = (tile % tx) * 64 + (threadIdx.x & 63);
+  const int c0 = (tile / tx) * (4 * CPT) + (threadIdx.x >> 6);
+  if (nu1 >= nf) return;
+  const int pair = zq & 1;
+  const int jq = zq >> 1;
+  const int w0 = jq / a.nibz;
+  const int m = jq - w0 * a.nibz;
+  const int q = m * nw + w0;
+  const int ta = pair == 0 ? 0 : 2, tb = ta + 1;
+  const int ka = pair == 0 ? 0 : 4;  // routines (0,1) feed d,m ; (2,3) feed s,t
+  const size_t per = nf2 * nw * a.nr;
+  double2 fa[CPT], fb[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int nu2 = c0 + 4 * c;
+    const size_t idx = (size_t)nu1 + (size_t)nf * (nu2 < nf ? nu2 : 0) + nf2 * q;
+    fa[c] = ld_stream(a.phi[ta] + idx);
+    fb[c] = ld_stream(a.phi[tb] + idx);
+  }
+  const double2* Ma = a.M[pair == 0 ? 0 : 2];
+  const double2* Mb = a.M[pair == 0 ? 1 : 3];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int nu2 = c0 + 4 * c;
+    if (nu2 >= nf) continue;
+    const size_t li = (size_t)nu1 + (size_t)nf * ((size_t)nu2 + (size_t)nf * w0);
+    const double2 la = a.lam[ta][li], lb = a.lam[tb][li];
+    double2 sa = make_double2(0.0, 0.0), sb = make_double2(0.0, 0.0);
+    for (int R = 0; R < a.nr; ++R) {
+      const size_t o = (size_t)nu1 + (size_t)nf * nu2 + nf2 * ((size_t)w0 + (size_t)nw * R);
+      const double2 m0 = Ma[(size_t)m + (size_t)a.nibz * R], m1 = Mb[(size_t)m + (size_t)a.nibz * R];
+      const double2 ya0 = a.Y[(ka + 0) * per + o], yb0 = a.Y[(ka + 1) * per + o];
+      const double2 ya1 = a.Y[(ka + 2) * per + o], yb1 = a.Y[(ka + 3) * per + o];
+      sa.x += m0.x * ya0.x - m0.y * ya0.y + m1.x * ya1.x - m1.y * ya1.y;
+      sa.y += m0.x * ya0.y + m0.y * ya0.x + m1.x * ya1.y + m1.y * ya1.x;
+      sb.x += m0.x * yb0.x - m0.y * yb0.y + m1.x * yb1.x - m1.y * yb1.y;
+      sb.y += m0.x * yb0.y + m0.y * yb0.x + m1.x * yb1.y + m1.y * yb1.x;
+    }
+    double2 oa = fa[c], ob = fb[c];
+    oa.x += la.x * a.nc_aux + sa.x * a.inv_nx;
+    oa.y += la.y * a.nc_aux + sa.y * a.inv_nx;
+    ob.x += lb.x * a.nc_aux + sb.x * a.inv_nx;
+    ob.y += lb.y * a.nc_aux + sb.y * a.inv_nx;
+    const size_t idx = (size_t)nu1 + (size_t)nf * nu2 + nf2 * q;
+    st_stream(a.F[ta] + idx, oa);
+    st_stream(a.F[tb] + idx, ob);
+  }
 }
 
 int launch_pae(tups_ctx* c) {
@@ -208,8 +352,39 @@ int launch_pae(tups_ctx* c) {
   a.inv_nx = 1.0 / s.nx;
   const double f0 = c->tab.ff[0];
   a.nc_aux = std::fabs(1.0 / f0 * 1.0 / f0);
-  TUPS_CHECK(2LL * s.nq <= 65535, TUPS_EINVAL, "too many q for the crossing grid");
-  dim3 g((unsigned)((s.nz + 63) / 64), (unsigned)((s.nz + 3) / 4), (unsigned)(2 * s.nq));
+  if (s.nl == 1 && c->pae_y) {
+    const size_t per = (size_t)s.nf * s.nf * s.nw * s.nr;
+    if (int rc = timer_begin(c)) return rc;
+    hipLaunchKernelGGL(pae_pregather_nl1, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, c->stream,
+                       c->phir[0], c->phir[1], c->phir[2], c->phir[3], c->pae_y, s.nf, s.nw, s.nr);
+    TUPS_HIP(hipGetLastError());
+    if (int rc = timer_end(c, TUPS_K_PAE_GATHER)) return rc;
+    PaeNl1Args b{};
+    for (int r = 0; r < 4; ++r) {
+      b.phi[r] = a.phi[r];
+      b.F[r] = a.F[r];
+      b.lam[r] = a.lam[r];
+      b.M[r] = a.M[r];
+    }
+    b.Y = c->pae_y;
+    b.nf = s.nf;
+    b.nw = s.nw;
+    b.nibz = s.nibz;
+    b.nr = s.nr;
+    b.nq = s.nq;
+    b.inv_nx = a.inv_nx;
+    b.nc_aux = a.nc_aux;
+    const long long nb1 = (long long)((s.nf + 63) / 64) * ((s.nf + 4 * CPT - 1) / (4 * CPT)) * 2 * s.nq;
+    TUPS_CHECK(nb1 < (1LL << 31), TUPS_EINVAL, "crossing grid too large");
+    if (int rc = timer_begin(c)) return rc;
+    hipLaunchKernelGGL(pae_stream_nl1, dim3((unsigned)nb1), dim3(256), 0, c->stream, b);
+    TUPS_HIP(hipGetLastError());
+    if (int rc = timer_end(c, TUPS_K_PAE)) return rc;
+    return TUPS_OK;
+  }
+  const long long nblk = (long long)((s.nz + 63) / 64) * ((s.nz + 4 * CPT - 1) / (4 * CPT)) * 2 * s.nq;
+  TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "crossing grid too large");
+  dim3 g((unsigned)nblk);
   if (int rc = timer_begin(c)) return rc;
   if (s.nl == 1)
     hipLaunchKernelGGL(pae_kernel<1>, g, dim3(256), 0, c->stream, a);

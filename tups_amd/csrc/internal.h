@@ -93,6 +93,7 @@ struct tups_ctx {
   int* d_ph_src = nullptr;
   double2* d_ph_coef = nullptr;
   int* d_flag = nullptr;
+  double2* pae_y = nullptr;     // Nl = 1 crossing: re-indexed PhiR, 8 x (Nf,Nf,Nw,NR)
 
 
   int bse_variant = -1;          // kernel variant chosen by the autotuner (-1: default)

@@ -17,7 +17,7 @@ module tups_gpu_iface
                                    TUPS_ARR_M = 4, TUPS_ARR_LAMBDA = 5
   integer(c_int32_t), parameter :: TUPS_BSE_LADDER_FP64 = 0
   integer(c_int32_t), parameter :: TUPS_K_BSE_LADDER = 0, TUPS_K_PHIR = 1, TUPS_K_BUILD_M = 2, &
-                                   TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4
+                                   TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4, TUPS_K_PAE_GATHER = 5
 
   type, bind(C) :: tups_dims
     integer(c_int32_t) :: nx, nf, nl, nr, nq, reserved
