@@ -124,6 +124,7 @@ def main():
     from tups_amd.runtime import K_PAE, K_PAE_GATHER
 
     cx = crossing_context(16, 64, 1)
+    cvar, _ = cx.autotune(K_PAE)
     for _ in range(2):
         cx.calculate_PAE()
     cx.set_timing(True)
@@ -138,7 +139,7 @@ def main():
     gbs = cbytes / (per_call_ms / 1e3) / 1e9
     crossing = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "pae_stream_nl1 (+ pae_pregather_nl1)",
+                "kernel": f"pae_stream_nl1 variant {cvar} (+ pae_pregather_nl1)",
                 "avg_launch_ms": round(pae_ms / pae_n, 4), "pregather_ms": round(gat_ms / max(1, gat_n), 4),
                 "bytes_per_call": cbytes,
                 "workload": f"calculate_PAE Nx=16 Nf=64 Nl=1 NR=1, Nq={cx.nq} (all IBZ q), 4 channels"}

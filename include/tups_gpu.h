@@ -135,8 +135,9 @@ int tups_pae(tups_ctx* ctx);
 /* G_r = F_r - G_r; SRC/loop.f90:219-222 */
 int tups_gamma_update(tups_ctx* ctx);
 
-/* Project autotuner: times the tile variants of `kernel` (TUPS_K_BSE_LADDER)
- * on the context's resident inputs and keeps the fastest for later calls.
+/* Project autotuner: times the tile variants of `kernel` (TUPS_K_BSE_LADDER,
+ * or TUPS_K_PAE for Nl = 1 after PhiR and M exist) on the context's resident
+ * inputs and keeps the fastest for later calls.
  * Synchronous.  `variant` receives the chosen variant id, `ms` its time per
  * launch; tups_variant_name() names a variant id. */
 int tups_autotune(tups_ctx* ctx, int32_t kernel, int32_t* variant, float* ms);

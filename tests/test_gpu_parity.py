@@ -230,3 +230,36 @@ def test_every_bse_variant_and_autotune(nx, nf, nl, nq):
             c.BSE(0)
             for r in range(4):
                 assert rel_fro(c.download(GAMMA, r), exp[r]) < TOL, (name, r)
+
+
+@pytest.mark.parametrize("nx,nf,nr", [(4, 8, 1), (6, 8, 2), (8, 16, 1)])
+def test_every_crossing_variant_nl1(nx, nf, nr):
+    """Nl=1 crossing (PhiR re-index pass + streaming kernel): every column
+    variant and the autotuner's pick reproduce the oracle, also for NR > 1."""
+    from tups_amd import F, GAMMA, LAMBDA, PHIR
+    from tups_amd.runtime import K_PAE
+
+    rng = np.random.default_rng(nx + nf + nr)
+    lat = Lattice(nx, nf, 1, nr=nr)
+    mt = P.calc_m_tensors(lat)
+    lam = [crandn(rng, (nf, nf, nf // 2)) for _ in range(4)]
+    with _ctx(nx=nx, nf=nf, nl=1, nr=nr) as c:
+        phi = [crandn(rng, (c.nz, c.nz, c.nq)) for _ in range(4)]
+        phir = [crandn(rng, (c.nz ** 2, c.nw, c.nr)) for _ in range(4)]
+        for r in range(4):
+            c.upload(GAMMA, r, phi[r])
+            c.upload(PHIR, r, phir[r])
+            c.upload(LAMBDA, r, lam[r])
+        c.calc_M()
+        exp = P.calculate_pae(lat, phi, lam, phir, mt)
+        variants = c.variants(K_PAE)
+        assert variants
+        for v, name in variants + [(None, "autotuned")]:
+            _say(f"crossing variant {name} nx={nx} nf={nf} nr={nr}")
+            if v is None:
+                c.autotune(K_PAE)
+            else:
+                c.set_variant(v, K_PAE)
+            c.calculate_PAE()
+            for r in range(4):
+                assert rel_fro(c.download(F, r), exp[r]) < TOL, (name, r)

@@ -464,12 +464,18 @@ int tups_gamma_update(tups_ctx* c) {
 
 int tups_autotune(tups_ctx* c, int32_t kernel, int32_t* variant, float* ms) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
-  TUPS_CHECK(kernel == TUPS_K_BSE_LADDER, TUPS_EINVAL, "only the BSE ladder has tile variants");
+  TUPS_CHECK(kernel == TUPS_K_BSE_LADDER || kernel == TUPS_K_PAE, TUPS_EINVAL,
+             "only the BSE ladder and the crossing have tile variants");
   int rc = ensure_tables(c);
   if (rc) return rc;
   int v = -1;
   float t = 0.f;
-  rc = autotune_bse(c, 2, &v, &t);
+  if (kernel == TUPS_K_PAE) {
+    TUPS_CHECK(c->m_built && c->phir_valid, TUPS_ESTATE, "crossing autotune needs M and PhiR");
+    rc = autotune_pae(c, 3, &v, &t);
+  } else {
+    rc = autotune_bse(c, 2, &v, &t);
+  }
   if (rc) return rc;
   if (variant) *variant = v;
   if (ms) *ms = t;
@@ -477,19 +483,27 @@ int tups_autotune(tups_ctx* c, int32_t kernel, int32_t* variant, float* ms) {
 }
 
 const char* tups_variant_name(int32_t kernel, int32_t variant) {
-  return kernel == TUPS_K_BSE_LADDER ? bse_variant_name(variant) : "";
+  return kernel == TUPS_K_BSE_LADDER ? bse_variant_name(variant)
+                                     : (kernel == TUPS_K_PAE ? pae_variant_name(variant) : "");
 }
 
 int tups_set_variant(tups_ctx* c, int32_t kernel, int32_t variant) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
-  TUPS_CHECK(kernel == TUPS_K_BSE_LADDER, TUPS_EINVAL, "only the BSE ladder has tile variants");
+  if (kernel == TUPS_K_PAE) {
+    TUPS_CHECK(variant == -1 || pae_variant_ok(c, variant), TUPS_EINVAL, "crossing variant not usable here");
+    c->pae_variant = variant;
+    return TUPS_OK;
+  }
+  TUPS_CHECK(kernel == TUPS_K_BSE_LADDER, TUPS_EINVAL, "kernel has no tile variants");
   TUPS_CHECK(variant == -1 || bse_variant_nl(variant) == c->sz.nl, TUPS_EINVAL,
              "variant does not exist for this Nl");
   c->bse_variant = variant;
   return TUPS_OK;
 }
 
-int tups_variant_count(int32_t kernel) { return kernel == TUPS_K_BSE_LADDER ? bse_variant_count() : 0; }
+int tups_variant_count(int32_t kernel) {
+  return kernel == TUPS_K_BSE_LADDER ? bse_variant_count() : (kernel == TUPS_K_PAE ? pae_variant_count() : 0);
+}
 
 int tups_sync(tups_ctx* c) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");

@@ -277,16 +277,19 @@ struct PaeNl1Args {
   double inv_nx, nc_aux;
 };
 
-// one thread per (nu1, nu2) entry of one (q, target pair), CPT columns per thread
+// one thread per (nu1, nu2) entry of one (q, target pair), C columns per
+// thread; with NR1 (one real-space shell) every load of all C columns is
+// issued before the first use.
+template <int C, bool NR1>
 __global__ __launch_bounds__(256) void pae_stream_nl1(PaeNl1Args a) {
   const int nf = a.nf, nw = a.nw;
   const size_t nf2 = (size_t)nf * nf;
-  const int tx = (nf + 63) >> 6, ty = (nf + 4 * CPT - 1) / (4 * CPT);
+  const int tx = (nf + 63) >> 6, ty = (nf + 4 * C - 1) / (4 * C);
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = logical % (tx * ty);
   const int zq = logical / (tx * ty);
   const int nu1 = (tile % tx) * 64 + (threadIdx.x & 63);
-  const int c0 = (tile / tx) * (4 * CPT) + (threadIdx.x >> 6);
+  const int c0 = (tile / tx) * (4 * C) + (threadIdx.x >> 6);
   if (nu1 >= nf) return;
   const int pair = zq & 1;
   const int jq = zq >> 1;
@@ -296,42 +299,211 @@ __global__ __launch_bounds__(256) void pae_stream_nl1(PaeNl1Args a) {
   const int ta = pair == 0 ? 0 : 2, tb = ta + 1;
   const int ka = pair == 0 ? 0 : 4;  // routines (0,1) feed d,m ; (2,3) feed s,t
   const size_t per = nf2 * nw * a.nr;
-  double2 fa[CPT], fb[CPT];
-#pragma unroll
-  for (int c = 0; c < CPT; ++c) {
-    const int nu2 = c0 + 4 * c;
-    const size_t idx = (size_t)nu1 + (size_t)nf * (nu2 < nf ? nu2 : 0) + nf2 * q;
-    fa[c] = ld_stream(a.phi[ta] + idx);
-    fb[c] = ld_stream(a.phi[tb] + idx);
-  }
   const double2* Ma = a.M[pair == 0 ? 0 : 2];
   const double2* Mb = a.M[pair == 0 ? 1 : 3];
+  double2 fa[C], fb[C], la[C], lb[C];
+  double2 y0[C], y1[C], y2[C], y3[C];
 #pragma unroll
-  for (int c = 0; c < CPT; ++c) {
+  for (int c = 0; c < C; ++c) {
+    const int nu2 = c0 + 4 * c < nf ? c0 + 4 * c : 0;
+    const size_t idx = (size_t)nu1 + (size_t)nf * nu2 + nf2 * q;
+    fa[c] = ld_stream(a.phi[ta] + idx);
+    fb[c] = ld_stream(a.phi[tb] + idx);
+    const size_t li = (size_t)nu1 + (size_t)nf * ((size_t)nu2 + (size_t)nf * w0);
+    la[c] = a.lam[ta][li];
+    lb[c] = a.lam[tb][li];
+    if (NR1) {
+      const size_t o = (size_t)nu1 + (size_t)nf * nu2 + nf2 * w0;
+      y0[c] = a.Y[(ka + 0) * per + o];
+      y1[c] = a.Y[(ka + 1) * per + o];
+      y2[c] = a.Y[(ka + 2) * per + o];
+      y3[c] = a.Y[(ka + 3) * per + o];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
     const int nu2 = c0 + 4 * c;
     if (nu2 >= nf) continue;
-    const size_t li = (size_t)nu1 + (size_t)nf * ((size_t)nu2 + (size_t)nf * w0);
-    const double2 la = a.lam[ta][li], lb = a.lam[tb][li];
     double2 sa = make_double2(0.0, 0.0), sb = make_double2(0.0, 0.0);
-    for (int R = 0; R < a.nr; ++R) {
-      const size_t o = (size_t)nu1 + (size_t)nf * nu2 + nf2 * ((size_t)w0 + (size_t)nw * R);
+    for (int R = 0; R < (NR1 ? 1 : a.nr); ++R) {
+      double2 ya0, yb0, ya1, yb1;
+      if (NR1) {
+        ya0 = y0[c];
+        yb0 = y1[c];
+        ya1 = y2[c];
+        yb1 = y3[c];
+      } else {
+        const size_t o = (size_t)nu1 + (size_t)nf * nu2 + nf2 * ((size_t)w0 + (size_t)nw * R);
+        ya0 = a.Y[(ka + 0) * per + o];
+        yb0 = a.Y[(ka + 1) * per + o];
+        ya1 = a.Y[(ka + 2) * per + o];
+        yb1 = a.Y[(ka + 3) * per + o];
+      }
       const double2 m0 = Ma[(size_t)m + (size_t)a.nibz * R], m1 = Mb[(size_t)m + (size_t)a.nibz * R];
-      const double2 ya0 = a.Y[(ka + 0) * per + o], yb0 = a.Y[(ka + 1) * per + o];
-      const double2 ya1 = a.Y[(ka + 2) * per + o], yb1 = a.Y[(ka + 3) * per + o];
       sa.x += m0.x * ya0.x - m0.y * ya0.y + m1.x * ya1.x - m1.y * ya1.y;
       sa.y += m0.x * ya0.y + m0.y * ya0.x + m1.x * ya1.y + m1.y * ya1.x;
       sb.x += m0.x * yb0.x - m0.y * yb0.y + m1.x * yb1.x - m1.y * yb1.y;
       sb.y += m0.x * yb0.y + m0.y * yb0.x + m1.x * yb1.y + m1.y * yb1.x;
     }
     double2 oa = fa[c], ob = fb[c];
-    oa.x += la.x * a.nc_aux + sa.x * a.inv_nx;
-    oa.y += la.y * a.nc_aux + sa.y * a.inv_nx;
-    ob.x += lb.x * a.nc_aux + sb.x * a.inv_nx;
-    ob.y += lb.y * a.nc_aux + sb.y * a.inv_nx;
+    oa.x += la[c].x * a.nc_aux + sa.x * a.inv_nx;
+    oa.y += la[c].y * a.nc_aux + sa.y * a.inv_nx;
+    ob.x += lb[c].x * a.nc_aux + sb.x * a.inv_nx;
+    ob.y += lb[c].y * a.nc_aux + sb.y * a.inv_nx;
     const size_t idx = (size_t)nu1 + (size_t)nf * nu2 + nf2 * q;
     st_stream(a.F[ta] + idx, oa);
     st_stream(a.F[tb] + idx, ob);
   }
+}
+
+// NR = 1: one thread per (nu1, nu2) of one target pair and KM momenta that
+// share the bosonic frequency qw: Y and Lambda depend on q only through qw,
+// so they are loaded once and reused for the KM streaming Phi -> F updates.
+template <int KM>
+__global__ __launch_bounds__(256) void pae_stream_nl1_mom(PaeNl1Args a) {
+  const int nf = a.nf, nw = a.nw;
+  const size_t nf2 = (size_t)nf * nf;
+  const int tx = (nf + 63) >> 6, ty = (nf + 3) >> 2;
+  const int ngm = (a.nibz + KM - 1) / KM;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = logical % (tx * ty);
+  int rest = logical / (tx * ty);
+  const int mg = rest % ngm;
+  rest /= ngm;
+  const int pair = rest & 1;
+  const int w0 = rest >> 1;
+  const int nu1 = (tile % tx) * 64 + (threadIdx.x & 63);
+  const int nu2 = (tile / tx) * 4 + (threadIdx.x >> 6);
+  if (nu1 >= nf || nu2 >= nf) return;
+  const int ta = pair == 0 ? 0 : 2, tb = ta + 1;
+  const int ka = pair == 0 ? 0 : 4;
+  const size_t per = nf2 * nw;  // NR = 1
+  const size_t o = (size_t)nu1 + (size_t)nf * nu2 + nf2 * w0;
+  const double2 ya0 = a.Y[(ka + 0) * per + o], yb0 = a.Y[(ka + 1) * per + o];
+  const double2 ya1 = a.Y[(ka + 2) * per + o], yb1 = a.Y[(ka + 3) * per + o];
+  const size_t li = (size_t)nu1 + (size_t)nf * ((size_t)nu2 + (size_t)nf * w0);
+  const double2 la = a.lam[ta][li], lb = a.lam[tb][li];
+  const double2* Ma = a.M[pair == 0 ? 0 : 2];
+  const double2* Mb = a.M[pair == 0 ? 1 : 3];
+  double2 fa[KM], fb[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const int m = mg * KM + k < a.nibz ? mg * KM + k : a.nibz - 1;
+    const size_t idx = (size_t)nu1 + (size_t)nf * nu2 + nf2 * ((size_t)m * nw + w0);
+    fa[k] = ld_stream(a.phi[ta] + idx);
+    fb[k] = ld_stream(a.phi[tb] + idx);
+  }
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const int m = mg * KM + k;
+    if (m >= a.nibz) break;
+    const double2 m0 = Ma[m], m1 = Mb[m];
+    double2 oa = fa[k], ob = fb[k];
+    oa.x += la.x * a.nc_aux + (m0.x * ya0.x - m0.y * ya0.y + m1.x * ya1.x - m1.y * ya1.y) * a.inv_nx;
+    oa.y += la.y * a.nc_aux + (m0.x * ya0.y + m0.y * ya0.x + m1.x * ya1.y + m1.y * ya1.x) * a.inv_nx;
+    ob.x += lb.x * a.nc_aux + (m0.x * yb0.x - m0.y * yb0.y + m1.x * yb1.x - m1.y * yb1.y) * a.inv_nx;
+    ob.y += lb.y * a.nc_aux + (m0.x * yb0.y + m0.y * yb0.x + m1.x * yb1.y + m1.y * yb1.x) * a.inv_nx;
+    const size_t idx = (size_t)nu1 + (size_t)nf * nu2 + nf2 * ((size_t)m * nw + w0);
+    st_stream(a.F[ta] + idx, oa);
+    st_stream(a.F[tb] + idx, ob);
+  }
+}
+
+typedef void (*PaeKernel)(PaeNl1Args);
+struct PaeVariant {
+  int cols;   // columns per thread (column kernels)
+  int km;     // momenta per thread (momentum kernels), 0 for column kernels
+  bool nr1;
+  PaeKernel fn;
+  const char* name;
+};
+static const PaeVariant kPaeVariants[] = {
+    {2, 0, true, pae_stream_nl1<2, true>, "nl1_c2"},
+    {4, 0, true, pae_stream_nl1<4, true>, "nl1_c4"},
+    {8, 0, true, pae_stream_nl1<8, true>, "nl1_c8"},
+    {4, 0, false, pae_stream_nl1<4, false>, "nl1_c4_nr"},
+    {1, 2, true, pae_stream_nl1_mom<2>, "nl1_m2"},
+    {1, 4, true, pae_stream_nl1_mom<4>, "nl1_m4"},
+    {1, 8, true, pae_stream_nl1_mom<8>, "nl1_m8"},
+};
+constexpr int kNumPaeVariants = sizeof(kPaeVariants) / sizeof(kPaeVariants[0]);
+
+int pae_variant_count() { return kNumPaeVariants; }
+const char* pae_variant_name(int v) { return v >= 0 && v < kNumPaeVariants ? kPaeVariants[v].name : ""; }
+// variant usable for this context (Nl = 1, and NR = 1 for the nr1 kernels)
+bool pae_variant_ok(const tups_ctx* c, int v) {
+  return v >= 0 && v < kNumPaeVariants && c->sz.nl == 1 && (!kPaeVariants[v].nr1 || c->sz.nr == 1);
+}
+
+static int pae_nl1_launch(tups_ctx* c, int v, const PaeNl1Args& b) {
+  const tups_sizes& s = c->sz;
+  const int cols = kPaeVariants[v].cols, km = kPaeVariants[v].km;
+  const long long tiles = (long long)((s.nf + 63) / 64) * ((s.nf + 4 * cols - 1) / (4 * cols));
+  const long long nb1 = km ? tiles * ((s.nibz + km - 1) / km) * 2 * s.nw : tiles * 2 * s.nq;
+  TUPS_CHECK(nb1 < (1LL << 31), TUPS_EINVAL, "crossing grid too large");
+  hipLaunchKernelGGL(kPaeVariants[v].fn, dim3((unsigned)nb1), dim3(256), 0, c->stream, b);
+  TUPS_HIP(hipGetLastError());
+  return TUPS_OK;
+}
+
+static PaeNl1Args nl1_args(tups_ctx* c) {
+  const tups_sizes& s = c->sz;
+  PaeNl1Args b{};
+  for (int r = 0; r < 4; ++r) {
+    b.phi[r] = c->gamma[r];
+    b.F[r] = c->fvert[r];
+    b.lam[r] = c->lam[r];
+    b.M[r] = c->mten[r];
+  }
+  b.Y = c->pae_y;
+  b.nf = s.nf;
+  b.nw = s.nw;
+  b.nibz = s.nibz;
+  b.nr = s.nr;
+  b.nq = s.nq;
+  b.inv_nx = 1.0 / s.nx;
+  const double f0 = c->tab.ff[0];
+  b.nc_aux = std::fabs(1.0 / f0 * 1.0 / f0);
+  return b;
+}
+
+static int pae_pregather(tups_ctx* c) {
+  const tups_sizes& s = c->sz;
+  const size_t per = (size_t)s.nf * s.nf * s.nw * s.nr;
+  hipLaunchKernelGGL(pae_pregather_nl1, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, c->stream,
+                     c->phir[0], c->phir[1], c->phir[2], c->phir[3], c->pae_y, s.nf, s.nw, s.nr);
+  TUPS_HIP(hipGetLastError());
+  return TUPS_OK;
+}
+
+// Times the usable streaming variants on the resident inputs (the output F is
+// rewritten identically each time) and keeps the fastest.
+int autotune_pae(tups_ctx* c, int reps, int* chosen, float* best_ms) {
+  TUPS_CHECK(c->sz.nl == 1 && c->pae_y, TUPS_EINVAL, "crossing variants exist for Nl = 1 only");
+  if (int rc = pae_pregather(c)) return rc;
+  const PaeNl1Args b = nl1_args(c);
+  float best = 0.f;
+  int arg = -1;
+  for (int v = 0; v < kNumPaeVariants; ++v) {
+    if (!pae_variant_ok(c, v)) continue;
+    if (int rc = pae_nl1_launch(c, v, b)) return rc;
+    TUPS_HIP(hipEventRecord(c->tev[0], c->stream));
+    for (int i = 0; i < reps; ++i)
+      if (int rc = pae_nl1_launch(c, v, b)) return rc;
+    TUPS_HIP(hipEventRecord(c->tev[1], c->stream));
+    TUPS_HIP(hipEventSynchronize(c->tev[1]));
+    float ms = 0.f;
+    TUPS_HIP(hipEventElapsedTime(&ms, c->tev[0], c->tev[1]));
+    ms /= reps;
+    if (arg < 0 || ms < best) {
+      best = ms;
+      arg = v;
+    }
+  }
+  c->pae_variant = arg;
+  if (chosen) *chosen = arg;
+  if (best_ms) *best_ms = best;
+  return TUPS_OK;
 }
 
 int launch_pae(tups_ctx* c) {
@@ -353,32 +525,14 @@ int launch_pae(tups_ctx* c) {
   const double f0 = c->tab.ff[0];
   a.nc_aux = std::fabs(1.0 / f0 * 1.0 / f0);
   if (s.nl == 1 && c->pae_y) {
-    const size_t per = (size_t)s.nf * s.nf * s.nw * s.nr;
+    int v = c->pae_variant;
+    if (!pae_variant_ok(c, v)) v = s.nr == 1 ? 1 : 3;
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL(pae_pregather_nl1, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, c->stream,
-                       c->phir[0], c->phir[1], c->phir[2], c->phir[3], c->pae_y, s.nf, s.nw, s.nr);
-    TUPS_HIP(hipGetLastError());
+    if (int rc = pae_pregather(c)) return rc;
     if (int rc = timer_end(c, TUPS_K_PAE_GATHER)) return rc;
-    PaeNl1Args b{};
-    for (int r = 0; r < 4; ++r) {
-      b.phi[r] = a.phi[r];
-      b.F[r] = a.F[r];
-      b.lam[r] = a.lam[r];
-      b.M[r] = a.M[r];
-    }
-    b.Y = c->pae_y;
-    b.nf = s.nf;
-    b.nw = s.nw;
-    b.nibz = s.nibz;
-    b.nr = s.nr;
-    b.nq = s.nq;
-    b.inv_nx = a.inv_nx;
-    b.nc_aux = a.nc_aux;
-    const long long nb1 = (long long)((s.nf + 63) / 64) * ((s.nf + 4 * CPT - 1) / (4 * CPT)) * 2 * s.nq;
-    TUPS_CHECK(nb1 < (1LL << 31), TUPS_EINVAL, "crossing grid too large");
+    const PaeNl1Args b = nl1_args(c);
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL(pae_stream_nl1, dim3((unsigned)nb1), dim3(256), 0, c->stream, b);
-    TUPS_HIP(hipGetLastError());
+    if (int rc = pae_nl1_launch(c, v, b)) return rc;
     if (int rc = timer_end(c, TUPS_K_PAE)) return rc;
     return TUPS_OK;
   }

@@ -97,6 +97,7 @@ struct tups_ctx {
 
 
   int bse_variant = -1;          // kernel variant chosen by the autotuner (-1: default)
+  int pae_variant = -1;
   bool timing = false;
   hipEvent_t tev[2] = {};
   double t_ms[TUPS_K_COUNT] = {};
@@ -114,6 +115,10 @@ int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms);
 const char* bse_variant_name(int v);
 int bse_variant_count();
 int bse_variant_nl(int v);
+int autotune_pae(tups_ctx* c, int reps, int* chosen, float* best_ms);
+int pae_variant_count();
+const char* pae_variant_name(int v);
+bool pae_variant_ok(const tups_ctx* c, int v);
 int launch_phir(tups_ctx* c);
 int launch_build_m(tups_ctx* c);
 int launch_pae(tups_ctx* c);

@@ -247,8 +247,11 @@ class TupsContext:
         out = []
         for v in range(self.lib.tups_variant_count(kernel)):
             name = self.lib.tups_variant_name(kernel, v).decode()
-            if name.startswith(f"nl{self.nl}_"):
-                out.append((v, name))
+            if not name.startswith(f"nl{self.nl}_"):
+                continue
+            if kernel == K_PAE and "_nr" not in name and self.nr != 1:
+                continue
+            out.append((v, name))
         return out
 
     def set_variant(self, variant: int, kernel: int = K_BSE_LADDER):
