@@ -8,8 +8,13 @@ Nq = 64 momenta x 16 bosonic frequencies = 1024 q; all four channels
 SRC/parquet_BSE.f90:62-132 loop), inputs resident in HBM.  One q-point = the
 update of all four channels at one q.
 
-Prints one JSON line with the measured throughput, a live roofline of the
-dominant kernel (HIP events on the library's own stream) and a bounded CPU
+Two forms of the same step are timed: the split (precision) form, whose GEMM
+runs as exact INT8 MFMA products of residues and whose error against the
+FP64 path is measured per channel, and the plain FP64 MFMA form.  `value` is
+the split form's rate; the FP64 form's rate and roofline sit beside it.
+
+Prints one JSON line with the measured throughput, live rooflines of the
+dominant kernels (HIP events on the library's own stream) and a bounded CPU
 baseline (the numpy oracle restatement with host BLAS, rank 0, N = 1 only).
 """
 from __future__ import annotations
@@ -21,6 +26,7 @@ import sys
 import time
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 MFMA, AMD published
+INT8_PEAK_TOPS = 5000.0  # MI355X dense INT8 MFMA, AMD published
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E, AMD published
 
 
@@ -32,6 +38,8 @@ def parse():
     ap.add_argument("--nq", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--moduli", type=int, default=12, help="CRT moduli of the split form")
+    ap.add_argument("--qbatch", type=int, default=0, help="q points per split pass (0: library default)")
     return ap.parse_args()
 
 
@@ -76,8 +84,8 @@ def main():
               "of this path is a design note, not a run", file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from tups_amd import TupsContext
-    from tups_amd.runtime import K_BSE_LADDER
+    from tups_amd import BSE_LADDER_FP64, BSE_LADDER_SPLIT, GAMMA, TupsContext
+    from tups_amd.runtime import K_BSE_LADDER, K_SPLIT_GEMM, K_SPLIT_PREP
     from tups_amd.synth import batched_bse_inputs
 
     nx, nf, nl, U = 8, 64, 5, 2.0
@@ -87,22 +95,23 @@ def main():
     D, nq = ctx.nz, ctx.nq
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s D={D} nq={nq}", file=sys.stderr, flush=True)
 
+    def timed(mode):
+        for _ in range(args.warmup):
+            ctx.BSE(0, mode=mode)
+        ctx.sync()
+        ctx.record(0)
+        for _ in range(args.steps):
+            ctx.BSE(0, mode=mode)
+        ctx.record(1)
+        ctx.sync()
+        return ctx.elapsed_ms(0, 1)
+
+    # --- FP64 form ---------------------------------------------------------------------
     variant, tune_ms = ctx.autotune()
     print(f"[bench] autotuner picked {variant} ({tune_ms:.2f} ms/launch)", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        ctx.BSE(0)
-    ctx.sync()
-    ctx.record(0)
-    for _ in range(args.steps):
-        ctx.BSE(0)
-    ctx.record(1)
-    ctx.sync()
-    total_ms = ctx.elapsed_ms(0, 1)
-    ms_step = total_ms / args.steps
-    qps = nq * args.steps / (total_ms / 1e3)
-    print(f"[bench] {args.steps} steps {total_ms:.1f} ms -> {qps:.1f} q-points/s", file=sys.stderr, flush=True)
-
-    # live roofline of the dominant kernel: per-launch HIP events on the library stream
+    ms64 = timed(BSE_LADDER_FP64)
+    qps64 = nq * args.steps / (ms64 / 1e3)
+    print(f"[bench] fp64: {args.steps} steps {ms64:.1f} ms -> {qps64:.1f} q-points/s", file=sys.stderr, flush=True)
     nprof = max(2, min(args.steps, 5))
     ctx.set_timing(True)
     for _ in range(nprof):
@@ -111,13 +120,50 @@ def main():
     ctx.set_timing(False)
     q_per_launch = nq * nprof / lad_n
     flops_per_launch = 16.0 * D ** 3 * 4 * q_per_launch  # 2 complex DxD GEMMs per (q, channel)
-    avg_launch_s = lad_ms / lad_n / 1e3
-    achieved = flops_per_launch / avg_launch_s / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": f"bse_ladder_kernel variant {variant}", "avg_launch_ms": round(lad_ms / lad_n, 4),
-                "flops_per_launch": flops_per_launch}
+    achieved = flops_per_launch / (lad_ms / lad_n / 1e3) / 1e12
+    roof64 = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+              "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+              "kernel": f"bse_ladder_kernel variant {variant}", "avg_launch_ms": round(lad_ms / lad_n, 4),
+              "flops_per_launch": flops_per_launch}
+
+    # --- split form ----------------------------------------------------------------------
+    ctx.set_split(args.moduli, args.qbatch)
+    batched_bse_inputs(ctx, U=U, seed=0)
+    ms_sp = timed(BSE_LADDER_SPLIT)
+    qps = nq * args.steps / (ms_sp / 1e3)
+    print(f"[bench] split: {args.steps} steps {ms_sp:.1f} ms -> {qps:.1f} q-points/s", file=sys.stderr, flush=True)
+    ctx.set_timing(True)
+    for _ in range(nprof):
+        ctx.BSE(0, mode=BSE_LADDER_SPLIT)
+    gemm_ms, gemm_n = ctx.kernel_time(K_SPLIT_GEMM)
+    prep_ms, prep_n = ctx.kernel_time(K_SPLIT_PREP)
+    ctx.set_timing(False)
+    q_per_gemm = nq * nprof / gemm_n
+    # algorithmic INT8 ops: 2 real GEMMs (D x D x 2D) per modulus per (q, channel)
+    ops_per_launch = 2.0 * args.moduli * 2.0 * D * D * (2 * D) * 4 * q_per_gemm
+    ach_sp = ops_per_launch / (gemm_ms / gemm_n / 1e3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(ach_sp, 1), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                "frac": round(ach_sp / INT8_PEAK_TOPS, 4), "traffic": None,
+                "kernel": f"crt_gemm ({args.moduli} moduli)", "avg_launch_ms": round(gemm_ms / gemm_n, 4),
+                "ops_per_launch": ops_per_launch, "prep_ms_per_launch": round(prep_ms / prep_n, 4),
+                "launches_per_step": gemm_n // nprof}
     ctx.close()
+
+    # split error per channel vs the FP64 form, on 64 q of the same workload
+    ec = TupsContext(nx=nx, nf=nf, nl=nl, nq=64, beta=2.0, f_damping=0.5)
+    batched_bse_inputs(ec, U=U, seed=5)
+    ec.BSE(0)
+    ref = [ec.download(GAMMA, r) for r in range(4)]
+    batched_bse_inputs(ec, U=U, seed=5)
+    ec.set_split(args.moduli, 0)
+    ec.BSE(0, mode=BSE_LADDER_SPLIT)
+    import numpy as np
+    errs = []
+    for r in range(4):
+        got = ec.download(GAMMA, r)
+        errs.append(float(np.linalg.norm(got - ref[r]) / np.linalg.norm(ref[r])))
+    ec.close()
+    print(f"[bench] split error per channel: {errs}", file=sys.stderr, flush=True)
 
     # crossing (calculate_PAE) at Nl = 1 on the 16x16 / Nf = 64 box: the HBM-bound regime
     from tups_amd.profile_workloads import crossing_bytes, crossing_context
@@ -156,15 +202,21 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": round(ms_sp / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "i8 (exact INT32 sums) + f64",
         "data": "synthetic",
         "config": {"workload": f"batched BSE ladder, 4 channels, D={D} (Nl={nl} x Nf={nf}), "
-                               f"Nq={nq} (64 momenta x 16 bosonic freq), FP64 path",
-                   "nx": nx, "nf": nf, "nl": nl, "nq": nq, "D": D, "path": "fp64-mfma"},
+                               f"Nq={nq} (64 momenta x 16 bosonic freq), split form ({args.moduli} CRT moduli)",
+                   "nx": nx, "nf": nf, "nl": nl, "nq": nq, "D": D, "path": "int8-crt-mfma",
+                   "moduli": args.moduli},
+        "split_error": {"max": max(errs), "per_channel": errs, "bar": 1e-10,
+                        "vs": "FP64 form of the same step, 64 q of the same workload"},
+        "fp64_path": {"value": round(qps64, 2), "unit": "q-points/s", "ms_per_step": round(ms64 / args.steps, 3),
+                      "roofline": roof64},
+        "split_vs_fp64": round(qps / qps64, 3),
         "roofline": roofline,
         "crossing_roofline": crossing,
         "cpu_baseline": cpu,

@@ -68,6 +68,8 @@ extern "C" {
 
 /* BSE modes */
 #define TUPS_BSE_LADDER_FP64 0  /* the reference's damped ladder, FP64 MFMA */
+#define TUPS_BSE_LADDER_SPLIT 1 /* same update; the GEMM runs as exact INT8 MFMA products of
+                                   residues modulo Gaussian moduli, recombined in FP64 */
 
 typedef struct tups_dims {
   int32_t nx;        /* Nx = Ny (even), SRC/parquet_ini.f90:304-305 */
@@ -126,6 +128,12 @@ int tups_upload_chi_dense(tups_ctx* ctx, int32_t pp, int32_t q0, int32_t nqs,
 /* --- the hot path (stream-ordered) */
 /* BSE(ite): Gamma_r <- Phi_r for r = d,m,s,t; SRC/parquet_BSE.f90:62-132 */
 int tups_bse(tups_ctx* ctx, int32_t mode);
+/* Split ladder settings (no counterpart in the reference): `moduli` CRT moduli
+ * (8..16, default 12: relative error ~2e-11 at Nz = 320; 13 ~1e-12; 16 FP64
+ * rounding), `qbatch` q points per residue+GEMM pass (0 = library default).
+ * The workspace holds 8 N P^2 + 8 P bytes per (q, channel) of one pass,
+ * P = Nz rounded up to a multiple of 64; the split mode needs Nz <= 1024. */
+int tups_set_split(tups_ctx* ctx, int32_t moduli, int32_t qbatch);
 /* calc_PhiR(ite): PhiR_r from Gamma_r (holding Phi_r); SRC/parquet_PhiR.f90:35-109 */
 int tups_phir(tups_ctx* ctx);
 /* calc_M_*: the four transformation tensors; SRC/parquet_EQMatrix.f90:21-360 */
@@ -162,6 +170,8 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 #define TUPS_K_PAE 3
 #define TUPS_K_GAMMA_UPDATE 4
 #define TUPS_K_PAE_GATHER 5   /* Nl = 1: PhiR re-indexed for contiguous crossing reads */
+#define TUPS_K_SPLIT_PREP 6   /* split ladder: operands scaled to integers and reduced to residues */
+#define TUPS_K_SPLIT_GEMM 7   /* split ladder: INT8 MFMA residue products, CRT, damping */
 #define TUPS_K_COUNT 8
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);

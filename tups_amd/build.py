@@ -26,7 +26,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["context.cpp", "tables.cpp", "bse.hip", "phir.hip", "eqmatrix.hip", "crossing.hip",
-           "vertex_ops.hip"]
+           "vertex_ops.hip", "bse_split.hip"]
 
 # Packed FP32 instructions are disabled for every device compile.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

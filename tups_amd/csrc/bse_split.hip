@@ -1,0 +1,473 @@
+// Split-precision BSE ladder: the complex GEMM of the ladder on INT8 MFMA.
+//
+// Same update as bse.hip (reference BSE(ite), SRC/parquet_BSE.f90:62-132):
+//   Phi = c_r f T + (1 - f)(F - Gamma),  T = Gamma (alpha chi F) + F (alpha chi Gamma)
+// written as one GEMM T = A B with A = [Gamma | F] (D x 2D) and
+// B = [alpha chi F ; alpha chi Gamma] (2D x D); chi is applied to B, whose
+// columns are contiguous in memory.
+//
+// Method: exact integer GEMM by residues (the modular form of the Ozaki
+// scheme, Ozaki, Uchino, Imamura 2025), with Gaussian moduli for the complex
+// product.
+//  1. Row i of A is scaled to integers A'_ik = rint(A_ik 2^(ta - e_i)), with
+//     2^(e_i - 1) <= max_k |A_ik| < 2^e_i, so |A'| <= 2^ta; column j of B
+//     likewise with tb and f_j.
+//  2. For pairwise coprime moduli m (all of them = 1 mod 4, each <= 241) with
+//     j^2 = -1 (mod m), a complex integer z = x + i y maps to the two residues
+//     (x + j y) mod m and (x - j y) mod m, and a complex product becomes two
+//     real products: (A'B')(+) = A'(+) B'(+), (A'B')(-) = A'(-) B'(-) (mod m).
+//     Symmetric residues lie in [-120, 120], so each product is one INT8 MFMA
+//     GEMM; its INT32 sums are exact for 2D * 120^2 < 2^31 (D <= 1024).
+//  3. Re C' = (P+ + P-)/2 and Im C' = (P+ - P-)/(2j) (mod m); the Chinese
+//     remainder theorem gives C' = sum_m u_m W_m (mod M), W_m = M/m ((M/m)^-1
+//     mod m), M = prod m.  ta + tb <= log2 M - log2(8D) - 1 keeps |C'| < M/4,
+//     so C' is the representative nearest zero.  The sum is held as an exact
+//     high part (W rounded to 41 significant bits, a multiple of 2^L) plus a
+//     low part, so C' = (S_hi - q M_hi) + (S_lo - q M_lo), q = rint(S / M),
+//     carries only the final rounding.
+//  4. T_ij = C'_ij 2^(e_i - ta + f_j - tb), then the damped mixing in FP64.
+// The error is that of rounding A and B to ta and tb bits relative to their
+// row and column maxima: 12 moduli (ta + tb = 79 at D = 320) give ~2e-11
+// relative Frobenius error, 16 moduli reach FP64 rounding.
+#include "internal.h"
+
+#include <cmath>
+
+namespace tups {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int CRT_MAX = 16;      // most moduli
+constexpr int CRT_TILE = 64;     // output tile of one workgroup
+constexpr int CRT_ROW = 80;      // padded LDS row of a 64-byte K chunk
+
+// pairwise coprime, all = 1 (mod 4): 221 = 13 * 17, 205 = 5 * 41, the rest prime
+constexpr int kModuli[CRT_MAX] = {241, 233, 229, 221, 205, 197, 193, 181,
+                                  173, 157, 149, 137, 113, 109, 101, 97};
+
+struct CrtArgs {
+  const double2* gam[4];
+  const double2* fv[4];
+  double2* out[4];
+  const double2* chi[2];
+  int8_t* A;                    // [mat][mod][sign][Dp rows][Kp], K contiguous
+  int8_t* B;                    // [mat][mod][sign][Dp cols][Kp], K contiguous
+  int* escale;                  // [mat][Dp] row exponents of A
+  int* fscale;                  // [mat][Dp] column exponents of B
+  int D, Dp, Kp, nf, nl;
+  int q0, nqb, tiles_n;
+  int nmod, ta, tb;
+  double alpha;
+  double cf[4];
+  double omf;
+  double m[CRT_MAX], inv_m[CRT_MAX], j[CRT_MAX], c26[CRT_MAX];
+  double inv2[CRT_MAX], inv2j[CRT_MAX], whi[CRT_MAX], wlo[CRT_MAX];
+  double mhi, mlo, inv_mm;
+};
+
+__device__ __forceinline__ size_t ws_off(const CrtArgs& a, int mat, int mod, int sign, int rc) {
+  return (((size_t)mat * a.nmod + mod) * 2 + sign) * (size_t)a.Dp * a.Kp + (size_t)rc * a.Kp;
+}
+
+// symmetric residue of an integer-valued y (|y| < 2^50): exact, |r| <= (m - 1) / 2
+__device__ __forceinline__ double sym_mod(double y, double m, double inv_m) {
+  return fma(-m, rint(y * inv_m), y);
+}
+
+// x (integer valued, |x| <= 2^52) = hi 2^26 + lo, 0 <= lo < 2^26
+__device__ __forceinline__ void limbs(double x, double& hi, double& lo) {
+  hi = floor(x * 0x1p-26);
+  lo = fma(-0x1p26, hi, x);
+}
+
+// the (+, -) residues of the complex integer (xr, xi) given as limbs, as bytes
+__device__ __forceinline__ void gauss_residues(const CrtArgs& a, int mod, double r1, double r0,
+                                               double i1, double i0, int& rp, int& rm) {
+  const double m = a.m[mod], im = a.inv_m[mod], jj = a.j[mod], c = a.c26[mod];
+  const double yr = fma(r1, c, r0), yi = fma(i1, c, i0);      // |y| < 2^35
+  const double yp = fma(jj, yi, yr), ym = fma(-jj, yi, yr);  // |y| < 2^43
+  rp = (int)sym_mod(yp, m, im) & 0xff;
+  rm = (int)sym_mod(ym, m, im) & 0xff;
+}
+
+// A = [Gamma | F]: one lane per row; wave w converts inner indices
+// 16 w + 64 n .. + 15 and stores 16 bytes per (modulus, sign).
+// grid: (Dp / 64, 4 * nqb)
+__global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
+  __shared__ double red[4][64];
+  const int mat = blockIdx.y;  // r * nqb + qb
+  const int r = mat / a.nqb, qb = mat - r * a.nqb, q = a.q0 + qb;
+  const int D = a.D;
+  const size_t DD = (size_t)D * D;
+  const double2* G = a.gam[r] + DD * q;
+  const double2* F = a.fv[r] + DD * q;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = blockIdx.x * 64 + lane;
+  const bool rok = row < D;
+  auto val = [&](int k) -> double2 {
+    if (!rok || k >= 2 * D) return make_double2(0.0, 0.0);
+    return k < D ? G[(size_t)k * D + row] : F[(size_t)(k - D) * D + row];
+  };
+  const int kq = a.Kp / 4;
+  double mx = 0.0;
+  for (int k = wave * kq; k < (wave + 1) * kq; ++k) {
+    const double2 v = val(k);
+    mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+  }
+  red[wave][lane] = mx;
+  __syncthreads();
+  mx = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+  int e = 0;
+  if (mx > 0.0) frexp(mx, &e);
+  if (wave == 0) a.escale[(size_t)mat * a.Dp + row] = e;
+  const double sc = ldexp(1.0, a.ta - e);
+  for (int k0 = wave * 16; k0 < a.Kp; k0 += 64) {
+    double r1[16], r0[16], i1[16], i0[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const double2 v = val(k0 + u);
+      limbs(rint(v.x * sc), r1[u], r0[u]);
+      limbs(rint(v.y * sc), i1[u], i0[u]);
+    }
+    for (int mod = 0; mod < a.nmod; ++mod) {
+      int wp[4] = {0, 0, 0, 0}, wm[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        int bp, bm;
+        gauss_residues(a, mod, r1[u], r0[u], i1[u], i0[u], bp, bm);
+        wp[u >> 2] |= bp << (8 * (u & 3));
+        wm[u >> 2] |= bm << (8 * (u & 3));
+      }
+      *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 0, row) + k0) = v4i{wp[0], wp[1], wp[2], wp[3]};
+      *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 1, row) + k0) = v4i{wm[0], wm[1], wm[2], wm[3]};
+    }
+  }
+}
+
+// B = [alpha chi F ; alpha chi Gamma]: one wave per column; lane l converts
+// inner indices 4 l + 256 n .. + 3 and stores one dword per (modulus, sign).
+// grid: (Dp / 4, 4 * nqb)
+__global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
+  const int mat = blockIdx.y;
+  const int r = mat / a.nqb, qb = mat - r * a.nqb, q = a.q0 + qb;
+  const int D = a.D, nf = a.nf, nl = a.nl;
+  const size_t DD = (size_t)D * D;
+  const double2* G = a.gam[r] + DD * q;
+  const double2* F = a.fv[r] + DD * q;
+  const double2* chiq = a.chi[r >> 1] + (size_t)nf * nl * nl * q;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = blockIdx.x * 4 + wave;
+  const bool cok = col < D;
+  // B(k, col), k = part D + l1 Nf + nu: alpha sum_l3 chi_nu(l1, l3) X((l3, nu), col)
+  auto mixed = [&](int k) -> double2 {
+    if (!cok || k >= 2 * D) return make_double2(0.0, 0.0);
+    const int part = k >= D;
+    const int kk = k - part * D;
+    const int l1 = kk / nf, nu = kk - l1 * nf;
+    const double2* X = (part ? G : F) + (size_t)col * D + nu;
+    const double2* cb = chiq + (size_t)nu * nl * nl + l1;
+    double sr = 0.0, si = 0.0;
+    for (int l3 = 0; l3 < nl; ++l3) {
+      const double2 c = cb[nl * l3];
+      const double2 x = X[l3 * nf];
+      sr = fma(c.x, x.x, fma(-c.y, x.y, sr));
+      si = fma(c.x, x.y, fma(c.y, x.x, si));
+    }
+    return make_double2(a.alpha * sr, a.alpha * si);
+  };
+  double mx = 0.0;
+  for (int k = lane; k < 2 * D; k += 64) {
+    const double2 v = mixed(k);
+    mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  int f = 0;
+  if (mx > 0.0) frexp(mx, &f);
+  if (lane == 0) a.fscale[(size_t)mat * a.Dp + col] = f;
+  const double sc = ldexp(1.0, a.tb - f);
+  for (int k0 = 4 * lane; k0 < a.Kp; k0 += 256) {
+    double r1[4], r0[4], i1[4], i0[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 v = mixed(k0 + u);
+      limbs(rint(v.x * sc), r1[u], r0[u]);
+      limbs(rint(v.y * sc), i1[u], i0[u]);
+    }
+    for (int mod = 0; mod < a.nmod; ++mod) {
+      int wp = 0, wm = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int bp, bm;
+        gauss_residues(a, mod, r1[u], r0[u], i1[u], i0[u], bp, bm);
+        wp |= bp << (8 * u);
+        wm |= bm << (8 * u);
+      }
+      *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 0, col) + k0) = wp;
+      *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 1, col) + k0) = wm;
+    }
+  }
+}
+
+// One workgroup per 64 x 64 tile of one (q, r); four waves, each 32 x 32 =
+// 2 x 2 blocks of v_mfma_i32_16x16x64_i8.  The loop runs over (modulus, K chunk)
+// steps with the next step's A and B chunks prefetched into registers and
+// double-buffered in LDS; after a modulus' last chunk its residues join the
+// CRT sums, held per output element as exact high and rounded low parts.
+__global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
+  __shared__ __attribute__((aligned(16))) int8_t As[2][2][CRT_TILE][CRT_ROW];
+  __shared__ __attribute__((aligned(16))) int8_t Bs[2][2][CRT_TILE][CRT_ROW];
+  const int tiles = a.tiles_n * a.tiles_n;
+  const int bid = blockIdx.x;
+  const int mat = bid / tiles;
+  const int t = bid - mat * tiles;
+  const int row0 = (t / a.tiles_n) * CRT_TILE, col0 = (t % a.tiles_n) * CRT_TILE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nk = a.Kp / 64;
+  const int nsteps = a.nmod * nk;
+  // staging: thread tid moves 16 bytes of row (tid >> 2), chunk (tid & 3), for
+  // piece v = 2 * which + sign (which 0: A rows, 1: B columns)
+  const int srow = tid >> 2, sch = (tid & 3) * 16;
+  auto gptr = [&](int step, int v) -> const v4i* {
+    const int md = step / nk, k0 = (step - md * nk) * 64;
+    const int which = v >> 1, sign = v & 1;
+    const int8_t* base = which ? a.B : a.A;
+    const int rc = (which ? col0 : row0) + srow;
+    return reinterpret_cast<const v4i*>(base + ws_off(a, mat, md, sign, rc) + k0 + sch);
+  };
+  auto lds_put = [&](int buf, const v4i* pf) {
+    *reinterpret_cast<v4i*>(&As[buf][0][srow][sch]) = pf[0];
+    *reinterpret_cast<v4i*>(&As[buf][1][srow][sch]) = pf[1];
+    *reinterpret_cast<v4i*>(&Bs[buf][0][srow][sch]) = pf[2];
+    *reinterpret_cast<v4i*>(&Bs[buf][1][srow][sch]) = pf[3];
+  };
+  v4i pf[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) pf[v] = *gptr(0, v);
+  lds_put(0, pf);
+  __syncthreads();
+
+  double shr[16], slr[16], shi[16], sli[16];  // CRT sums, element (i, j, e) -> 8 i + 4 j + e
+#pragma unroll
+  for (int x = 0; x < 16; ++x) shr[x] = slr[x] = shi[x] = sli[x] = 0.0;
+  v4i accp[2][2], accm[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) accp[i][j] = accm[i][j] = v4i{0, 0, 0, 0};
+
+  int kstep = 0, mod = 0;
+  for (int step = 0; step < nsteps; ++step) {
+    const int buf = step & 1;
+    const bool more = step + 1 < nsteps;
+    if (more) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) pf[v] = *gptr(step + 1, v);
+    }
+    v4i ap[2], am[2], bp[2], bm[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      ap[i] = *reinterpret_cast<const v4i*>(&As[buf][0][wr + i * 16 + li][16 * lk]);
+      am[i] = *reinterpret_cast<const v4i*>(&As[buf][1][wr + i * 16 + li][16 * lk]);
+      bp[i] = *reinterpret_cast<const v4i*>(&Bs[buf][0][wc + i * 16 + li][16 * lk]);
+      bm[i] = *reinterpret_cast<const v4i*>(&Bs[buf][1][wc + i * 16 + li][16 * lk]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        accp[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ap[i], bp[j], accp[i][j], 0, 0, 0);
+        accm[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(am[i], bm[j], accm[i][j], 0, 0, 0);
+      }
+    if (++kstep == nk) {
+      // residues of this modulus -> CRT sums
+      const double m = a.m[mod], im = a.inv_m[mod], h2 = a.inv2[mod], h2j = a.inv2j[mod];
+      const double wh = a.whi[mod], wl = a.wlo[mod];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int x = 8 * i + 4 * j + e;
+            const int p = accp[i][j][e], n = accm[i][j][e];
+            const double ur = sym_mod((double)(p + n) * h2, m, im);
+            const double ui = sym_mod((double)(p - n) * h2j, m, im);
+            shr[x] = fma(ur, wh, shr[x]);
+            slr[x] = fma(ur, wl, slr[x]);
+            shi[x] = fma(ui, wh, shi[x]);
+            sli[x] = fma(ui, wl, sli[x]);
+          }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) accp[i][j] = accm[i][j] = v4i{0, 0, 0, 0};
+      kstep = 0;
+      ++mod;
+    }
+    if (more) lds_put(buf ^ 1, pf);
+    __syncthreads();
+  }
+
+  // C' = (S_hi - q M_hi) + (S_lo - q M_lo); T = C' 2^(e_i - ta + f_j - tb);
+  // Phi = c f T + (1 - f)(F - Gamma).  Element e of lane l of a 16 x 16 INT32
+  // block sits at row 4 (l / 16) + e, column l % 16.
+  const int D = a.D;
+  const size_t DD = (size_t)D * D;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  const double2* Gm = a.gam[r] + DD * q;
+  const double2* Fm = a.fv[r] + DD * q;
+  double2* O = a.out[r] + DD * q;
+  const double cf = a.cf[r];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int gj = col0 + wc + j * 16 + li;
+      if (gj >= D) continue;
+      const int fj = a.fscale[(size_t)mat * a.Dp + gj];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int gi = row0 + wr + i * 16 + 4 * lk + e;
+        if (gi >= D) continue;
+        const int x = 8 * i + 4 * j + e;
+        const double qr = rint((shr[x] + slr[x]) * a.inv_mm);
+        const double qi = rint((shi[x] + sli[x]) * a.inv_mm);
+        const double cr = fma(-qr, a.mhi, shr[x]) + fma(-qr, a.mlo, slr[x]);
+        const double ci = fma(-qi, a.mhi, shi[x]) + fma(-qi, a.mlo, sli[x]);
+        const double s = ldexp(cf, a.escale[(size_t)mat * a.Dp + gi] - a.ta + fj - a.tb);
+        const size_t idx = (size_t)gj * D + gi;
+        const double2 fv = Fm[idx], gv = Gm[idx];
+        O[idx] = make_double2(fma(s, cr, a.omf * (fv.x - gv.x)), fma(s, ci, a.omf * (fv.y - gv.y)));
+      }
+    }
+}
+
+// ---- host: moduli constants ---------------------------------------------------------
+
+typedef unsigned __int128 u128;
+
+static int bitlen(u128 x) {
+  int n = 0;
+  while (x) {
+    ++n;
+    x >>= 1;
+  }
+  return n;
+}
+
+// nearest double (ties away) of an integer below 2^128
+static double u128_to_double(u128 x) {
+  const int n = bitlen(x);
+  if (n <= 53) return (double)(unsigned long long)x;
+  const int sh = n - 53;
+  u128 top = x >> sh;
+  const u128 rem = x - (top << sh);
+  if (rem >= ((u128)1 << (sh - 1))) top += 1;
+  return std::ldexp((double)(unsigned long long)top, sh);
+}
+
+static long long inv_mod(long long x, long long m) {  // x^-1 mod m, gcd(x, m) = 1
+  const long long v = ((x % m) + m) % m;
+  for (long long y = 1; y < m; ++y)
+    if (v * y % m == 1) return y;
+  return 0;
+}
+
+static void crt_constants(CrtArgs& a, int nmod, int D) {
+  a.nmod = nmod;
+  u128 M = 1;
+  for (int i = 0; i < nmod; ++i) M *= (u128)kModuli[i];
+  const int L = std::max(0, bitlen(M) - 41);
+  for (int i = 0; i < nmod; ++i) {
+    const long long m = kModuli[i];
+    long long j = 1;
+    while ((j * j + 1) % m) ++j;
+    const u128 Mi = M / (u128)m;
+    const long long yi = inv_mod((long long)(Mi % (u128)m), m);
+    const u128 W = Mi * (u128)yi;  // < M
+    a.m[i] = (double)m;
+    a.inv_m[i] = 1.0 / (double)m;
+    a.j[i] = (double)j;
+    a.c26[i] = (double)(((long long)1 << 26) % m);
+    a.inv2[i] = (double)((m + 1) / 2);
+    a.inv2j[i] = (double)inv_mod(2 * j, m);
+    a.whi[i] = std::ldexp((double)(unsigned long long)(W >> L), L);  // < 2^41 significant
+    a.wlo[i] = u128_to_double(W - ((W >> L) << L));
+  }
+  a.mhi = std::ldexp((double)(unsigned long long)(M >> L), L);
+  a.mlo = u128_to_double(M - ((M >> L) << L));
+  a.inv_mm = 1.0 / u128_to_double(M);
+  // |C'| <= 2 (2D) 2^(ta + tb) <= M / 4
+  const double log2M = std::log2(u128_to_double(M));
+  const int tsum = (int)std::floor(log2M - std::log2(8.0 * D) - 1.0);
+  a.ta = std::min(52, tsum / 2);
+  a.tb = std::min(52, tsum - tsum / 2);
+}
+
+int launch_bse_split(tups_ctx* c) {
+  const tups_sizes& s = c->sz;
+  const int nmod = c->split_moduli;
+  TUPS_CHECK(nmod >= 8 && nmod <= CRT_MAX, TUPS_EINVAL, "split moduli must be in 8..16");
+  TUPS_CHECK(s.nz <= 1024, TUPS_EINVAL, "split path needs Nz <= 1024 (exact INT32 sums)");
+  const size_t Dp = ((size_t)s.nz + 63) / 64 * 64;
+  const size_t Kp = (2 * (size_t)s.nz + 63) / 64 * 64;
+  // per q point: 4 channels x (A and B) x 2 signs x nmod residue planes + exponents
+  const size_t per_q = 4 * (4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int));
+  // default pass: about 1 GiB of residues
+  const size_t want = c->split_qbatch > 0 ? (size_t)c->split_qbatch : ((size_t)1 << 30) / per_q;
+  const int qbatch = (int)std::max<size_t>(1, std::min<size_t>(s.nq, want));
+  const size_t need = per_q * qbatch;
+  if (c->split_ws_bytes < need) {
+    if (c->split_ws) TUPS_HIP(hipFree(c->split_ws));
+    c->split_ws = nullptr;
+    c->split_ws_bytes = 0;
+    TUPS_HIP(hipMalloc(&c->split_ws, need));
+    c->split_ws_bytes = need;
+  }
+  CrtArgs a{};
+  for (int r = 0; r < 4; ++r) {
+    a.gam[r] = c->gamma[r];
+    a.fv[r] = c->fvert[r];
+    a.out[r] = c->spare[r];
+  }
+  a.chi[0] = c->chi[0];
+  a.chi[1] = c->chi[1];
+  a.D = s.nz;
+  a.Dp = (int)Dp;
+  a.Kp = (int)Kp;
+  a.nf = s.nf;
+  a.nl = s.nl;
+  a.tiles_n = (int)(Dp / CRT_TILE);
+  a.alpha = s.alpha;
+  const double cr[4] = {0.5, 0.5, -0.25, 0.25};
+  for (int r = 0; r < 4; ++r) a.cf[r] = cr[r] * c->f_damping;
+  a.omf = 1.0 - c->f_damping;
+  crt_constants(a, nmod, s.nz);
+  for (int q0 = 0; q0 < s.nq; q0 += qbatch) {
+    a.q0 = q0;
+    a.nqb = std::min(qbatch, s.nq - q0);
+    const size_t mats = 4 * (size_t)a.nqb;
+    int8_t* base = static_cast<int8_t*>(c->split_ws);
+    a.A = base;
+    a.B = base + mats * 2 * nmod * Dp * Kp;
+    a.escale = reinterpret_cast<int*>(a.B + mats * 2 * nmod * Dp * Kp);
+    a.fscale = a.escale + mats * Dp;
+    if (int rc = timer_begin(c)) return rc;
+    hipLaunchKernelGGL(crt_prep_a, dim3((unsigned)(Dp / 64), (unsigned)mats), dim3(256), 0, c->stream, a);
+    TUPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(crt_prep_b, dim3((unsigned)(Dp / 4), (unsigned)mats), dim3(256), 0, c->stream, a);
+    TUPS_HIP(hipGetLastError());
+    if (int rc = timer_end(c, TUPS_K_SPLIT_PREP)) return rc;
+    const long long nblk = (long long)a.tiles_n * a.tiles_n * mats;
+    TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "split grid too large");
+    if (int rc = timer_begin(c)) return rc;
+    hipLaunchKernelGGL(crt_gemm, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+    TUPS_HIP(hipGetLastError());
+    if (int rc = timer_end(c, TUPS_K_SPLIT_GEMM)) return rc;
+  }
+  return TUPS_OK;
+}
+
+}  // namespace tups

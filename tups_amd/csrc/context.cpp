@@ -285,6 +285,7 @@ int tups_destroy(tups_ctx* c) {
   }
   f(c->chi[0]);
   f(c->chi[1]);
+  f(c->split_ws);
   f(c->d_ffk);
   f(c->d_ffik);
   f(c->d_eqsr);
@@ -416,14 +417,24 @@ int tups_upload_chi_dense(tups_ctx* c, int32_t pp, int32_t q0, int32_t nqs, cons
 
 int tups_bse(tups_ctx* c, int32_t mode) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
-  TUPS_CHECK(mode == TUPS_BSE_LADDER_FP64, TUPS_EINVAL, "unknown BSE mode");
+  TUPS_CHECK(mode == TUPS_BSE_LADDER_FP64 || mode == TUPS_BSE_LADDER_SPLIT, TUPS_EINVAL,
+             "unknown BSE mode");
   int rc = ensure_tables(c);
   if (rc) return rc;
-  rc = launch_bse_ladder(c);
+  rc = mode == TUPS_BSE_LADDER_SPLIT ? launch_bse_split(c) : launch_bse_ladder(c);
   if (rc) return rc;
   // Phi overwrites G_r_LL (SRC/parquet_BSE.f90:81): swap the ping-pong pair
   for (int r = 0; r < 4; ++r) std::swap(c->gamma[r], c->spare[r]);
   c->phir_valid = false;
+  return TUPS_OK;
+}
+
+int tups_set_split(tups_ctx* c, int32_t moduli, int32_t qbatch) {
+  TUPS_CHECK(c, TUPS_EINVAL, "null context");
+  TUPS_CHECK(moduli >= 8 && moduli <= 16, TUPS_EINVAL, "split moduli must be in 8..16");
+  TUPS_CHECK(qbatch >= 0, TUPS_EINVAL, "qbatch must be >= 0");
+  c->split_moduli = moduli;
+  c->split_qbatch = qbatch;
   return TUPS_OK;
 }
 

@@ -98,6 +98,10 @@ struct tups_ctx {
 
   int bse_variant = -1;          // kernel variant chosen by the autotuner (-1: default)
   int pae_variant = -1;
+  int split_moduli = 12;         // split ladder: CRT moduli (8..16)
+  int split_qbatch = 0;          // split ladder: q points per pass (0: default)
+  void* split_ws = nullptr;      // split ladder: slices and exponents of one pass
+  size_t split_ws_bytes = 0;
   bool timing = false;
   hipEvent_t tev[2] = {};
   double t_ms[TUPS_K_COUNT] = {};
@@ -111,6 +115,7 @@ struct tups_ctx {
 namespace tups {
 // kernels launchers (defined in the .hip sources)
 int launch_bse_ladder(tups_ctx* c);
+int launch_bse_split(tups_ctx* c);
 int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms);
 const char* bse_variant_name(int v);
 int bse_variant_count();

@@ -1,6 +1,7 @@
 """Named workloads for rocprofv3 runs of the hot-path kernels.
 
     python -m tups_amd.profile_workloads bse      --steps 3   # batched BSE, D=320, Nq=1024
+    python -m tups_amd.profile_workloads split    --steps 3   # same, split (INT8 CRT) form
     python -m tups_amd.profile_workloads crossing --steps 5   # PAE crossing, Nl=1, Nx=16, Nf=64
     python -m tups_amd.profile_workloads phir     --steps 5   # PhiR, Nl=1, Nx=16, Nf=64
     python -m tups_amd.profile_workloads calib    --steps 3   # G = F - G streaming kernel
@@ -17,7 +18,7 @@ import sys
 
 import numpy as np
 
-from .runtime import GAMMA, LAMBDA, TupsContext
+from .runtime import BSE_LADDER_SPLIT, GAMMA, LAMBDA, TupsContext
 from .synth import batched_bse_inputs
 
 
@@ -68,19 +69,27 @@ def crossing_flops(c: TupsContext) -> float:
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["bse", "crossing", "phir", "calib"])
+    ap.add_argument("workload", choices=["bse", "split", "crossing", "phir", "calib"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--nx", type=int, default=16)
     ap.add_argument("--nf", type=int, default=64)
     ap.add_argument("--nl", type=int, default=1)
+    ap.add_argument("--moduli", type=int, default=12)
     args = ap.parse_args(argv)
-    if args.workload in ("bse", "calib"):
+    if args.workload in ("bse", "split", "calib"):
         c = TupsContext(nx=8, nf=64, nl=5, nq=1024, beta=2.0, f_damping=0.5)
         batched_bse_inputs(c)
         if args.workload == "bse":
+            name, _ = c.autotune()
             for _ in range(args.steps):
                 c.BSE(0)
-            print(f"bse: {args.steps} launches, flops/launch {16.0 * c.nz ** 3 * 4 * c.nq:.4e}")
+            print(f"bse: variant {name}, {args.steps} launches, flops/launch {16.0 * c.nz ** 3 * 4 * c.nq:.4e}")
+        elif args.workload == "split":
+            c.set_split(args.moduli)
+            for _ in range(args.steps):
+                c.BSE(0, mode=BSE_LADDER_SPLIT)
+            ops = 2.0 * args.moduli * 2.0 * c.nz ** 2 * (2 * c.nz) * 4 * c.nq
+            print(f"split: {args.moduli} moduli, {args.steps} steps, INT8 ops/step {ops:.4e}")
         else:
             for _ in range(args.steps):
                 c.gamma_update()

@@ -20,9 +20,10 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
 
 GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
-K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER = range(6)
+K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER, K_SPLIT_PREP, K_SPLIT_GEMM = range(8)
 CH_D, CH_M, CH_S, CH_T = range(4)
 BSE_LADDER_FP64 = 0
+BSE_LADDER_SPLIT = 1
 M_NAMES = ("phbar_to_ph", "pp_to_ph", "ph_to_pp", "phbar_to_pp")
 
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE"}
@@ -64,6 +65,7 @@ SIGNATURES = {
     "tups_download": ([_V, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_upload_chi_dense": ([_V, C.c_int32, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_bse": ([_V, C.c_int32], C.c_int),
+    "tups_set_split": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_phir": ([_V], C.c_int),
     "tups_build_m": ([_V], C.c_int),
     "tups_pae": ([_V], C.c_int),
@@ -216,6 +218,10 @@ class TupsContext:
     def BSE(self, ite: int = 0, mode: int = BSE_LADDER_FP64):
         """BSE(ite), SRC/parquet_BSE.f90:19-135: G_r_LL <- Phi_r for r = d, m, s, t."""
         _check(self.lib.tups_bse(self._h, mode), "tups_bse")
+
+    def set_split(self, moduli: int = 12, qbatch: int = 0):
+        """CRT moduli and q points per pass of the split (INT8 MFMA) ladder."""
+        _check(self.lib.tups_set_split(self._h, moduli, qbatch), "tups_set_split")
 
     def calc_PhiR(self, ite: int = 0):
         """calc_PhiR(ite), SRC/parquet_PhiR.f90:35-109."""
