@@ -84,8 +84,8 @@ def main():
               "of this path is a design note, not a run", file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from tups_amd import BSE_LADDER_FP64, BSE_LADDER_SPLIT, GAMMA, TupsContext
-    from tups_amd.runtime import K_BSE_LADDER, K_SPLIT_GEMM, K_SPLIT_PREP
+    from tups_amd import BSE_CLOSED_LU, BSE_LADDER_FP64, BSE_LADDER_SPLIT, GAMMA, TupsContext
+    from tups_amd.runtime import K_BSE_CLOSED, K_BSE_LADDER, K_SPLIT_GEMM, K_SPLIT_PREP
     from tups_amd.synth import batched_bse_inputs
 
     nx, nf, nl, U = 8, 64, 5, 2.0
@@ -147,6 +147,23 @@ def main():
                 "kernel": f"crt_gemm ({args.moduli} moduli)", "avg_launch_ms": round(gemm_ms / gemm_n, 4),
                 "ops_per_launch": ops_per_launch, "prep_ms_per_launch": round(prep_ms / prep_n, 4),
                 "launches_per_step": gemm_n // nprof}
+    # --- closed form (fixed point by pivoted Gauss-Jordan) --------------------------------
+    batched_bse_inputs(ctx, U=U, seed=0)
+    ctx.BSE(0, mode=BSE_CLOSED_LU)  # warm-up (allocates its workspace)
+    ctx.sync()
+    ncl = max(1, min(args.steps, 2))
+    ctx.set_timing(True)
+    for _ in range(ncl):
+        ctx.BSE(0, mode=BSE_CLOSED_LU)
+    cl_ms, cl_n = ctx.kernel_time(K_BSE_CLOSED)
+    ctx.set_timing(False)
+    cl_step_ms = cl_ms / ncl
+    cl_flops = 12.0 * D ** 3 * 4 * nq  # Gauss-Jordan on D right-hand sides, 4 channels
+    closed = {"value": round(nq / (cl_step_ms / 1e3), 2), "unit": "q-points/s", "ms_per_call": round(cl_step_ms, 3),
+              "achieved": round(cl_flops / (cl_step_ms / 1e3) / 1e12, 3), "peak": FP64_PEAK_TFLOPS,
+              "unit_roofline": "TFLOP/s", "frac": round(cl_flops / (cl_step_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+              "flops_per_call": cl_flops, "kernel": "closed_build + closed_solve"}
+    print(f"[bench] closed form: {cl_step_ms:.1f} ms per call", file=sys.stderr, flush=True)
     ctx.close()
 
     # split error per channel vs the FP64 form, on 64 q of the same workload
@@ -217,6 +234,7 @@ def main():
         "fp64_path": {"value": round(qps64, 2), "unit": "q-points/s", "ms_per_step": round(ms64 / args.steps, 3),
                       "roofline": roof64},
         "split_vs_fp64": round(qps / qps64, 3),
+        "closed_form": closed,
         "roofline": roofline,
         "crossing_roofline": crossing,
         "cpu_baseline": cpu,

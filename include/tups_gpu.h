@@ -70,6 +70,10 @@ extern "C" {
 #define TUPS_BSE_LADDER_FP64 0  /* the reference's damped ladder, FP64 MFMA */
 #define TUPS_BSE_LADDER_SPLIT 1 /* same update; the GEMM runs as exact INT8 MFMA products of
                                    residues modulo Gaussian moduli, recombined in FP64 */
+#define TUPS_BSE_CLOSED_LU 2    /* closed form: the ladder's fixed point for fixed Gamma,
+                                   Phi = (I - kappa_r Gamma chi)^-1 Gamma - Gamma, kappa_r =
+                                   2 c_r alpha, by pivoted Gauss-Jordan (FP64 MFMA updates);
+                                   returns TUPS_ENONFINITE on a zero pivot column */
 
 typedef struct tups_dims {
   int32_t nx;        /* Nx = Ny (even), SRC/parquet_ini.f90:304-305 */
@@ -128,12 +132,29 @@ int tups_upload_chi_dense(tups_ctx* ctx, int32_t pp, int32_t q0, int32_t nqs,
 /* --- the hot path (stream-ordered) */
 /* BSE(ite): Gamma_r <- Phi_r for r = d,m,s,t; SRC/parquet_BSE.f90:62-132 */
 int tups_bse(tups_ctx* ctx, int32_t mode);
+/* calc_chi for every q slot (SRC/parquet_chi.f90:438-523 with get_green_coarse,
+ * SRC/parquet_util.f90:1303-1327): fills the resident TUPS_CHI bubbles (ph:
+ * channel 0, pp: channel 1) from the self-energy.  sigma: Sigma(Nt) complex,
+ * Nt = Nx Ny Nf, index ((kx-1) Ny + ky-1) Nf + nu (the reference's layout);
+ * sigma_h: Sigma_H(Nx, Ny) complex column-major, used outside the frequency box,
+ * or NULL for the reference's rule Sigma_H(kx, ky) = Sigma(k, Nf)
+ * (SRC/parquet_chi.f90:103-110); mu the chemical potential; ngrain the
+ * coarse-graining Ngrain.  Synchronous w.r.t. the host buffers. */
+int tups_calc_chi(tups_ctx* ctx, const double* sigma, const double* sigma_h, double mu, int32_t ngrain);
+
 /* Split ladder settings (no counterpart in the reference): `moduli` CRT moduli
  * (8..16, default 12: relative error ~2e-11 at Nz = 320; 13 ~1e-12; 16 FP64
  * rounding), `qbatch` q points per residue+GEMM pass (0 = library default).
  * The workspace holds 8 N P^2 + 8 P bytes per (q, channel) of one pass,
  * P = Nz rounded up to a multiple of 64; the split mode needs Nz <= 1024. */
 int tups_set_split(tups_ctx* ctx, int32_t moduli, int32_t qbatch);
+/* One iteration of the path (no counterpart as one call; SRC/loop.f90:78-139,
+ * 219-222 calls the steps one by one): BSE(mode) -> calc_PhiR -> calculate_PAE
+ * -> G = F - G, for the ladder modes.  use_graph != 0 captures the sequence
+ * into a HIP graph on first use (per mode and vertex-buffer parity) and replays
+ * it afterwards.  Needs a full-IBZ context with the M tensors built. */
+int tups_iteration(tups_ctx* ctx, int32_t mode, int32_t use_graph);
+
 /* calc_PhiR(ite): PhiR_r from Gamma_r (holding Phi_r); SRC/parquet_PhiR.f90:35-109 */
 int tups_phir(tups_ctx* ctx);
 /* calc_M_*: the four transformation tensors; SRC/parquet_EQMatrix.f90:21-360 */
@@ -172,7 +193,9 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 #define TUPS_K_PAE_GATHER 5   /* Nl = 1: PhiR re-indexed for contiguous crossing reads */
 #define TUPS_K_SPLIT_PREP 6   /* split ladder: operands scaled to integers and reduced to residues */
 #define TUPS_K_SPLIT_GEMM 7   /* split ladder: INT8 MFMA residue products, CRT, damping */
-#define TUPS_K_COUNT 8
+#define TUPS_K_BSE_CLOSED 8   /* closed form: build of [I - kappa Gamma chi | Gamma] and its solve */
+#define TUPS_K_CALC_CHI 9     /* bubbles chi_ph, chi_pp for every q slot */
+#define TUPS_K_COUNT 12
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);
 /* scans the resident array for NaN/Inf; returns TUPS_ENONFINITE if any */

@@ -13,7 +13,11 @@ Pinned by (tests/test_oracle_bse.py):
   damped update returns exactly F - Gamma for every damping f;
 * the ite-0 special case F = Gamma: Phi = 2 f c_r alpha Gamma chi Gamma;
 * calc_chi against a brute-force triple loop on a tiny grid, and the Nl=1
-  closed form chi(nu,nu) = (1/Nc) sum_k G G.
+  closed form chi(nu,nu) = (1/Nc) sum_k G G;
+* bse_closed (the closed form of the ladder, no reference counterpart: the
+  reference only iterates the damped update) by the same fixed-point identity
+  run the other way: the ladder applied to F = Gamma + Phi_closed returns
+  Phi_closed for every damping f.
 """
 from __future__ import annotations
 
@@ -58,6 +62,27 @@ def bse_ladder(lat: Lattice, gamma, F, chi, beta: float, f_damping: float):
     return out
 
 
+def bse_closed(lat: Lattice, gamma, chi, beta: float):
+    """Fixed point of BSE(ite) for fixed Gamma: Phi_r = (I - kappa_r Gamma_r chi)^-1 Gamma_r - Gamma_r,
+    kappa_r = 2 c_r alpha (c_r of SRC/parquet_BSE.f90:73,89,105,120), by LAPACK (numpy.linalg.solve).
+
+    gamma: 4 arrays (Nz, Nz, Nq); chi: (chi_ph, chi_pp) dense (Nz, Nz, Nq).
+    """
+    alpha = 1.0 / (beta * lat.nx * lat.ny)
+    out = []
+    for r in range(4):
+        kappa = 2.0 * C_CHANNEL[r] * alpha
+        g = gamma[r]
+        ch = chi[CHI_OF_CHANNEL[r]]
+        phi = np.empty_like(g, order="F")
+        eye = np.eye(g.shape[0])
+        for q in range(g.shape[2]):
+            a = eye - kappa * (g[:, :, q] @ ch[:, :, q])
+            phi[:, :, q] = np.linalg.solve(a, g[:, :, q]) - g[:, :, q]
+        out.append(phi)
+    return out
+
+
 def chi_compact_to_dense(lat: Lattice, chi_c: np.ndarray) -> np.ndarray:
     """(Nl, Nl, Nf, Nq) block-diagonal bubble -> dense (Nz, Nz, Nq) (Fortran order)."""
     nl, nf, nq = lat.nl, lat.nf, chi_c.shape[3]
@@ -85,17 +110,21 @@ def ek_grain(lat: Lattice, ngrain: int) -> np.ndarray:
     return out
 
 
-def green_coarse(lat: Lattice, ekg, sigma, mu, beta, gx, gy, kx, ky, nu):
-    """get_green_coarse for 1-based (kx, ky, nu), 0-based grain (gx, gy); Sigma_H = 0."""
+def green_coarse(lat: Lattice, ekg, sigma, mu, beta, gx, gy, kx, ky, nu, sigma_h=None):
+    """get_green_coarse for 1-based (kx, ky, nu), 0-based grain (gx, gy).
+
+    sigma_h: Sigma_H(Nx, Ny) used outside the box (None: zero)."""
     z = matsubara(nu, lat.nf, beta)
     e = ekg[gx, gy, kx - 1, ky - 1]
     if nu < 1 or nu > lat.nf:
-        return 1.0 / (z - e + mu)
+        sh = 0.0 if sigma_h is None else sigma_h[kx - 1, ky - 1]
+        return 1.0 / (z - e + mu - sh)
     s = sigma[((kx - 1) * lat.ny + ky - 1) * lat.nf + nu - 1]
     return 1.0 / (z - e + mu - s)
 
 
-def calc_chi(lat: Lattice, qx: int, qy: int, qw: int, sigma, mu: float, beta: float, ngrain: int = 1):
+def calc_chi(lat: Lattice, qx: int, qy: int, qw: int, sigma, mu: float, beta: float, ngrain: int = 1,
+             sigma_h=None):
     """Bubbles chi_ph_LL, chi_pp_LL (Nz, Nz) for one 1-based bosonic (qx, qy, qw).
 
     SRC/parquet_chi.f90:438-523; loops vectorised over the fermionic index,
@@ -121,10 +150,11 @@ def calc_chi(lat: Lattice, qx: int, qy: int, qw: int, sigma, mu: float, beta: fl
                 acc_pp = 0j
                 for gx in range(ngrain):
                     for gy in range(ngrain):
-                        gk = green_coarse(lat, ekg, sigma, mu, beta, gx, gy, kx, ky, nu)
-                        gkq = green_coarse(lat, ekg, sigma, mu, beta, gx, gy, kqx, kqy, nu_kq)
+                        gk = green_coarse(lat, ekg, sigma, mu, beta, gx, gy, kx, ky, nu, sigma_h)
+                        gkq = green_coarse(lat, ekg, sigma, mu, beta, gx, gy, kqx, kqy, nu_kq, sigma_h)
                         acc_ph += gk * gkq * 1.0 / ngrain / ngrain
-                        gqmk = green_coarse(lat, ekg, sigma, mu, beta, ngrain - gx - 1, ngrain - gy - 1, qmkx, qmky, nu_qmk)
+                        gqmk = green_coarse(lat, ekg, sigma, mu, beta, ngrain - gx - 1, ngrain - gy - 1, qmkx, qmky,
+                                            nu_qmk, sigma_h)
                         acc_pp += gk * gqmk * 1.0 / ngrain / ngrain
                 chi_k_ph[kx - 1, ky - 1, nu - 1] = acc_ph
                 chi_k_pp[kx - 1, ky - 1, nu - 1] = acc_pp

@@ -1,0 +1,100 @@
+"""GPU parity of the closed-form BSE (TUPS_BSE_CLOSED_LU) against the oracle.
+
+The closed form returns the fixed point of BSE(ite) (SRC/parquet_BSE.f90:62-132)
+for fixed Gamma: Phi = (I - kappa_r Gamma chi)^-1 Gamma - Gamma.  It is pinned
+three ways: against the oracle's LAPACK solve, by the library's own ladder
+returning Phi unchanged when fed F = Gamma + Phi, and by the singular case
+(kappa Gamma chi = I) being reported instead of returned.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import bse as B
+from oracle.lattice import Lattice
+from tests.helpers import random_chi_compact, random_vertices, rel_fro
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _ctx(**kw):
+    from tups_amd import TupsContext
+
+    return TupsContext(**kw)
+
+
+def _say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+@pytest.mark.parametrize(
+    "nx,nf,nl,nq",
+    [(4, 16, 1, 0),     # Nz=16: one panel
+     (4, 8, 5, 0),      # Nz=40: partial last panel (8 columns)
+     (4, 6, 5, 3),      # Nz=30: panel of 14, odd tile edges
+     (4, 8, 9, 2),      # Nz=72: two update tiles
+     (8, 64, 5, 2)],    # Nz=320: the batched-BSE shape
+)
+def test_closed_form_parity(nx, nf, nl, nq):
+    from tups_amd import BSE_CLOSED_LU, CHI, F, GAMMA
+
+    _say(f"closed parity nx={nx} nf={nf} nl={nl} nq={nq}")
+    rng = np.random.default_rng(nx * 31 + nf + nl)
+    lat = Lattice(nx, nf, nl)
+    with _ctx(nx=nx, nf=nf, nl=nl, nq=nq, beta=2.0, f_damping=0.5) as c:
+        gam = random_vertices(rng, c.nz, c.nq, diag=2.0, scale=0.3)
+        chic = [random_chi_compact(rng, nl, nf, c.nq, 3.0) for _ in range(2)]
+        for r in range(4):
+            c.upload(GAMMA, r, gam[r])
+            c.upload(F, r, gam[r])
+        for p in range(2):
+            c.upload(CHI, p, chic[p])
+        c.BSE(0, mode=BSE_CLOSED_LU)
+        got = [c.download(GAMMA, r) for r in range(4)]
+    exp = B.bse_closed(lat, gam, [B.chi_compact_to_dense(lat, x) for x in chic], 2.0)
+    for r in range(4):
+        err = rel_fro(got[r], exp[r])
+        _say(f"  channel {r}: rel err {err:.2e}")
+        assert err < TOL
+
+
+def test_closed_form_is_fixed_point_of_gpu_ladder():
+    from tups_amd import BSE_CLOSED_LU, CHI, F, GAMMA
+
+    rng = np.random.default_rng(3)
+    nx, nf, nl = 4, 8, 5
+    with _ctx(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.3) as c:
+        gam = random_vertices(rng, c.nz, c.nq, diag=2.0, scale=0.3)
+        chic = [random_chi_compact(rng, nl, nf, c.nq, 3.0) for _ in range(2)]
+        for p in range(2):
+            c.upload(CHI, p, chic[p])
+        for r in range(4):
+            c.upload(GAMMA, r, gam[r])
+        c.BSE(0, mode=BSE_CLOSED_LU)
+        phi = [c.download(GAMMA, r) for r in range(4)]
+        for r in range(4):
+            c.upload(GAMMA, r, gam[r])
+            c.upload(F, r, gam[r] + phi[r])
+        c.BSE(0)  # the damped FP64 ladder
+        again = [c.download(GAMMA, r) for r in range(4)]
+    for r in range(4):
+        assert rel_fro(again[r], phi[r]) < TOL, r
+
+
+def test_closed_form_reports_singular_system():
+    """Gamma_d = I and chi_ph = 1/kappa_d make I - kappa_d Gamma_d chi_ph = 0."""
+    from tups_amd import BSE_CLOSED_LU, CHI, GAMMA, TupsError
+
+    nx, nf, nl = 4, 8, 1
+    with _ctx(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as c:
+        D, nq = c.nz, c.nq
+        eye = np.asfortranarray(np.repeat(np.eye(D, dtype=complex)[:, :, None], nq, axis=2))
+        for r in range(4):
+            c.upload(GAMMA, r, eye)
+        kappa_d = 2 * 0.5 * c.alpha
+        c.upload(CHI, 0, np.asfortranarray(np.full((1, 1, nf, nq), 1.0 / kappa_d, complex)))
+        c.upload(CHI, 1, np.asfortranarray(np.zeros((1, 1, nf, nq), complex)))
+        with pytest.raises(TupsError):
+            c.BSE(0, mode=BSE_CLOSED_LU)

@@ -1,0 +1,70 @@
+"""One parquet iteration of the path as a replayed HIP graph (tups_iteration).
+
+The graph runs the same kernels as the step-by-step calls BSE(ite) ->
+calc_PhiR -> calculate_PAE -> G = F - G (SRC/loop.f90:78-139,219-222), so every
+replay must equal the eager sequence bit for bit.  Replays alternate between
+the two vertex-buffer parities, and host work with synchronisation runs
+between them.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tests.helpers import crandn, random_chi_compact, random_vertices
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def _setup(c, seed):
+    from tups_amd import CHI, F, GAMMA, LAMBDA
+
+    rng = np.random.default_rng(seed)
+    gam = random_vertices(rng, c.nz, c.nq, diag=0.5, scale=0.05)
+    fv = random_vertices(rng, c.nz, c.nq, diag=0.5, scale=0.05)
+    for r in range(4):
+        c.upload(GAMMA, r, gam[r])
+        c.upload(F, r, fv[r])
+        c.upload(LAMBDA, r, crandn(rng, (c.nf, c.nf, c.nw), 0.1))
+    for p in range(2):
+        c.upload(CHI, p, random_chi_compact(rng, c.nl, c.nf, c.nq, 0.5))
+    c.calc_M()
+    c.sync()
+
+
+@pytest.mark.parametrize("nx,nf,nl,split", [(4, 8, 1, False), (4, 8, 5, False), (6, 8, 1, True)])
+def test_graph_iteration_equals_eager(nx, nf, nl, split):
+    from tups_amd import BSE_LADDER_FP64, BSE_LADDER_SPLIT, F, GAMMA, PHIR, TupsContext
+
+    mode = BSE_LADDER_SPLIT if split else BSE_LADDER_FP64
+    _say(f"graph iteration nx={nx} nf={nf} nl={nl} mode={mode}")
+    with TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as a, \
+            TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as b, \
+            TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as e:
+        for c in (a, b, e):
+            _setup(c, 17)
+        for it in range(4):
+            a.BSE(it, mode=mode)
+            a.calc_PhiR(it)
+            a.calculate_PAE()
+            a.gamma_update()
+            b.iteration(mode, graph=True)
+            e.iteration(mode, graph=False)
+            for which in (GAMMA, F, PHIR):
+                for r in range(4):
+                    x = a.download(which, r)
+                    assert np.array_equal(x, b.download(which, r)), (it, which, r)
+                    assert np.array_equal(x, e.download(which, r)), (it, which, r)
+            assert np.isfinite(a.download(GAMMA, 0)).all()
+
+
+def test_iteration_needs_m_tensors():
+    from tups_amd import TupsContext, TupsError
+
+    with TupsContext(nx=4, nf=8, nl=1, beta=2.0, f_damping=0.5) as c:
+        with pytest.raises(TupsError):
+            c.iteration(graph=True)

@@ -40,6 +40,25 @@ def test_ladder_fixed_point(nl, nf, f):
         assert rel_fro(phi[r], F[r] - gam[r]) < 1e-12, r
 
 
+@pytest.mark.parametrize("nl,nf", [(1, 8), (5, 4), (9, 4)])
+def test_closed_form_is_the_ladder_fixed_point(nl, nf):
+    """bse_closed gives Phi with ladder(Gamma, F = Gamma + Phi) = Phi for any damping:
+    the closed form is pinned by the reference's own update, not by its formula."""
+    rng = np.random.default_rng(11)
+    lat = Lattice(4, nf, nl)
+    nq, nz, beta = 2, lat.nz, 2.0
+    gam = random_vertices(rng, nz, nq, diag=2.0, scale=0.3)
+    chi = [B.chi_compact_to_dense(lat, random_chi_compact(rng, nl, nf, nq, 2.0)) for _ in range(2)]
+    phi = B.bse_closed(lat, gam, chi, beta)
+    F = [gam[r] + phi[r] for r in range(4)]
+    for f in (1.0, 0.4):
+        again = B.bse_ladder(lat, gam, F, chi, beta, f)
+        for r in range(4):
+            assert rel_fro(again[r], phi[r]) < 1e-12, (r, f)
+    # and it is not the trivial solution
+    assert min(np.linalg.norm(p) for p in phi) > 1e-3
+
+
 def test_ladder_first_iteration():
     """F = Gamma (the reference's first iteration, SRC/preprocessing.f90:100-116):
     Phi = 2 f c_r alpha Gamma chi Gamma."""
@@ -133,6 +152,23 @@ def test_calc_chi_nl1_closed_form():
                                     (ky + qy - 2) % 4 + 1, nu + qw - 1)
                 acc += g1 * g2
         assert abs(ph[nu - 1, nu - 1] - acc / lat.nc) < 1e-13
+
+
+def test_calc_chi_hartree_outside_box_is_a_mu_shift():
+    """A constant self-energy s inside the box with Sigma_H = s outside is the
+    same as Sigma = 0 at mu - s: pins that frequencies outside 1..Nf (reached by
+    nu + w - 1 and Nf - nu + w) take Sigma_H, as get_green_coarse does."""
+    lat = Lattice(4, 8, 5)
+    beta, mu, s = 2.0, 0.2, 0.3 - 0.1j
+    sigma = np.full(lat.nc * lat.nf, s)
+    sigma_h = np.full((lat.nx, lat.ny), s)
+    for q in [(2, 1, 3), (3, 3, 4)]:
+        ph, pp = B.calc_chi(lat, *q, sigma, mu, beta, 1, sigma_h)
+        ph0, pp0 = B.calc_chi(lat, *q, np.zeros_like(sigma), mu - s, beta, 1)
+        assert rel_fro(ph, ph0) < 1e-14 and rel_fro(pp, pp0) < 1e-14
+        # without Sigma_H the out-of-box frequencies differ
+        ph1, _ = B.calc_chi(lat, *q, sigma, mu, beta, 1)
+        assert rel_fro(ph1, ph0) > 1e-6
 
 
 def test_gamma_update():

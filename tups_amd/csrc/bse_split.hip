@@ -69,6 +69,12 @@ __device__ __forceinline__ size_t ws_off(const CrtArgs& a, int mat, int mod, int
   return (((size_t)mat * a.nmod + mod) * 2 + sign) * (size_t)a.Dp * a.Kp + (size_t)rc * a.Kp;
 }
 
+__device__ __forceinline__ int xcd_contiguous(int bid, int nblk) {
+  const int xcd = bid & 7, idx = bid >> 3;
+  const int per = nblk >> 3, rem = nblk & 7;
+  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
 // symmetric residue of an integer-valued y (|y| < 2^50): exact, |r| <= (m - 1) / 2
 __device__ __forceinline__ double sym_mod(double y, double m, double inv_m) {
   return fma(-m, rint(y * inv_m), y);
@@ -90,108 +96,130 @@ __device__ __forceinline__ void gauss_residues(const CrtArgs& a, int mod, double
   rm = (int)sym_mod(ym, m, im) & 0xff;
 }
 
-// A = [Gamma | F]: one lane per row; wave w converts inner indices
-// 16 w + 64 n .. + 15 and stores 16 bytes per (modulus, sign).
-// grid: (Dp / 64, 4 * nqb)
-__global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
+// A = [Gamma | F] rows: exponent e_i of each row maximum.  One lane per row,
+// the four waves split the inner dimension.  grid: (Dp / 64, 4 * nqb)
+__global__ __launch_bounds__(256) void crt_rowmax_a(CrtArgs a) {
   __shared__ double red[4][64];
   const int mat = blockIdx.y;  // r * nqb + qb
-  const int r = mat / a.nqb, qb = mat - r * a.nqb, q = a.q0 + qb;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
   const int D = a.D;
   const size_t DD = (size_t)D * D;
   const double2* G = a.gam[r] + DD * q;
   const double2* F = a.fv[r] + DD * q;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = blockIdx.x * 64 + lane;
-  const bool rok = row < D;
-  auto val = [&](int k) -> double2 {
-    if (!rok || k >= 2 * D) return make_double2(0.0, 0.0);
-    return k < D ? G[(size_t)k * D + row] : F[(size_t)(k - D) * D + row];
-  };
-  const int kq = a.Kp / 4;
   double mx = 0.0;
-  for (int k = wave * kq; k < (wave + 1) * kq; ++k) {
-    const double2 v = val(k);
-    mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+  if (row < D) {
+    const int kq = (2 * D + 3) / 4;
+    const int k1 = min(2 * D, (wave + 1) * kq);
+    for (int k = wave * kq; k < k1; ++k) {
+      const double2 v = k < D ? G[(size_t)k * D + row] : F[(size_t)(k - D) * D + row];
+      mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+    }
   }
   red[wave][lane] = mx;
   __syncthreads();
-  mx = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
-  int e = 0;
-  if (mx > 0.0) frexp(mx, &e);
-  if (wave == 0) a.escale[(size_t)mat * a.Dp + row] = e;
-  const double sc = ldexp(1.0, a.ta - e);
-  for (int k0 = wave * 16; k0 < a.Kp; k0 += 64) {
-    double r1[16], r0[16], i1[16], i0[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const double2 v = val(k0 + u);
-      limbs(rint(v.x * sc), r1[u], r0[u]);
-      limbs(rint(v.y * sc), i1[u], i0[u]);
-    }
-    for (int mod = 0; mod < a.nmod; ++mod) {
-      int wp[4] = {0, 0, 0, 0}, wm[4] = {0, 0, 0, 0};
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        int bp, bm;
-        gauss_residues(a, mod, r1[u], r0[u], i1[u], i0[u], bp, bm);
-        wp[u >> 2] |= bp << (8 * (u & 3));
-        wm[u >> 2] |= bm << (8 * (u & 3));
-      }
-      *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 0, row) + k0) = v4i{wp[0], wp[1], wp[2], wp[3]};
-      *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 1, row) + k0) = v4i{wm[0], wm[1], wm[2], wm[3]};
-    }
+  if (wave == 0) {
+    mx = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+    int e = 0;
+    if (mx > 0.0) frexp(mx, &e);
+    a.escale[(size_t)mat * a.Dp + row] = e;
   }
 }
 
-// B = [alpha chi F ; alpha chi Gamma]: one wave per column; lane l converts
-// inner indices 4 l + 256 n .. + 3 and stores one dword per (modulus, sign).
-// grid: (Dp / 4, 4 * nqb)
-__global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
+// A = [Gamma | F] residues: one workgroup per 64 rows x 64 inner indices; lane
+// = row, wave w converts inner indices 16 w .. 16 w + 15 of the chunk and
+// stores 16 bytes per (modulus, sign).  grid: ((Dp / 64) * (Kp / 64), 4 * nqb)
+__global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
   const int mat = blockIdx.y;
-  const int r = mat / a.nqb, qb = mat - r * a.nqb, q = a.q0 + qb;
-  const int D = a.D, nf = a.nf, nl = a.nl;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  const int D = a.D;
   const size_t DD = (size_t)D * D;
   const double2* G = a.gam[r] + DD * q;
   const double2* F = a.fv[r] + DD * q;
+  const int nkc = a.Kp / 64;
+  const int rb = blockIdx.x / nkc, kc = blockIdx.x - rb * nkc;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = rb * 64 + lane;
+  const bool rok = row < D;
+  const int k0 = kc * 64 + wave * 16;
+  const int e = a.escale[(size_t)mat * a.Dp + row];
+  const double sc = ldexp(1.0, a.ta - e);
+  double r1[16], r0[16], i1[16], i0[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int k = k0 + u;
+    double2 v = make_double2(0.0, 0.0);
+    if (rok && k < 2 * D) v = k < D ? G[(size_t)k * D + row] : F[(size_t)(k - D) * D + row];
+    limbs(rint(v.x * sc), r1[u], r0[u]);
+    limbs(rint(v.y * sc), i1[u], i0[u]);
+  }
+  for (int mod = 0; mod < a.nmod; ++mod) {
+    int wp[4] = {0, 0, 0, 0}, wm[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      int bp, bm;
+      gauss_residues(a, mod, r1[u], r0[u], i1[u], i0[u], bp, bm);
+      wp[u >> 2] |= bp << (8 * (u & 3));
+      wm[u >> 2] |= bm << (8 * (u & 3));
+    }
+    *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 0, row) + k0) = v4i{wp[0], wp[1], wp[2], wp[3]};
+    *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 1, row) + k0) = v4i{wm[0], wm[1], wm[2], wm[3]};
+  }
+}
+
+// B = [alpha chi F ; alpha chi Gamma]: one wave per column.  The wave first
+// forms the column's 2D mixed values with lanes along nu (coalesced reads of
+// X and chi) into LDS, then lane l converts inner indices 4 l + 256 n .. + 3
+// and stores one dword per (modulus, sign).  grid: (Dp / 4, 4 * nqb);
+// dynamic LDS 4 * Kp * 16 bytes.
+__global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
+  extern __shared__ double2 colv[];  // [wave][Kp]
+  const int mat = blockIdx.y;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  const int D = a.D, nf = a.nf, nl = a.nl;
+  const size_t DD = (size_t)D * D;
   const double2* chiq = a.chi[r >> 1] + (size_t)nf * nl * nl * q;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = blockIdx.x * 4 + wave;
   const bool cok = col < D;
-  // B(k, col), k = part D + l1 Nf + nu: alpha sum_l3 chi_nu(l1, l3) X((l3, nu), col)
-  auto mixed = [&](int k) -> double2 {
-    if (!cok || k >= 2 * D) return make_double2(0.0, 0.0);
-    const int part = k >= D;
-    const int kk = k - part * D;
-    const int l1 = kk / nf, nu = kk - l1 * nf;
-    const double2* X = (part ? G : F) + (size_t)col * D + nu;
-    const double2* cb = chiq + (size_t)nu * nl * nl + l1;
-    double sr = 0.0, si = 0.0;
-    for (int l3 = 0; l3 < nl; ++l3) {
-      const double2 c = cb[nl * l3];
-      const double2 x = X[l3 * nf];
-      sr = fma(c.x, x.x, fma(-c.y, x.y, sr));
-      si = fma(c.x, x.y, fma(c.y, x.x, si));
-    }
-    return make_double2(a.alpha * sr, a.alpha * si);
-  };
+  double2* v = colv + (size_t)wave * a.Kp;
+  // B((part, l1, nu), col) = alpha sum_l3 chi_nu(l1, l3) X((l3, nu), col), X = F (part 0), Gamma (part 1)
   double mx = 0.0;
-  for (int k = lane; k < 2 * D; k += 64) {
-    const double2 v = mixed(k);
-    mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+  for (int part = 0; part < 2; ++part) {
+    const double2* X = (part ? a.gam[r] : a.fv[r]) + DD * q + (size_t)col * D;
+    for (int nu = lane; nu < nf; nu += 64) {
+      const double2* cb = chiq + (size_t)nu * nl * nl;
+      for (int l1 = 0; l1 < nl; ++l1) {
+        double sr = 0.0, si = 0.0;
+        if (cok) {
+          for (int l3 = 0; l3 < nl; ++l3) {
+            const double2 c = cb[l1 + nl * l3];
+            const double2 x = X[l3 * nf + nu];
+            sr = fma(c.x, x.x, fma(-c.y, x.y, sr));
+            si = fma(c.x, x.y, fma(c.y, x.x, si));
+          }
+        }
+        const double2 m = make_double2(a.alpha * sr, a.alpha * si);
+        v[part * D + l1 * nf + nu] = m;
+        mx = fmax(mx, fmax(fabs(m.x), fabs(m.y)));
+      }
+    }
   }
+  for (int k = 2 * D + lane; k < a.Kp; k += 64) v[k] = make_double2(0.0, 0.0);
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
   int f = 0;
   if (mx > 0.0) frexp(mx, &f);
   if (lane == 0) a.fscale[(size_t)mat * a.Dp + col] = f;
   const double sc = ldexp(1.0, a.tb - f);
+  __syncthreads();  // the column in LDS is complete (each wave reads only its own)
   for (int k0 = 4 * lane; k0 < a.Kp; k0 += 256) {
     double r1[4], r0[4], i1[4], i0[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const double2 v = mixed(k0 + u);
-      limbs(rint(v.x * sc), r1[u], r0[u]);
-      limbs(rint(v.y * sc), i1[u], i0[u]);
+      const double2 x = v[k0 + u];
+      limbs(rint(x.x * sc), r1[u], r0[u]);
+      limbs(rint(x.y * sc), i1[u], i0[u]);
     }
     for (int mod = 0; mod < a.nmod; ++mod) {
       int wp = 0, wm = 0;
@@ -217,7 +245,9 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
   __shared__ __attribute__((aligned(16))) int8_t As[2][2][CRT_TILE][CRT_ROW];
   __shared__ __attribute__((aligned(16))) int8_t Bs[2][2][CRT_TILE][CRT_ROW];
   const int tiles = a.tiles_n * a.tiles_n;
-  const int bid = blockIdx.x;
+  // workgroups go round-robin over the 8 XCDs: give each XCD a contiguous run
+  // of logical tiles, so all tiles of one matrix share that XCD's L2
+  const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
   const int mat = bid / tiles;
   const int t = bid - mat * tiles;
   const int row0 = (t / a.tiles_n) * CRT_TILE, col0 = (t % a.tiles_n) * CRT_TILE;
@@ -242,12 +272,6 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
     *reinterpret_cast<v4i*>(&Bs[buf][0][srow][sch]) = pf[2];
     *reinterpret_cast<v4i*>(&Bs[buf][1][srow][sch]) = pf[3];
   };
-  v4i pf[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) pf[v] = *gptr(0, v);
-  lds_put(0, pf);
-  __syncthreads();
-
   double shr[16], slr[16], shi[16], sli[16];  // CRT sums, element (i, j, e) -> 8 i + 4 j + e
 #pragma unroll
   for (int x = 0; x < 16; ++x) shr[x] = slr[x] = shi[x] = sli[x] = 0.0;
@@ -258,13 +282,7 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
     for (int j = 0; j < 2; ++j) accp[i][j] = accm[i][j] = v4i{0, 0, 0, 0};
 
   int kstep = 0, mod = 0;
-  for (int step = 0; step < nsteps; ++step) {
-    const int buf = step & 1;
-    const bool more = step + 1 < nsteps;
-    if (more) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) pf[v] = *gptr(step + 1, v);
-    }
+  auto compute = [&](int buf) {
     v4i ap[2], am[2], bp[2], bm[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -306,7 +324,34 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
       kstep = 0;
       ++mod;
     }
-    if (more) lds_put(buf ^ 1, pf);
+  };
+
+  // two register sets keep the loads of the next two steps in flight: at
+  // step s, P (or Q) holds step s + 1's chunks and the other set receives s + 2's
+  v4i P[4], Q[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) P[v] = *gptr(0, v);
+  lds_put(0, P);
+  if (nsteps > 1) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) P[v] = *gptr(1, v);
+  }
+  __syncthreads();
+  for (int step = 0; step < nsteps; step += 2) {
+    if (step + 2 < nsteps) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Q[v] = *gptr(step + 2, v);
+    }
+    compute(0);
+    if (step + 1 < nsteps) lds_put(1, P);
+    __syncthreads();
+    if (step + 1 >= nsteps) break;
+    if (step + 3 < nsteps) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) P[v] = *gptr(step + 3, v);
+    }
+    compute(1);
+    if (step + 2 < nsteps) lds_put(0, Q);
     __syncthreads();
   }
 
@@ -406,7 +451,9 @@ static void crt_constants(CrtArgs& a, int nmod, int D) {
   a.tb = std::min(52, tsum - tsum / 2);
 }
 
-int launch_bse_split(tups_ctx* c) {
+// q points per pass and workspace of the split ladder; allocates (outside any
+// stream capture) when the workspace is too small
+int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
   const tups_sizes& s = c->sz;
   const int nmod = c->split_moduli;
   TUPS_CHECK(nmod >= 8 && nmod <= CRT_MAX, TUPS_EINVAL, "split moduli must be in 8..16");
@@ -426,6 +473,21 @@ int launch_bse_split(tups_ctx* c) {
     TUPS_HIP(hipMalloc(&c->split_ws, need));
     c->split_ws_bytes = need;
   }
+  const size_t lds_b = 4 * Kp * sizeof(double2);  // <= 128 KiB for Nz <= 1024
+  if (lds_b > 65536)
+    TUPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(crt_prep_b),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+  if (qbatch_out) *qbatch_out = qbatch;
+  return TUPS_OK;
+}
+
+int launch_bse_split(tups_ctx* c) {
+  const tups_sizes& s = c->sz;
+  const int nmod = c->split_moduli;
+  int qbatch = 0;
+  if (int rc = bse_split_reserve(c, &qbatch)) return rc;
+  const size_t Dp = ((size_t)s.nz + 63) / 64 * 64;
+  const size_t Kp = (2 * (size_t)s.nz + 63) / 64 * 64;
   CrtArgs a{};
   for (int r = 0; r < 4; ++r) {
     a.gam[r] = c->gamma[r];
@@ -445,6 +507,7 @@ int launch_bse_split(tups_ctx* c) {
   for (int r = 0; r < 4; ++r) a.cf[r] = cr[r] * c->f_damping;
   a.omf = 1.0 - c->f_damping;
   crt_constants(a, nmod, s.nz);
+  const size_t lds_b = 4 * Kp * sizeof(double2);
   for (int q0 = 0; q0 < s.nq; q0 += qbatch) {
     a.q0 = q0;
     a.nqb = std::min(qbatch, s.nq - q0);
@@ -455,9 +518,13 @@ int launch_bse_split(tups_ctx* c) {
     a.escale = reinterpret_cast<int*>(a.B + mats * 2 * nmod * Dp * Kp);
     a.fscale = a.escale + mats * Dp;
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL(crt_prep_a, dim3((unsigned)(Dp / 64), (unsigned)mats), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(crt_rowmax_a, dim3((unsigned)(Dp / 64), (unsigned)mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(crt_prep_b, dim3((unsigned)(Dp / 4), (unsigned)mats), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(crt_prep_a, dim3((unsigned)((Dp / 64) * (Kp / 64)), (unsigned)mats), dim3(256), 0,
+                       c->stream, a);
+    TUPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(crt_prep_b, dim3((unsigned)(Dp / 4), (unsigned)mats), dim3(256),
+                       lds_b, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_SPLIT_PREP)) return rc;
     const long long nblk = (long long)a.tiles_n * a.tiles_n * mats;

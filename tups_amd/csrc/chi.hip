@@ -1,0 +1,168 @@
+// Bubbles chi_ph, chi_pp in form-factor space for every q slot:
+// calc_chi (SRC/parquet_chi.f90:438-523) with get_green_coarse
+// (SRC/parquet_util.f90:1303-1327), Ek_grain (SRC/parquet_util.f90:143-164)
+// and the index maps index_FaddB / index_minusF (SRC/parquet_util.f90:1345-1381).
+//
+//   chi_k_ph(k, nu) = 1/Ng^2 sum_g G(g; k, nu) G(g; k + q, nu + w - 1)
+//   chi_k_pp(k, nu) = 1/Ng^2 sum_g G(g; k, nu) G(Ng + 1 - g; q - k, Nf - nu + w)
+//   chi_X(l1, l2, nu) = sum_k chi_k_X(k, nu) FF(k, l1) FF(k, l2)
+// G(g; k, nu) = 1 / (i nu_n - Ek_grain(g, k) + mu - Sigma(k, nu)), with the
+// Hartree value Sigma_H(k) outside the box 1..Nf.  Only the Nl x Nl blocks of
+// the block-diagonal bubble are formed (the resident TUPS_CHI layout).
+//
+// One workgroup per q slot; the frequency range is walked in chunks: the
+// k-space bubble of a chunk is formed in LDS, then projected on the l pairs
+// with k summed in the reference's order (kx fastest).
+#include "internal.h"
+
+#include <cmath>
+
+namespace tups {
+
+struct ChiArgs {
+  double2* chi_ph;
+  double2* chi_pp;
+  const double2* sigma;   // (Nx Ny Nf): ((kx-1) Ny + ky-1) Nf + nu-1
+  const double2* sigh;    // Sigma_H(Nx, Ny), column-major
+  const double* ff;       // FF[l Nc + ky Nx + kx]
+  const int* qm;          // IBZ momentum slot of each q
+  const int* qw;          // bosonic index of each q (1-based)
+  const int* ibz;         // IBZ momenta: x[0..nibz), y[0..nibz) (1-based)
+  int nx, nf, nl, nibz, ngrain, chunk;
+  double mu, beta;
+};
+
+__device__ __forceinline__ double2 green(const ChiArgs& a, int gx, int gy, int kx, int ky, int nu) {
+  // 1-based grain (gx, gy), momentum (kx, ky) and frequency nu (may lie outside 1..Nf)
+  const int nx = a.nx, ng = a.ngrain;
+  const double pi = 3.14159265358979323846;
+  const double dk = 2.0 * pi / nx;
+  const int half = (ng + 1) / 2;
+  const double px = dk * (kx - 1) + (gx - half) * dk / (double)ng;
+  const double py = dk * (ky - 1) + (gy - half) * dk / (double)ng;
+  const double ek = -2.0 * cos(px) - 2.0 * cos(py);
+  const double wn = pi / a.beta * (2.0 * (nu - a.nf / 2 - 1) + 1.0);
+  double2 s;
+  if (nu < 1 || nu > a.nf)
+    s = a.sigh[(kx - 1) + (size_t)nx * (ky - 1)];
+  else
+    s = a.sigma[((size_t)(kx - 1) * nx + (ky - 1)) * a.nf + (nu - 1)];
+  const double re = -ek + a.mu - s.x, im = wn - s.y;  // z = re + i im
+  const double d = re * re + im * im;
+  return make_double2(re / d, -im / d);
+}
+
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// grid: nq; dynamic LDS 2 * chunk * Nc * 16 bytes
+__global__ __launch_bounds__(256) void chi_kernel(ChiArgs a) {
+  extern __shared__ double2 ck[];  // [2][chunk][Nc]
+  const int q = blockIdx.x;
+  const int nx = a.nx, nc = nx * nx, nf = a.nf, nl = a.nl, ng = a.ngrain;
+  const int m = a.qm[q];
+  const int qx = a.ibz[m], qy = a.ibz[a.nibz + m], w = a.qw[q];
+  double2* cph = ck;
+  double2* cpp = ck + (size_t)a.chunk * nc;
+  const double inv_g2 = 1.0 / ng / ng;
+  for (int nu0 = 0; nu0 < nf; nu0 += a.chunk) {
+    const int nch = min(a.chunk, nf - nu0);
+    for (int u = threadIdx.x; u < nch * nc; u += blockDim.x) {
+      const int k = u % nc, o = u / nc;
+      const int kx = k % nx + 1, ky = k / nx + 1, nu = nu0 + o + 1;
+      const int kqx = (kx - 1 + qx - 1) % nx + 1, kqy = (ky - 1 + qy - 1) % nx + 1;
+      const int mkx = (-(kx - 1) + nx) % nx + 1, mky = (-(ky - 1) + nx) % nx + 1;
+      const int qmkx = (mkx - 1 + qx - 1) % nx + 1, qmky = (mky - 1 + qy - 1) % nx + 1;
+      const int nu_kq = nu + w - 1, nu_qmk = (-nu + nf + 1) + w - 1;
+      double2 sph = make_double2(0.0, 0.0), spp = make_double2(0.0, 0.0);
+      for (int gx = 1; gx <= ng; ++gx)
+        for (int gy = 1; gy <= ng; ++gy) {
+          const double2 gk = green(a, gx, gy, kx, ky, nu);
+          const double2 p = cmul2(gk, green(a, gx, gy, kqx, kqy, nu_kq));
+          sph.x += p.x * inv_g2;
+          sph.y += p.y * inv_g2;
+          const double2 t = cmul2(gk, green(a, ng - gx + 1, ng - gy + 1, qmkx, qmky, nu_qmk));
+          spp.x += t.x * inv_g2;
+          spp.y += t.y * inv_g2;
+        }
+      cph[(size_t)o * nc + k] = sph;
+      cpp[(size_t)o * nc + k] = spp;
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < nch * nl * nl; u += blockDim.x) {
+      const int l = u % (nl * nl), o = u / (nl * nl);
+      const int l1 = l % nl, l2 = l / nl;
+      const double* f1 = a.ff + (size_t)l1 * nc;
+      const double* f2 = a.ff + (size_t)l2 * nc;
+      double2 sph = make_double2(0.0, 0.0), spp = make_double2(0.0, 0.0);
+      for (int k = 0; k < nc; ++k) {  // k = (ky - 1) Nx + kx - 1: kx fastest
+        const double wgt = f1[k] * f2[k];
+        const double2 xp = cph[(size_t)o * nc + k], xq = cpp[(size_t)o * nc + k];
+        sph.x += xp.x * wgt;
+        sph.y += xp.y * wgt;
+        spp.x += xq.x * wgt;
+        spp.y += xq.y * wgt;
+      }
+      const size_t idx = (size_t)l1 + (size_t)nl * l2 + (size_t)nl * nl * (nu0 + o) + (size_t)nl * nl * nf * q;
+      a.chi_ph[idx] = sph;
+      a.chi_pp[idx] = spp;
+    }
+    __syncthreads();
+  }
+}
+
+int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain) {
+  const tups_sizes& s = c->sz;
+  const int nc = s.nx * s.ny;
+  const size_t nt = (size_t)nc * s.nf;
+  TUPS_CHECK(ngrain >= 1 && ngrain <= 64, TUPS_EINVAL, "ngrain must be in 1..64");
+  if (!c->d_sigma) TUPS_HIP(hipMalloc((void**)&c->d_sigma, nt * sizeof(double2)));
+  if (!c->d_sigh) TUPS_HIP(hipMalloc((void**)&c->d_sigh, (size_t)nc * sizeof(double2)));
+  std::vector<double> sh(2 * (size_t)nc);
+  if (sigma_h) {
+    std::copy(sigma_h, sigma_h + 2 * (size_t)nc, sh.begin());
+  } else {
+    // Sigma_H(kx, ky) = Sigma at the last box frequency (SRC/parquet_chi.f90:103-110)
+    for (int kx = 1; kx <= s.nx; ++kx)
+      for (int ky = 1; ky <= s.ny; ++ky) {
+        const size_t src = ((size_t)(kx - 1) * s.ny + (ky - 1)) * s.nf + (s.nf - 1);
+        const size_t dst = (size_t)(kx - 1) + (size_t)s.nx * (ky - 1);
+        sh[2 * dst] = sigma[2 * src];
+        sh[2 * dst + 1] = sigma[2 * src + 1];
+      }
+  }
+  TUPS_HIP(hipMemcpyAsync(c->d_sigma, sigma, nt * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  TUPS_HIP(hipMemcpyAsync(c->d_sigh, sh.data(), (size_t)nc * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  ChiArgs a{};
+  a.chi_ph = c->chi[0];
+  a.chi_pp = c->chi[1];
+  a.sigma = c->d_sigma;
+  a.sigh = c->d_sigh;
+  a.ff = c->d_ffk;
+  a.qm = c->d_qm;
+  a.qw = c->d_qw;
+  a.ibz = c->d_ibz;
+  a.nx = s.nx;
+  a.nf = s.nf;
+  a.nl = s.nl;
+  a.nibz = s.nibz;
+  a.ngrain = ngrain;
+  a.chunk = std::max(1, std::min(s.nf, 2048 / nc));
+  a.mu = mu;
+  a.beta = c->beta;
+  const size_t lds = 2 * (size_t)a.chunk * nc * sizeof(double2);
+  TUPS_CHECK(lds <= 160 * 1024, TUPS_EINVAL, "calc_chi supports Nx <= 64");
+  if (lds > 65536)
+    TUPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(chi_kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (int rc = timer_begin(c)) return rc;
+  hipLaunchKernelGGL(chi_kernel, dim3((unsigned)s.nq), dim3(256), lds, c->stream, a);
+  TUPS_HIP(hipGetLastError());
+  if (int rc = timer_end(c, TUPS_K_CALC_CHI)) return rc;
+  // the host buffers may be reused as soon as the call returns
+  TUPS_HIP(hipStreamSynchronize(c->stream));
+  return TUPS_OK;
+}
+
+}  // namespace tups

@@ -80,6 +80,7 @@ int upload_vec(T** dst, const std::vector<T>& src, hipStream_t st) {
 }  // namespace
 
 int upload_tables(tups_ctx* c) {
+  drop_graphs(c);  // graphs hold the table pointers
   Tables& t = c->tab;
   const int nl = t.nl, nc = t.nc;
   int rc;
@@ -139,7 +140,7 @@ int timer_end(tups_ctx* c, int kernel) {
   return TUPS_OK;
 }
 
-static int ensure_tables(tups_ctx* c) {
+int ensure_tables(tups_ctx* c) {
   if (!c->tables_dirty) return TUPS_OK;
   return upload_tables(c);
 }
@@ -272,6 +273,7 @@ int tups_create(const tups_dims* d, tups_ctx** out) {
 int tups_destroy(tups_ctx* c) {
   if (!c) return TUPS_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   auto f = [](void* p) {
     if (p) hipFree(p);
   };
@@ -286,6 +288,9 @@ int tups_destroy(tups_ctx* c) {
   f(c->chi[0]);
   f(c->chi[1]);
   f(c->split_ws);
+  f(c->closed_ws);
+  f(c->d_sigma);
+  f(c->d_sigh);
   f(c->d_ffk);
   f(c->d_ffik);
   f(c->d_eqsr);
@@ -417,22 +422,32 @@ int tups_upload_chi_dense(tups_ctx* c, int32_t pp, int32_t q0, int32_t nqs, cons
 
 int tups_bse(tups_ctx* c, int32_t mode) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
-  TUPS_CHECK(mode == TUPS_BSE_LADDER_FP64 || mode == TUPS_BSE_LADDER_SPLIT, TUPS_EINVAL,
-             "unknown BSE mode");
+  TUPS_CHECK(mode == TUPS_BSE_LADDER_FP64 || mode == TUPS_BSE_LADDER_SPLIT || mode == TUPS_BSE_CLOSED_LU,
+             TUPS_EINVAL, "unknown BSE mode");
   int rc = ensure_tables(c);
   if (rc) return rc;
-  rc = mode == TUPS_BSE_LADDER_SPLIT ? launch_bse_split(c) : launch_bse_ladder(c);
+  rc = mode == TUPS_BSE_LADDER_SPLIT ? launch_bse_split(c)
+       : mode == TUPS_BSE_CLOSED_LU  ? launch_bse_closed(c)
+                                     : launch_bse_ladder(c);
   if (rc) return rc;
   // Phi overwrites G_r_LL (SRC/parquet_BSE.f90:81): swap the ping-pong pair
-  for (int r = 0; r < 4; ++r) std::swap(c->gamma[r], c->spare[r]);
+  swap_vertex_buffers(c);
   c->phir_valid = false;
   return TUPS_OK;
+}
+
+int tups_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int32_t ngrain) {
+  TUPS_CHECK(c && sigma, TUPS_EINVAL, "null argument");
+  int rc = ensure_tables(c);
+  if (rc) return rc;
+  return launch_calc_chi(c, sigma, sigma_h, mu, ngrain);
 }
 
 int tups_set_split(tups_ctx* c, int32_t moduli, int32_t qbatch) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
   TUPS_CHECK(moduli >= 8 && moduli <= 16, TUPS_EINVAL, "split moduli must be in 8..16");
   TUPS_CHECK(qbatch >= 0, TUPS_EINVAL, "qbatch must be >= 0");
+  drop_graphs(c);
   c->split_moduli = moduli;
   c->split_qbatch = qbatch;
   return TUPS_OK;
@@ -481,6 +496,7 @@ int tups_autotune(tups_ctx* c, int32_t kernel, int32_t* variant, float* ms) {
   if (rc) return rc;
   int v = -1;
   float t = 0.f;
+  drop_graphs(c);  // a new variant changes the captured launches
   if (kernel == TUPS_K_PAE) {
     TUPS_CHECK(c->m_built && c->phir_valid, TUPS_ESTATE, "crossing autotune needs M and PhiR");
     rc = autotune_pae(c, 3, &v, &t);
@@ -500,6 +516,7 @@ const char* tups_variant_name(int32_t kernel, int32_t variant) {
 
 int tups_set_variant(tups_ctx* c, int32_t kernel, int32_t variant) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
+  drop_graphs(c);
   if (kernel == TUPS_K_PAE) {
     TUPS_CHECK(variant == -1 || pae_variant_ok(c, variant), TUPS_EINVAL, "crossing variant not usable here");
     c->pae_variant = variant;

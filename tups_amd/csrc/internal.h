@@ -93,6 +93,8 @@ struct tups_ctx {
   int* d_ph_src = nullptr;
   double2* d_ph_coef = nullptr;
   int* d_flag = nullptr;
+  double2* d_sigma = nullptr;   // calc_chi: Sigma(Nx Ny Nf)
+  double2* d_sigh = nullptr;    // calc_chi: Sigma_H(Nx, Ny)
   double2* pae_y = nullptr;     // Nl = 1 crossing: re-indexed PhiR, 8 x (Nf,Nf,Nw,NR)
 
 
@@ -100,12 +102,17 @@ struct tups_ctx {
   int pae_variant = -1;
   int split_moduli = 12;         // split ladder: CRT moduli (8..16)
   int split_qbatch = 0;          // split ladder: q points per pass (0: default)
-  void* split_ws = nullptr;      // split ladder: slices and exponents of one pass
+  void* split_ws = nullptr;      // split ladder: residues and exponents of one pass
   size_t split_ws_bytes = 0;
+  void* closed_ws = nullptr;     // closed form: augmented matrices of one pass
+  size_t closed_ws_bytes = 0;
   bool timing = false;
   hipEvent_t tev[2] = {};
   double t_ms[TUPS_K_COUNT] = {};
   long long t_n[TUPS_K_COUNT] = {};
+
+  int parity = 0;                // vertex ping-pong: number of swaps mod 2
+  hipGraphExec_t graph[4] = {};  // one iteration per (ladder mode, parity)
 
   bool tables_dirty = true;
   bool m_built = false;
@@ -116,6 +123,12 @@ namespace tups {
 // kernels launchers (defined in the .hip sources)
 int launch_bse_ladder(tups_ctx* c);
 int launch_bse_split(tups_ctx* c);
+int bse_split_reserve(tups_ctx* c, int* qbatch);
+void swap_vertex_buffers(tups_ctx* c);
+int ensure_tables(tups_ctx* c);  // rebuild and upload the tables when stale
+void drop_graphs(tups_ctx* c);
+int launch_bse_closed(tups_ctx* c);
+int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain);
 int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms);
 const char* bse_variant_name(int v);
 int bse_variant_count();

@@ -24,6 +24,8 @@ module tups_dropin
   type(c_ptr), save :: tups_h = c_null_ptr
   logical, save :: m_built = .false.
   logical, save :: lambda_on_device = .false.
+  ! arithmetic of BSE(ite): TUPS_BSE_LADDER_FP64 (default) or TUPS_BSE_LADDER_SPLIT
+  integer(c_int32_t), save :: tups_bse_mode = TUPS_BSE_LADDER_FP64
 
 contains
 
@@ -81,7 +83,7 @@ contains
     call put(TUPS_ARR_F, TUPS_CH_M, F_m_LL)
     call put(TUPS_ARR_F, TUPS_CH_S, F_s_LL)
     call put(TUPS_ARR_F, TUPS_CH_T, F_t_LL)
-    call tups_check(tups_bse(tups_h, TUPS_BSE_LADDER_FP64), 'BSE')
+    call tups_check(tups_bse(tups_h, tups_bse_mode), 'BSE')
     call get(TUPS_ARR_GAMMA, TUPS_CH_D, G_d_LL)
     call get(TUPS_ARR_GAMMA, TUPS_CH_M, G_m_LL)
     call get(TUPS_ARR_GAMMA, TUPS_CH_S, G_s_LL)

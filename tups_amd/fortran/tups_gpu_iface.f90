@@ -15,10 +15,12 @@ module tups_gpu_iface
   integer(c_int32_t), parameter :: TUPS_CH_D = 0, TUPS_CH_M = 1, TUPS_CH_S = 2, TUPS_CH_T = 3
   integer(c_int32_t), parameter :: TUPS_ARR_GAMMA = 0, TUPS_ARR_F = 1, TUPS_ARR_CHI = 2, TUPS_ARR_PHIR = 3, &
                                    TUPS_ARR_M = 4, TUPS_ARR_LAMBDA = 5
-  integer(c_int32_t), parameter :: TUPS_BSE_LADDER_FP64 = 0, TUPS_BSE_LADDER_SPLIT = 1
+  integer(c_int32_t), parameter :: TUPS_BSE_LADDER_FP64 = 0, TUPS_BSE_LADDER_SPLIT = 1, &
+                                   TUPS_BSE_CLOSED_LU = 2
   integer(c_int32_t), parameter :: TUPS_K_BSE_LADDER = 0, TUPS_K_PHIR = 1, TUPS_K_BUILD_M = 2, &
                                    TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4, TUPS_K_PAE_GATHER = 5, &
-                                   TUPS_K_SPLIT_PREP = 6, TUPS_K_SPLIT_GEMM = 7
+                                   TUPS_K_SPLIT_PREP = 6, TUPS_K_SPLIT_GEMM = 7, TUPS_K_BSE_CLOSED = 8, &
+                                   TUPS_K_CALC_CHI = 9
 
   type, bind(C) :: tups_dims
     integer(c_int32_t) :: nx, nf, nl, nr, nq, reserved
@@ -130,6 +132,22 @@ module tups_gpu_iface
       integer(c_int32_t), value :: mode
       integer(c_int) :: rc
     end function tups_bse
+
+    function tups_iteration(ctx, mode, use_graph) bind(C, name="tups_iteration") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode, use_graph
+      integer(c_int) :: rc
+    end function tups_iteration
+
+    function tups_calc_chi(ctx, sigma, sigma_h, mu, ngrain) bind(C, name="tups_calc_chi") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: sigma, sigma_h
+      real(c_double), value :: mu
+      integer(c_int32_t), value :: ngrain
+      integer(c_int) :: rc
+    end function tups_calc_chi
 
     function tups_set_split(ctx, moduli, qbatch) bind(C, name="tups_set_split") result(rc)
       import :: c_ptr, c_int32_t, c_int

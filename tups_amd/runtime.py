@@ -20,10 +20,12 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
 
 GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
-K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER, K_SPLIT_PREP, K_SPLIT_GEMM = range(8)
+(K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER, K_SPLIT_PREP, K_SPLIT_GEMM,
+ K_BSE_CLOSED, K_CALC_CHI) = range(10)
 CH_D, CH_M, CH_S, CH_T = range(4)
 BSE_LADDER_FP64 = 0
 BSE_LADDER_SPLIT = 1
+BSE_CLOSED_LU = 2
 M_NAMES = ("phbar_to_ph", "pp_to_ph", "ph_to_pp", "phbar_to_pp")
 
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE"}
@@ -66,6 +68,8 @@ SIGNATURES = {
     "tups_upload_chi_dense": ([_V, C.c_int32, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_bse": ([_V, C.c_int32], C.c_int),
     "tups_set_split": ([_V, C.c_int32, C.c_int32], C.c_int),
+    "tups_calc_chi": ([_V, _D, _D, C.c_double, C.c_int32], C.c_int),
+    "tups_iteration": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_phir": ([_V], C.c_int),
     "tups_build_m": ([_V], C.c_int),
     "tups_pae": ([_V], C.c_int),
@@ -222,6 +226,25 @@ class TupsContext:
     def set_split(self, moduli: int = 12, qbatch: int = 0):
         """CRT moduli and q points per pass of the split (INT8 MFMA) ladder."""
         _check(self.lib.tups_set_split(self._h, moduli, qbatch), "tups_set_split")
+
+    def calc_chi(self, sigma, mu: float, ngrain: int = 1, sigma_h=None):
+        """calc_chi for every q slot (SRC/parquet_chi.f90:438-523) into the resident bubbles.
+
+        sigma: complex (Nx*Ny*Nf,) in the reference's order ((kx-1)*Ny + ky-1)*Nf + nu;
+        sigma_h: complex (Nx, Ny) Hartree values used outside the box, or None for
+        Sigma(k, Nf) as the reference sets them."""
+        s = np.ascontiguousarray(np.asarray(sigma, dtype=np.complex128).ravel())
+        assert s.size == self.nx * self.nx * self.nf
+        sh = None
+        if sigma_h is not None:
+            sh = np.asfortranarray(np.asarray(sigma_h, dtype=np.complex128).reshape(self.nx, self.nx))
+        _check(self.lib.tups_calc_chi(self._h, _dptr(s), _dptr(sh) if sh is not None else None, float(mu),
+                                      int(ngrain)), "tups_calc_chi")
+
+    def iteration(self, mode: int = BSE_LADDER_FP64, graph: bool = True):
+        """BSE -> calc_PhiR -> calculate_PAE -> G = F - G (SRC/loop.f90:78-139,219-222);
+        with graph=True the sequence is captured once and replayed as a HIP graph."""
+        _check(self.lib.tups_iteration(self._h, mode, 1 if graph else 0), "tups_iteration")
 
     def calc_PhiR(self, ite: int = 0):
         """calc_PhiR(ite), SRC/parquet_PhiR.f90:35-109."""
