@@ -51,7 +51,7 @@ def test_calc_chi_parity(nx, nf, nl, ngrain, explicit_h):
         c.calc_chi(sigma, mu, ngrain, sigma_h if explicit_h else None)
         ph = c.download(CHI, 0)
         pp = c.download(CHI, 1)
-        qx, qy, qw = c.get_q_map()
+        qx, qy, qw = c.q_map()
     slots = sorted(set(np.linspace(0, len(qx) - 1, 6).astype(int).tolist()))
     for q in slots:
         eph, epp = B.calc_chi(lat, int(qx[q]), int(qy[q]), int(qw[q]), sigma, mu, beta, ngrain, sigma_h)
