@@ -43,6 +43,20 @@ def parse():
     return ap.parse_args()
 
 
+def profile_traffic(name):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 counter summary
+    (FETCH_SIZE x 2 KiB + WRITE_SIZE x 1 KiB), or None when the summary is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(nx, nf, nl, budget_s):
     """Oracle ladder (4 ZGEMMs per channel, host BLAS) on a bounded sample of q."""
     import numpy as np
@@ -122,7 +136,8 @@ def main():
     flops_per_launch = 16.0 * D ** 3 * 4 * q_per_launch  # 2 complex DxD GEMMs per (q, channel)
     achieved = flops_per_launch / (lad_ms / lad_n / 1e3) / 1e12
     roof64 = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-              "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+              "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+              "traffic": profile_traffic("bse_ladder_fp64_D320_nq1024.md"),
               "kernel": f"bse_ladder_kernel variant {variant}", "avg_launch_ms": round(lad_ms / lad_n, 4),
               "flops_per_launch": flops_per_launch}
 
@@ -143,7 +158,8 @@ def main():
     ops_per_launch = 2.0 * args.moduli * 2.0 * D * D * (2 * D) * 4 * q_per_gemm
     ach_sp = ops_per_launch / (gemm_ms / gemm_n / 1e3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(ach_sp, 1), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                "frac": round(ach_sp / INT8_PEAK_TOPS, 4), "traffic": None,
+                "frac": round(ach_sp / INT8_PEAK_TOPS, 4),
+                "traffic": profile_traffic("bse_split_crt_D320_nq1024.md"),
                 "kernel": f"crt_gemm ({args.moduli} moduli)", "avg_launch_ms": round(gemm_ms / gemm_n, 4),
                 "ops_per_launch": ops_per_launch, "prep_ms_per_launch": round(prep_ms / prep_n, 4),
                 "launches_per_step": gemm_n // nprof}
@@ -201,7 +217,8 @@ def main():
     per_call_ms = (pae_ms + gat_ms) / pae_n
     gbs = cbytes / (per_call_ms / 1e3) / 1e9
     crossing = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "traffic": profile_traffic("crossing_pae_nl1_nx16_nf64.md"),
                 "kernel": f"pae_stream_nl1 variant {cvar} (+ pae_pregather_nl1)",
                 "avg_launch_ms": round(pae_ms / pae_n, 4), "pregather_ms": round(gat_ms / max(1, gat_n), 4),
                 "bytes_per_call": cbytes,

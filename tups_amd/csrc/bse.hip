@@ -58,14 +58,19 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-template <int NL, int G, bool DB>
+// K3: the three-multiplication complex product (Re A + Im A)(Re B + Im B) - ...:
+// P1 = Re A Re B, P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B),
+// Re = P1 - P2, Im = P3 - P1 - P2 -- three real MFMAs per complex block instead
+// of four, with a third LDS plane holding Re + Im of each operand.
+template <int NL, int G, bool DB, bool K3 = false>
 __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
   constexpr int KC = NL * G;   // inner-dimension chunk: G frequencies x NL form factors
   constexpr int PP = G / 4;    // (row|col, nu) pairs per thread: 64 * G / 256
   constexpr int NB = DB ? 2 : 1;
+  constexpr int NPL = K3 ? 3 : 2;  // LDS planes per operand: Re, Im (, Re + Im)
   static_assert(KC % 4 == 0, "chunk must be a multiple of the MFMA k");
-  __shared__ double As[NB][2][KC][BM];
-  __shared__ double Bs[NB][2][KC][BN];
+  __shared__ double As[NB][NPL][KC][BM];
+  __shared__ double Bs[NB][NPL][KC][BN];
 
   const int tiles = a.tiles_n * a.tiles_n;
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
@@ -90,13 +95,15 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
   const int li = lane & 15;
   const int lk = lane >> 4;
 
-  d4 acc_re[2][2], acc_im[2][2];
+  // K3: acc_re = P1, acc_im = P3, acc_p2 = P2
+  d4 acc_re[2][2], acc_im[2][2], acc_p2[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       acc_re[i][j] = d4{0.0, 0.0, 0.0, 0.0};
       acc_im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      acc_p2[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     }
 
   const int nch = (nf + G - 1) / G;
@@ -156,6 +163,7 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
         }
         As[sb][0][l2 * G + g][i] = alpha * sr;
         As[sb][1][l2 * G + g][i] = alpha * si;
+        if (K3) As[sb][NPL - 1][l2 * G + g][i] = alpha * sr + alpha * si;
       }
     }
 #pragma unroll
@@ -166,6 +174,7 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
       for (int l1 = 0; l1 < NL; ++l1) {
         Bs[sb][0][l1 * G + g][j] = rb[p][l1].x;
         Bs[sb][1][l1 * G + g][j] = rb[p][l1].y;
+        if (K3) Bs[sb][NPL - 1][l1 * G + g][j] = rb[p][l1].x + rb[p][l1].y;
       }
     }
   };
@@ -174,23 +183,33 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
 #pragma unroll
     for (int ks = 0; ks < KC / 4; ++ks) {
       const int kr = ks * 4 + lk;
-      double ar[2], ai[2], br[2], bi[2];
+      double ar[2], ai[2], br[2], bi[2], as[2], bs[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         ar[i] = As[buf][0][kr][wr + i * 16 + li];
         ai[i] = As[buf][1][kr][wr + i * 16 + li];
         br[i] = Bs[buf][0][kr][wc + i * 16 + li];
         bi[i] = Bs[buf][1][kr][wc + i * 16 + li];
+        if (K3) {
+          as[i] = As[buf][NPL - 1][kr][wr + i * 16 + li];
+          bs[i] = Bs[buf][NPL - 1][kr][wc + i * 16 + li];
+        }
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc_re[i][j], 0, 0, 0);
-          // blgp bit 0 negates the A operand of an F64 MFMA: acc -= Im(A) Im(B)
-          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acc_re[i][j], 0, 0, 1);
-          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acc_im[i][j], 0, 0, 0);
-          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acc_im[i][j], 0, 0, 0);
+          if (K3) {
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc_re[i][j], 0, 0, 0);
+            acc_p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acc_p2[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[i], bs[j], acc_im[i][j], 0, 0, 0);
+          } else {
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc_re[i][j], 0, 0, 0);
+            // blgp bit 0 negates the A operand of an F64 MFMA: acc -= Im(A) Im(B)
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acc_re[i][j], 0, 0, 1);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acc_im[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acc_im[i][j], 0, 0, 0);
+          }
         }
     }
   };
@@ -233,8 +252,12 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
           const size_t idx = (size_t)gj * D + gi;
           const double2 f = Fm[idx];
           const double2 g = Gm[idx];
-          O[idx] = make_double2(cf * acc_re[i][j][e] + a.omf * (f.x - g.x),
-                                cf * acc_im[i][j][e] + a.omf * (f.y - g.y));
+          double tr = acc_re[i][j][e], ti = acc_im[i][j][e];
+          if (K3) {
+            ti = ti - tr - acc_p2[i][j][e];
+            tr = tr - acc_p2[i][j][e];
+          }
+          O[idx] = make_double2(cf * tr + a.omf * (f.x - g.x), cf * ti + a.omf * (f.y - g.y));
         }
       }
     }
@@ -258,6 +281,10 @@ static const BseVariant kBseVariants[] = {
     {5, bse_ladder_kernel<5, 8, false>, "nl5_g8"},
     {9, bse_ladder_kernel<9, 4, false>, "nl9_g4"},
     {9, bse_ladder_kernel<9, 4, true>, "nl9_g4_db"},
+    {1, bse_ladder_kernel<1, 16, false, true>, "nl1_g16_3m"},
+    {5, bse_ladder_kernel<5, 4, false, true>, "nl5_g4_3m"},
+    {5, bse_ladder_kernel<5, 8, false, true>, "nl5_g8_3m"},
+    {9, bse_ladder_kernel<9, 4, false, true>, "nl9_g4_3m"},
 };
 constexpr int kNumBseVariants = sizeof(kBseVariants) / sizeof(kBseVariants[0]);
 

@@ -39,7 +39,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int CRT_MAX = 16;      // most moduli
 constexpr int CRT_TILE = 64;     // output tile of one workgroup
-constexpr int CRT_ROW = 80;      // padded LDS row of a 64-byte K chunk
 
 // pairwise coprime, all = 1 (mod 4): 221 = 13 * 17, 205 = 5 * 41, the rest prime
 constexpr int kModuli[CRT_MAX] = {241, 233, 229, 221, 205, 197, 193, 181,
@@ -50,9 +49,10 @@ struct CrtArgs {
   const double2* fv[4];
   double2* out[4];
   const double2* chi[2];
-  int8_t* A;                    // [mat][mod][sign][Dp rows][Kp], K contiguous
-  int8_t* B;                    // [mat][mod][sign][Dp cols][Kp], K contiguous
+  int8_t* A;                    // [mat][mod][sign][Kp / 64][Dp rows][64]: 64-byte K chunks
+  int8_t* B;                    // [mat][mod][sign][Kp / 64][Dp cols][64]
   int* escale;                  // [mat][Dp] row exponents of A
+  double* rmax;                 // [mat][8][Dp] partial row maxima of A
   int* fscale;                  // [mat][Dp] column exponents of B
   int D, Dp, Kp, nf, nl;
   int q0, nqb, tiles_n;
@@ -65,8 +65,11 @@ struct CrtArgs {
   double mhi, mlo, inv_mm;
 };
 
-__device__ __forceinline__ size_t ws_off(const CrtArgs& a, int mat, int mod, int sign, int rc) {
-  return (((size_t)mat * a.nmod + mod) * 2 + sign) * (size_t)a.Dp * a.Kp + (size_t)rc * a.Kp;
+// byte offset of inner index k of row (column) rc: a 64-row tile of one K chunk
+// is 4 KiB contiguous
+__device__ __forceinline__ size_t ws_off(const CrtArgs& a, int mat, int mod, int sign, int rc, int k) {
+  return (((size_t)mat * a.nmod + mod) * 2 + sign) * (size_t)a.Dp * a.Kp + (size_t)(k >> 6) * a.Dp * 64 +
+         (size_t)rc * 64 + (k & 63);
 }
 
 __device__ __forceinline__ int xcd_contiguous(int bid, int nblk) {
@@ -96,8 +99,10 @@ __device__ __forceinline__ void gauss_residues(const CrtArgs& a, int mod, double
   rm = (int)sym_mod(ym, m, im) & 0xff;
 }
 
-// A = [Gamma | F] rows: exponent e_i of each row maximum.  One lane per row,
-// the four waves split the inner dimension.  grid: (Dp / 64, 4 * nqb)
+// A = [Gamma | F] rows: partial row maxima over 8 blocks of the inner
+// dimension (lane = row; the four waves split a block).
+// grid: ((Dp / 64) * 8, 4 * nqb); out: rmax[mat][8][Dp]
+constexpr int CRT_KB = 8;
 __global__ __launch_bounds__(256) void crt_rowmax_a(CrtArgs a) {
   __shared__ double red[4][64];
   const int mat = blockIdx.y;  // r * nqb + qb
@@ -106,25 +111,24 @@ __global__ __launch_bounds__(256) void crt_rowmax_a(CrtArgs a) {
   const size_t DD = (size_t)D * D;
   const double2* G = a.gam[r] + DD * q;
   const double2* F = a.fv[r] + DD * q;
+  const int rb = blockIdx.x / CRT_KB, kb = blockIdx.x - rb * CRT_KB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int row = blockIdx.x * 64 + lane;
+  const int row = rb * 64 + lane;
+  const int kw = (2 * D + 4 * CRT_KB - 1) / (4 * CRT_KB);  // inner indices per wave
+  const int k0 = (kb * 4 + wave) * kw, k1 = min(2 * D, k0 + kw);
   double mx = 0.0;
   if (row < D) {
-    const int kq = (2 * D + 3) / 4;
-    const int k1 = min(2 * D, (wave + 1) * kq);
-    for (int k = wave * kq; k < k1; ++k) {
+#pragma unroll 4
+    for (int k = k0; k < k1; ++k) {
       const double2 v = k < D ? G[(size_t)k * D + row] : F[(size_t)(k - D) * D + row];
       mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
     }
   }
   red[wave][lane] = mx;
   __syncthreads();
-  if (wave == 0) {
-    mx = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
-    int e = 0;
-    if (mx > 0.0) frexp(mx, &e);
-    a.escale[(size_t)mat * a.Dp + row] = e;
-  }
+  if (wave == 0)
+    a.rmax[((size_t)mat * CRT_KB + kb) * a.Dp + row] =
+        fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
 }
 
 // A = [Gamma | F] residues: one workgroup per 64 rows x 64 inner indices; lane
@@ -143,7 +147,12 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
   const int row = rb * 64 + lane;
   const bool rok = row < D;
   const int k0 = kc * 64 + wave * 16;
-  const int e = a.escale[(size_t)mat * a.Dp + row];
+  double mx = 0.0;
+#pragma unroll
+  for (int kb = 0; kb < CRT_KB; ++kb) mx = fmax(mx, a.rmax[((size_t)mat * CRT_KB + kb) * a.Dp + row]);
+  int e = 0;
+  if (mx > 0.0) frexp(mx, &e);
+  if (kc == 0 && wave == 0) a.escale[(size_t)mat * a.Dp + row] = e;
   const double sc = ldexp(1.0, a.ta - e);
   double r1[16], r0[16], i1[16], i0[16];
 #pragma unroll
@@ -163,8 +172,8 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
       wp[u >> 2] |= bp << (8 * (u & 3));
       wm[u >> 2] |= bm << (8 * (u & 3));
     }
-    *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 0, row) + k0) = v4i{wp[0], wp[1], wp[2], wp[3]};
-    *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 1, row) + k0) = v4i{wm[0], wm[1], wm[2], wm[3]};
+    *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 0, row, k0)) = v4i{wp[0], wp[1], wp[2], wp[3]};
+    *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 1, row, k0)) = v4i{wm[0], wm[1], wm[2], wm[3]};
   }
 }
 
@@ -173,11 +182,13 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
 // X and chi) into LDS, then lane l converts inner indices 4 l + 256 n .. + 3
 // and stores one dword per (modulus, sign).  grid: (Dp / 4, 4 * nqb);
 // dynamic LDS 4 * Kp * 16 bytes.
+template <int NL>
 __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
   extern __shared__ double2 colv[];  // [wave][Kp]
   const int mat = blockIdx.y;
   const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
-  const int D = a.D, nf = a.nf, nl = a.nl;
+  const int D = a.D, nf = a.nf;
+  constexpr int nl = NL;
   const size_t DD = (size_t)D * D;
   const double2* chiq = a.chi[r >> 1] + (size_t)nf * nl * nl * q;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -186,24 +197,30 @@ __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
   double2* v = colv + (size_t)wave * a.Kp;
   // B((part, l1, nu), col) = alpha sum_l3 chi_nu(l1, l3) X((l3, nu), col), X = F (part 0), Gamma (part 1)
   double mx = 0.0;
-  for (int part = 0; part < 2; ++part) {
-    const double2* X = (part ? a.gam[r] : a.fv[r]) + DD * q + (size_t)col * D;
-    for (int nu = lane; nu < nf; nu += 64) {
-      const double2* cb = chiq + (size_t)nu * nl * nl;
-      for (int l1 = 0; l1 < nl; ++l1) {
-        double sr = 0.0, si = 0.0;
-        if (cok) {
-          for (int l3 = 0; l3 < nl; ++l3) {
-            const double2 c = cb[l1 + nl * l3];
-            const double2 x = X[l3 * nf + nu];
-            sr = fma(c.x, x.x, fma(-c.y, x.y, sr));
-            si = fma(c.x, x.y, fma(c.y, x.x, si));
-          }
-        }
-        const double2 m = make_double2(a.alpha * sr, a.alpha * si);
-        v[part * D + l1 * nf + nu] = m;
-        mx = fmax(mx, fmax(fabs(m.x), fabs(m.y)));
+  for (int nu = lane; nu < nf; nu += 64) {
+    const double2* cb = chiq + (size_t)nu * nl * nl;
+    double2 xf[NL], xg[NL];
+#pragma unroll
+    for (int l3 = 0; l3 < NL; ++l3) {
+      xf[l3] = cok ? a.fv[r][DD * q + (size_t)col * D + l3 * nf + nu] : make_double2(0.0, 0.0);
+      xg[l3] = cok ? a.gam[r][DD * q + (size_t)col * D + l3 * nf + nu] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int l1 = 0; l1 < NL; ++l1) {
+      double fr = 0.0, fi = 0.0, gr = 0.0, gi = 0.0;
+#pragma unroll
+      for (int l3 = 0; l3 < NL; ++l3) {
+        const double2 c = cb[l1 + nl * l3];
+        fr = fma(c.x, xf[l3].x, fma(-c.y, xf[l3].y, fr));
+        fi = fma(c.x, xf[l3].y, fma(c.y, xf[l3].x, fi));
+        gr = fma(c.x, xg[l3].x, fma(-c.y, xg[l3].y, gr));
+        gi = fma(c.x, xg[l3].y, fma(c.y, xg[l3].x, gi));
       }
+      const double2 mf = make_double2(a.alpha * fr, a.alpha * fi);
+      const double2 mg = make_double2(a.alpha * gr, a.alpha * gi);
+      v[l1 * nf + nu] = mf;
+      v[D + l1 * nf + nu] = mg;
+      mx = fmax(mx, fmax(fmax(fabs(mf.x), fabs(mf.y)), fmax(fabs(mg.x), fabs(mg.y))));
     }
   }
   for (int k = 2 * D + lane; k < a.Kp; k += 64) v[k] = make_double2(0.0, 0.0);
@@ -230,20 +247,25 @@ __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
         wp |= bp << (8 * u);
         wm |= bm << (8 * u);
       }
-      *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 0, col) + k0) = wp;
-      *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 1, col) + k0) = wm;
+      *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 0, col, k0)) = wp;
+      *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 1, col, k0)) = wm;
     }
   }
 }
 
 // One workgroup per 64 x 64 tile of one (q, r); four waves, each 32 x 32 =
 // 2 x 2 blocks of v_mfma_i32_16x16x64_i8.  The loop runs over (modulus, K chunk)
-// steps with the next step's A and B chunks prefetched into registers and
-// double-buffered in LDS; after a modulus' last chunk its residues join the
-// CRT sums, held per output element as exact high and rounded low parts.
+// steps through a ring of four LDS stages (A+, A-, B+, B- chunks of 64 rows x
+// 64 bytes, 16 KiB; the 16-byte columns of each row XOR-swizzled by row / 4 so
+// the fragment reads are free of bank conflicts).  Each loop iteration computes
+// two steps from two stages, then stores the two steps that arrived in
+// registers during it into the two free stages and issues the loads of the two
+// after them: every load has a full iteration to arrive, and all waits come
+// before the next loads in program order.  After a modulus' last chunk its
+// residues join the CRT sums, held per output element as exact high and
+// rounded low parts.
 __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
-  __shared__ __attribute__((aligned(16))) int8_t As[2][2][CRT_TILE][CRT_ROW];
-  __shared__ __attribute__((aligned(16))) int8_t Bs[2][2][CRT_TILE][CRT_ROW];
+  __shared__ __attribute__((aligned(16))) int8_t S[4][4][CRT_TILE][64];  // [stage][piece][row][byte]
   const int tiles = a.tiles_n * a.tiles_n;
   // workgroups go round-robin over the 8 XCDs: give each XCD a contiguous run
   // of logical tiles, so all tiles of one matrix share that XCD's L2
@@ -256,21 +278,27 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
   const int li = lane & 15, lk = lane >> 4;
   const int nk = a.Kp / 64;
   const int nsteps = a.nmod * nk;
-  // staging: thread tid moves 16 bytes of row (tid >> 2), chunk (tid & 3), for
+  // staging: thread tid moves 16 bytes (column tid & 3) of row tid >> 2 of each
   // piece v = 2 * which + sign (which 0: A rows, 1: B columns)
-  const int srow = tid >> 2, sch = (tid & 3) * 16;
+  const int srow = tid >> 2, sch = tid & 3;
+  const int sput = 16 * (sch ^ ((srow >> 2) & 3));
   auto gptr = [&](int step, int v) -> const v4i* {
     const int md = step / nk, k0 = (step - md * nk) * 64;
     const int which = v >> 1, sign = v & 1;
     const int8_t* base = which ? a.B : a.A;
     const int rc = (which ? col0 : row0) + srow;
-    return reinterpret_cast<const v4i*>(base + ws_off(a, mat, md, sign, rc) + k0 + sch);
+    return reinterpret_cast<const v4i*>(base + ws_off(a, mat, md, sign, rc, k0 + 16 * sch));
   };
-  auto lds_put = [&](int buf, const v4i* pf) {
-    *reinterpret_cast<v4i*>(&As[buf][0][srow][sch]) = pf[0];
-    *reinterpret_cast<v4i*>(&As[buf][1][srow][sch]) = pf[1];
-    *reinterpret_cast<v4i*>(&Bs[buf][0][srow][sch]) = pf[2];
-    *reinterpret_cast<v4i*>(&Bs[buf][1][srow][sch]) = pf[3];
+  auto load = [&](int step, v4i* r) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) r[v] = *gptr(step, v);
+  };
+  auto put = [&](int st, const v4i* r) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) *reinterpret_cast<v4i*>(&S[st][v][srow][sput]) = r[v];
+  };
+  auto frag = [&](int st, int v, int row) -> v4i {
+    return *reinterpret_cast<const v4i*>(&S[st][v][row][16 * (lk ^ ((row >> 2) & 3))]);
   };
   double shr[16], slr[16], shi[16], sli[16];  // CRT sums, element (i, j, e) -> 8 i + 4 j + e
 #pragma unroll
@@ -282,14 +310,14 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
     for (int j = 0; j < 2; ++j) accp[i][j] = accm[i][j] = v4i{0, 0, 0, 0};
 
   int kstep = 0, mod = 0;
-  auto compute = [&](int buf) {
+  auto compute = [&](int st) {
     v4i ap[2], am[2], bp[2], bm[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      ap[i] = *reinterpret_cast<const v4i*>(&As[buf][0][wr + i * 16 + li][16 * lk]);
-      am[i] = *reinterpret_cast<const v4i*>(&As[buf][1][wr + i * 16 + li][16 * lk]);
-      bp[i] = *reinterpret_cast<const v4i*>(&Bs[buf][0][wc + i * 16 + li][16 * lk]);
-      bm[i] = *reinterpret_cast<const v4i*>(&Bs[buf][1][wc + i * 16 + li][16 * lk]);
+      ap[i] = frag(st, 0, wr + i * 16 + li);
+      am[i] = frag(st, 1, wr + i * 16 + li);
+      bp[i] = frag(st, 2, wc + i * 16 + li);
+      bm[i] = frag(st, 3, wc + i * 16 + li);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -326,32 +354,23 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
     }
   };
 
-  // two register sets keep the loads of the next two steps in flight: at
-  // step s, P (or Q) holds step s + 1's chunks and the other set receives s + 2's
-  v4i P[4], Q[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) P[v] = *gptr(0, v);
-  lds_put(0, P);
-  if (nsteps > 1) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) P[v] = *gptr(1, v);
-  }
+  // prologue: steps 0 and 1 into stages 0 and 1; steps 2 and 3 in flight
+  v4i R0[4], R1[4];
+  load(0, R0);
+  if (nsteps > 1) load(1, R1);
+  put(0, R0);
+  if (nsteps > 1) put(1, R1);
+  if (nsteps > 2) load(2, R0);
+  if (nsteps > 3) load(3, R1);
   __syncthreads();
-  for (int step = 0; step < nsteps; step += 2) {
-    if (step + 2 < nsteps) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Q[v] = *gptr(step + 2, v);
-    }
-    compute(0);
-    if (step + 1 < nsteps) lds_put(1, P);
-    __syncthreads();
-    if (step + 1 >= nsteps) break;
-    if (step + 3 < nsteps) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) P[v] = *gptr(step + 3, v);
-    }
-    compute(1);
-    if (step + 2 < nsteps) lds_put(0, Q);
+  for (int s = 0; s < nsteps; s += 2) {
+    compute(s & 3);
+    if (s + 1 < nsteps) compute((s + 1) & 3);
+    // stages (s + 2) & 3 and (s + 3) & 3 were last read two iterations ago
+    if (s + 2 < nsteps) put((s + 2) & 3, R0);
+    if (s + 3 < nsteps) put((s + 3) & 3, R1);
+    if (s + 4 < nsteps) load(s + 4, R0);
+    if (s + 5 < nsteps) load(s + 5, R1);
     __syncthreads();
   }
 
@@ -461,7 +480,7 @@ int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
   const size_t Dp = ((size_t)s.nz + 63) / 64 * 64;
   const size_t Kp = (2 * (size_t)s.nz + 63) / 64 * 64;
   // per q point: 4 channels x (A and B) x 2 signs x nmod residue planes + exponents
-  const size_t per_q = 4 * (4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int));
+  const size_t per_q = 4 * (4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int) + CRT_KB * Dp * sizeof(double));
   // default pass: about 1 GiB of residues
   const size_t want = c->split_qbatch > 0 ? (size_t)c->split_qbatch : ((size_t)1 << 30) / per_q;
   const int qbatch = (int)std::max<size_t>(1, std::min<size_t>(s.nq, want));
@@ -475,8 +494,9 @@ int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
   }
   const size_t lds_b = 4 * Kp * sizeof(double2);  // <= 128 KiB for Nz <= 1024
   if (lds_b > 65536)
-    TUPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(crt_prep_b),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    for (const void* f : {reinterpret_cast<const void*>(crt_prep_b<1>), reinterpret_cast<const void*>(crt_prep_b<5>),
+                          reinterpret_cast<const void*>(crt_prep_b<9>)})
+      TUPS_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
   if (qbatch_out) *qbatch_out = qbatch;
   return TUPS_OK;
 }
@@ -515,16 +535,18 @@ int launch_bse_split(tups_ctx* c) {
     int8_t* base = static_cast<int8_t*>(c->split_ws);
     a.A = base;
     a.B = base + mats * 2 * nmod * Dp * Kp;
-    a.escale = reinterpret_cast<int*>(a.B + mats * 2 * nmod * Dp * Kp);
+    a.rmax = reinterpret_cast<double*>(a.B + mats * 2 * nmod * Dp * Kp);  // 8-byte aligned: Kp % 64 == 0
+    a.escale = reinterpret_cast<int*>(a.rmax + mats * CRT_KB * Dp);
     a.fscale = a.escale + mats * Dp;
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL(crt_rowmax_a, dim3((unsigned)(Dp / 64), (unsigned)mats), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(crt_rowmax_a, dim3((unsigned)(Dp / 64 * CRT_KB), (unsigned)mats), dim3(256), 0, c->stream,
+                       a);
     TUPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(crt_prep_a, dim3((unsigned)((Dp / 64) * (Kp / 64)), (unsigned)mats), dim3(256), 0,
                        c->stream, a);
     TUPS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(crt_prep_b, dim3((unsigned)(Dp / 4), (unsigned)mats), dim3(256),
-                       lds_b, c->stream, a);
+    void (*prep_b)(CrtArgs) = s.nl == 1 ? crt_prep_b<1> : (s.nl == 5 ? crt_prep_b<5> : crt_prep_b<9>);
+    hipLaunchKernelGGL(prep_b, dim3((unsigned)(Dp / 4), (unsigned)mats), dim3(256), lds_b, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_SPLIT_PREP)) return rc;
     const long long nblk = (long long)a.tiles_n * a.tiles_n * mats;

@@ -18,7 +18,7 @@ import sys
 
 import numpy as np
 
-from .runtime import BSE_LADDER_SPLIT, GAMMA, LAMBDA, TupsContext
+from .runtime import BSE_LADDER_SPLIT, GAMMA, K_PAE, LAMBDA, TupsContext
 from .synth import batched_bse_inputs
 
 
@@ -97,6 +97,9 @@ def main(argv=None):
             print(f"calib: {args.steps} launches, fetch {32.0 * n:.6e} B, write {16.0 * n:.6e} B per launch")
     else:
         c = crossing_context(args.nx, args.nf, args.nl)
+        if args.workload == "crossing" and args.nl == 1:
+            name, _ = c.autotune(K_PAE)
+            print(f"crossing: variant {name}")
         for _ in range(args.steps):
             if args.workload == "crossing":
                 c.calculate_PAE()
