@@ -225,6 +225,28 @@ def main():
                 "workload": f"calculate_PAE Nx=16 Nf=64 Nl=1 NR=1, Nq={cx.nq} (all IBZ q), 4 channels"}
     cx.close()
 
+    # crossing at Nl = 5 (Nx = 8, Nf = 64, NR = 4, all 480 IBZ q): the compute-heavy regime
+    from tups_amd.profile_workloads import crossing_flops
+    c5 = crossing_context(8, 64, 5)
+    c5.calculate_PAE()
+    c5.sync()
+    c5.set_timing(True)
+    for _ in range(2):
+        c5.calculate_PAE()
+    p5_ms, p5_n = c5.kernel_time(K_PAE)
+    y5_ms, _ = c5.kernel_time(K_PAE_GATHER)  # the per-qw gather of PhiR into the GEMM operand
+    p5_ms += y5_ms
+    c5.set_timing(False)
+    fl5 = crossing_flops(c5)
+    t5 = p5_ms / p5_n / 1e3
+    crossing5 = {"bound": "mfma", "achieved": round(fl5 / t5 / 1e12, 3), "peak": FP64_PEAK_TFLOPS,
+                 "unit": "TFLOP/s", "frac": round(fl5 / t5 / 1e12 / FP64_PEAK_TFLOPS, 4),
+                 "avg_launch_ms": round(p5_ms / p5_n, 3), "flops_per_call": fl5,
+                 "bytes_per_call": crossing_bytes(c5), "kernel": "pae_ybuild + pae_mm (FP64 MFMA GEMM per bosonic frequency)",
+                 "workload": f"calculate_PAE Nx=8 Nf=64 Nl=5 NR={c5.nr}, Nq={c5.nq} (all IBZ q), 4 channels"}
+    print(f"[bench] crossing Nl=5: {p5_ms / p5_n:.2f} ms per call", file=sys.stderr, flush=True)
+    c5.close()
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(nx, nf, nl, args.cpu_seconds)
@@ -254,6 +276,7 @@ def main():
         "closed_form": closed,
         "roofline": roofline,
         "crossing_roofline": crossing,
+        "crossing_nl5": crossing5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

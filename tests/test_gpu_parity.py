@@ -106,7 +106,10 @@ def test_m_tensor_parity(nx, nl, nr):
         assert err < TOL
 
 
-@pytest.mark.parametrize("nx,nf,nl", [(4, 8, 1), (4, 4, 5), (4, 4, 9), (6, 8, 1)])
+@pytest.mark.parametrize("nx,nf,nl", [(4, 8, 1), (4, 4, 5), (4, 4, 9), (6, 8, 1),
+                                      (4, 8, 5),    # Nf^2 = 64: one exact column tile of the GEMM path
+                                      (4, 12, 5),   # Nf^2 = 144: partial column tile
+                                      (6, 4, 9)])   # Nl^2 NIBZ = 810 rows: many row tiles
 def test_pae_parity(nx, nf, nl):
     from tups_amd import F, GAMMA, LAMBDA, PHIR
 

@@ -246,6 +246,8 @@ int tups_create(const tups_dims* d, tups_ctx** out) {
   }
   for (int p = 0; p < 2 && rc == 0; ++p) rc = dalloc(&c->chi[p], array_elems(c, TUPS_CHI));
   if (!rc && s.full_ibz && s.nl == 1) rc = dalloc(&c->pae_y, (size_t)8 * s.nf * s.nf * s.nw * s.nr);
+  if (!rc && s.full_ibz && s.nl > 1)
+    rc = dalloc(&c->pae_ym, (size_t)8 * s.nf * s.nf * s.nw * s.nl * s.nl * s.nr);
   if (!rc) {
     he = hipMalloc((void**)&c->d_flag, sizeof(int));
     if (he != hipSuccess) rc = TUPS_ENOMEM;
@@ -303,6 +305,7 @@ int tups_destroy(tups_ctx* c) {
   f(c->d_ph_coef);
   f(c->d_flag);
   f(c->pae_y);
+  f(c->pae_ym);
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   for (int i = 0; i < 2; ++i)

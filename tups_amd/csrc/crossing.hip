@@ -21,27 +21,18 @@
 //   sum_{R2,l3,l4} M(l1,l2,l3,l4,Q,R2) * Xsrc((l3,nu_a),(l4,nu_b),w2,R2) / Nx
 // where Xsrc is the linear combination of the source channels' PhiR given by
 // the factor table of SRC/parquet_EQOuter.f90:85-112, and _TR terms carry
-// conj() and signs(l3)*signs(l4).  One thread owns one (q, row, col) entry of a
-// target pair, gathers its terms, adds Lambda*Nc on the l=1 block
-// (SRC/parquet_EQOuter.f90:48-59) and writes F once: F is read (as Phi) and
-// written exactly once, the minimum traffic of the crossing.
+// conj() and signs(l3)*signs(l4).  Both paths below first gather the source
+// PhiR of every routine along its frequency map per bosonic frequency (the
+// term depends on q only through qw), then stream Phi -> F adding the terms
+// and Lambda*Nc on the l=1 block (SRC/parquet_EQOuter.f90:48-59): F is read
+// (as Phi) and written exactly once, the minimum traffic of the crossing.
 #include "internal.h"
 
 namespace tups {
 
-struct PaeArgs {
-  const double2* phi[4];
-  double2* F[4];
-  const double2* phir[4];
-  const double2* M[4];     // phbar_to_ph, pp_to_ph, ph_to_pp, phbar_to_pp
-  const double2* lam[4];
-  const int* signs;
-  int nf, nw, nibz, nr;
-  double inv_nx;
-  double nc_aux;           // |1/FF(1,1,1)^2|
-};
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int clampnu(int x, int nf) { return x < 1 ? 1 : (x > nf ? nf : x); }
 
@@ -77,137 +68,12 @@ __device__ __forceinline__ Term make_term(bool plain_ok, bool tr_ok, int w2p, in
   return t;
 }
 
-// acc_a += wt sum M * (ca0*X0 + ca1*X1); acc_b += wt sum M * (cb0*X0 + cb1*X1)
-template <int NL>
-__device__ __forceinline__ void crossing_term(const PaeArgs& a, const double2* M, int s0, int s1,
-                                              double ca0, double ca1, double cb0, double cb1,
-                                              int l1, int l2, int m, const Term& t,
-                                              double2& acc_a, double2& acc_b) {
-  const int nf = a.nf, nz = nf * NL;
-  const size_t nz2 = (size_t)nz * nz;
-  const double2* p0 = a.phir[s0];
-  const double2* p1 = a.phir[s1];
-  constexpr int NL4 = NL * NL * NL * NL;
-  const double im_sign = t.tr ? -1.0 : 1.0;
-  for (int R = 0; R < a.nr; ++R) {
-    const size_t pbase = nz2 * ((size_t)(t.w2 - 1) + (size_t)a.nw * R);
-    const double2* mb = M + (size_t)NL4 * ((size_t)m + (size_t)a.nibz * R) + l1 + NL * l2;
-    double2 ya_sum = make_double2(0.0, 0.0), yb_sum = make_double2(0.0, 0.0);
-#pragma unroll 1
-    for (int l4 = 0; l4 < NL; ++l4) {
-      const size_t colo = pbase + (size_t)nz * (l4 * nf + t.nub - 1) + (t.nua - 1);
-#pragma unroll
-      for (int l3 = 0; l3 < NL; ++l3) {
-        const size_t pi = colo + (size_t)(l3 * nf);
-        const double2 x0 = p0[pi];
-        const double2 x1 = p1[pi];
-        double sg = t.wt;
-        if (NL > 1) sg *= t.tr ? (double)(a.signs[l3] * a.signs[l4]) : 1.0;
-        const double2 mv = mb[NL * NL * (l3 + NL * l4)];
-        // y = sg * (c0 x0 + c1 x1) with conj for time reversal
-        const double2 ya = make_double2(sg * (ca0 * x0.x + ca1 * x1.x), sg * im_sign * (ca0 * x0.y + ca1 * x1.y));
-        const double2 yb = make_double2(sg * (cb0 * x0.x + cb1 * x1.x), sg * im_sign * (cb0 * x0.y + cb1 * x1.y));
-        ya_sum.x += mv.x * ya.x - mv.y * ya.y;
-        ya_sum.y += mv.x * ya.y + mv.y * ya.x;
-        yb_sum.x += mv.x * yb.x - mv.y * yb.y;
-        yb_sum.y += mv.x * yb.y + mv.y * yb.x;
-      }
-    }
-    acc_a.x += ya_sum.x;
-    acc_a.y += ya_sum.y;
-    acc_b.x += yb_sum.x;
-    acc_b.y += yb_sum.y;
-  }
-}
-
 __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a
   // contiguous logical range (here: a contiguous range of bosonic frequencies)
   const int xcd = bid & 7, idx = bid >> 3;
   const int per = nblk >> 3, rem = nblk & 7;
   return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-}
-
-// 1-D grid remapped per XCD; logical block = ((qw-major q) * 2 + pair) * tiles + tile,
-// tile = 64 rows x (4 * CPT) columns of one (q, pair); a thread owns CPT columns
-// of one row so that each wave keeps several streaming loads in flight.
-constexpr int CPT = 4;
-
-template <int NL>
-__global__ __launch_bounds__(256) void pae_kernel(PaeArgs a) {
-  const int nf = a.nf, nz = nf * NL, nw = a.nw;
-  const size_t nz2 = (size_t)nz * nz;
-  const int tx = (nz + 63) >> 6, ty = (nz + 4 * CPT - 1) / (4 * CPT);
-  const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const int tile = logical % (tx * ty);
-  const int zq = logical / (tx * ty);
-  const int row = (tile % tx) * 64 + (threadIdx.x & 63);
-  const int col0 = (tile / tx) * (4 * CPT) + (threadIdx.x >> 6);
-  if (row >= nz) return;
-  const int pair = zq & 1;  // 0: targets d,m ; 1: targets s,t
-  const int jq = zq >> 1;
-  const int w0 = jq / a.nibz;
-  const int m = jq - w0 * a.nibz;
-  const int q = m * nw + w0;        // Index_bosonic_IBZ order: momentum-major
-  const int qw = w0 + 1;
-  const int l1 = row / nf, nu1 = row - l1 * nf + 1;
-  const int ta = pair == 0 ? 0 : 2, tb = ta + 1;
-
-  // streaming operands of all CPT columns first
-  double2 fa[CPT], fb[CPT];
-#pragma unroll
-  for (int c = 0; c < CPT; ++c) {
-    const int col = col0 + 4 * c;
-    const size_t idx = (size_t)row + (size_t)nz * (col < nz ? col : 0) + nz2 * q;
-    fa[c] = ld_stream(a.phi[ta] + idx);
-    fb[c] = ld_stream(a.phi[tb] + idx);
-  }
-#pragma unroll
-  for (int c = 0; c < CPT; ++c) {
-    const int col = col0 + 4 * c;
-    if (col >= nz) continue;
-    const int l2 = col / nf, nu2 = col - l2 * nf + 1;
-    double2 la = make_double2(0.0, 0.0), lb = make_double2(0.0, 0.0);
-    if (l1 == 0 && l2 == 0) {
-      const size_t li = (size_t)(nu1 - 1) + (size_t)nf * ((size_t)(nu2 - 1) + (size_t)nf * (qw - 1));
-      la = a.lam[ta][li];
-      lb = a.lam[tb][li];
-    }
-    const int dn = nu2 - nu1;
-    // phbar-type term (same frequency geometry for both target pairs)
-    const bool pb_plain = dn >= 0 && dn + 1 <= nw;
-    const bool pb_tr = dn < 0 && 1 - dn <= nw;
-    double2 acc_a = make_double2(0.0, 0.0), acc_b = make_double2(0.0, 0.0);
-    if (pair == 0) {
-      // phbar -> ph from d (-1/2, -1/2) and m (-3/2, +1/2)
-      const Term tb1 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(nu1 + qw - 1, nf), 1 - dn,
-                                 nf + 1 - nu1, clampnu(nf + 1 - (nu1 + qw - 1), nf));
-      // pp -> ph from s (+1/2, -1/2) and t (+3/2, +1/2)
-      const int w2 = nu1 + nu2 + qw - nf - 1, w2t = nf + 3 - qw - nu1 - nu2;
-      const Term tp = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
-                                nf + 1 - nu1, nu1 + qw + w2t - 2);
-      crossing_term<NL>(a, a.M[0], 0, 1, -0.5, -1.5, -0.5, 0.5, l1, l2, m, tb1, acc_a, acc_b);
-      crossing_term<NL>(a, a.M[1], 2, 3, 0.5, 1.5, -0.5, 0.5, l1, l2, m, tp, acc_a, acc_b);
-    } else {
-      // ph -> pp from d (+1/2, +1/2) and m (-3/2, +1/2)
-      const int w2 = qw - nu1 - nu2 + nf + 1, w2t = nu1 + nu2 - qw - nf + 1;
-      const Term th = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
-                                nf + 1 - nu1, nu1 - qw - w2t + 2);
-      // phbar -> pp from d (+1/2, -1/2) and m (-3/2, -1/2)
-      const Term tb2 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(qw - nu1 - dn + nf, nf), 1 - dn,
-                                 nf + 1 - nu1, clampnu(nu1 - qw - (1 - dn) + 2, nf));
-      crossing_term<NL>(a, a.M[2], 0, 1, 0.5, -1.5, 0.5, 0.5, l1, l2, m, th, acc_a, acc_b);
-      crossing_term<NL>(a, a.M[3], 0, 1, 0.5, -1.5, -0.5, -0.5, l1, l2, m, tb2, acc_a, acc_b);
-    }
-    double2 oa = fa[c], ob = fb[c];
-    oa.x += la.x * a.nc_aux + acc_a.x * a.inv_nx;
-    oa.y += la.y * a.nc_aux + acc_a.y * a.inv_nx;
-    ob.x += lb.x * a.nc_aux + acc_b.x * a.inv_nx;
-    ob.y += lb.y * a.nc_aux + acc_b.y * a.inv_nx;
-    const size_t idx = (size_t)row + (size_t)nz * col + nz2 * q;
-    st_stream(a.F[ta] + idx, oa);
-    st_stream(a.F[tb] + idx, ob);
-  }
 }
 
 // ---- Nl = 1: gather pre-pass + streaming crossing ---------------------------------
@@ -506,47 +372,241 @@ int autotune_pae(tups_ctx* c, int reps, int* chosen, float* best_ms) {
   return TUPS_OK;
 }
 
-int launch_pae(tups_ctx* c) {
+// ---- Nl > 1: the crossing as complex GEMMs on FP64 MFMA --------------------------
+//
+// For a target pair p (0: d,m; 1: s,t) each of its two routines k contributes
+//   (1/Nx) sum_{l3,l4,R} M_k(l1,l2,l3,l4,Q,R) Y_kt((l3,l4,R); nu1, nu2, qw)
+// where Y_kt is the routine's PhiR gathered along its frequency map, combined
+// over the source channels with target t's factors, time reversal (conj and
+// signs(l3) signs(l4)) folded in and zero where the routine does not reach the
+// entry: it depends on q only through qw.  With rows r = (l1, l2, Q) (Nl^2
+// NIBZ), columns n = (nu1, nu2) (Nf^2) and the two routines' inner indices
+// (Nl^2 NR each) concatenated, each (qw, pair, target) is one complex GEMM
+//   OUT = [M_k0 | M_k1] [Y_k0t ; Y_k1t]
+// whose epilogue adds Phi and Lambda Nc (l1 = l2 = 1 block) and writes F once.
+
+struct PaeMmArgs {
+  const double2* phi[4];
+  double2* F[4];
+  const double2* lam[4];
+  const double2* M[4];
+  const double2* phir[4];
+  const int* signs;
+  double2* Y;  // [kt = 4 pair + 2 routine + target][qw][j = l3 + Nl l4 + Nl^2 R][n = nu1 + Nf nu2]
+  int nf, nl, nw, nibz, nr, nz, kk, rr, tiles_r, tiles_n;
+  double inv_nx, nc_aux;
+};
+
+// Y for all (kt, qw, j, n): one thread per n for a fixed (j, qw); grid ((Nf^2 + 255) / 256, kk, nw)
+__global__ __launch_bounds__(256) void pae_ybuild(PaeMmArgs a) {
+  const int nf = a.nf, nl = a.nl, nw = a.nw, nz = a.nz;
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= nf * nf) return;
+  const int j = blockIdx.y, w0 = blockIdx.z, qw = w0 + 1;
+  const int l3 = j % nl, l4 = (j / nl) % nl, R = j / (nl * nl);
+  const int nu1 = n % nf + 1, nu2 = n / nf + 1;
+  const double sgn = (double)(a.signs[l3] * a.signs[l4]);
+  auto comb = [&](int s0, int s1, const Term& t, double c0, double c1) {
+    const size_t i = (((size_t)R * nw + (t.w2 - 1)) * nz + (size_t)(l4 * nf + t.nub - 1)) * nz +
+                     (size_t)(l3 * nf + t.nua - 1);
+    const double2 x0 = a.phir[s0][i], x1 = a.phir[s1][i];
+    const double sg = t.wt * (t.tr ? sgn : 1.0);
+    const double im = t.tr ? -1.0 : 1.0;
+    return make_double2(sg * (c0 * x0.x + c1 * x1.x), sg * im * (c0 * x0.y + c1 * x1.y));
+  };
+  const int dn = nu2 - nu1;
+  const bool pb_plain = dn >= 0 && dn + 1 <= nw;
+  const bool pb_tr = dn < 0 && 1 - dn <= nw;
+  const Term t0 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(nu1 + qw - 1, nf), 1 - dn,
+                            nf + 1 - nu1, clampnu(nf + 1 - (nu1 + qw - 1), nf));
+  int w2 = nu1 + nu2 + qw - nf - 1, w2t = nf + 3 - qw - nu1 - nu2;
+  const Term t1 = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
+                            nf + 1 - nu1, nu1 + qw + w2t - 2);
+  w2 = qw - nu1 - nu2 + nf + 1;
+  w2t = nu1 + nu2 - qw - nf + 1;
+  const Term t2 = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
+                            nf + 1 - nu1, nu1 - qw - w2t + 2);
+  const Term t3 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(qw - nu1 - dn + nf, nf), 1 - dn,
+                            nf + 1 - nu1, clampnu(nu1 - qw - (1 - dn) + 2, nf));
+  const size_t nn = (size_t)nf * nf;
+  auto at = [&](int kt) { return ((size_t)(kt * nw + w0) * a.kk + j) * nn + n; };
+  a.Y[at(0)] = comb(0, 1, t0, -0.5, -1.5);  // phbar->ph -> d
+  a.Y[at(1)] = comb(0, 1, t0, -0.5, 0.5);   //           -> m
+  a.Y[at(2)] = comb(2, 3, t1, 0.5, 1.5);    // pp->ph    -> d
+  a.Y[at(3)] = comb(2, 3, t1, -0.5, 0.5);   //           -> m
+  a.Y[at(4)] = comb(0, 1, t2, 0.5, -1.5);   // ph->pp    -> s
+  a.Y[at(5)] = comb(0, 1, t2, 0.5, 0.5);    //           -> t
+  a.Y[at(6)] = comb(0, 1, t3, 0.5, -1.5);   // phbar->pp -> s
+  a.Y[at(7)] = comb(0, 1, t3, -0.5, -0.5);  //           -> t
+}
+
+// One workgroup per 64 x 64 output tile of one (qw, pair, target); four waves of
+// 2 x 2 blocks of v_mfma_f64_16x16x4f64 (four real products per complex block);
+// the inner dimension walks 16 at a time with the next chunk prefetched into
+// registers.  grid: tiles_r * tiles_n * nw * 4 (remapped per XCD).
+constexpr int MMK = 16;
+__global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
+  __shared__ double As[2][MMK][64];
+  __shared__ double Bs[2][MMK][64];
+  const int tiles = a.tiles_r * a.tiles_n;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = logical % tiles;
+  const int z = logical / tiles;  // = (w0 * 2 + pair) * 2 + target
+  const int t = z & 1, p = (z >> 1) & 1, w0 = z >> 2;
+  const int row0 = (tile / a.tiles_n) * 64, col0 = (tile % a.tiles_n) * 64;
+  const int nl2 = a.nl * a.nl, kk = a.kk, ktot = 2 * kk;
+  const size_t nn = (size_t)a.nf * a.nf;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  // staging: A element (row tid & 63, k (tid >> 6) + 4 i), B element (k (tid >> 6) + 4 i, column tid & 63)
+  const int sr = tid & 63, sk = tid >> 6;
+  const int gr = row0 + sr;
+  const bool rok = gr < a.rr;
+  const int rl = rok ? gr % nl2 : 0, rm = rok ? gr / nl2 : 0;
+  const int gn = col0 + sr;
+  const bool nok = gn < (int)nn;
+  double2 ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = k0 + sk + 4 * i;
+      const bool kok = k < ktot;
+      const int rt = k >= kk ? 1 : 0, jj = k - rt * kk;
+      const int jl = jj % nl2, jr = jj / nl2;
+      ra[i] = (rok && kok) ? a.M[2 * p + rt][(size_t)rl + (size_t)nl2 * jl + (size_t)nl2 * nl2 * (rm + (size_t)a.nibz * jr)]
+                           : make_double2(0.0, 0.0);
+      const int kt = 4 * p + 2 * rt + t;
+      rb[i] = (nok && kok) ? a.Y[((size_t)(kt * a.nw + w0) * kk + jj) * nn + gn] : make_double2(0.0, 0.0);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      As[0][sk + 4 * i][sr] = ra[i].x;
+      As[1][sk + 4 * i][sr] = ra[i].y;
+      Bs[0][sk + 4 * i][sr] = rb[i].x;
+      Bs[1][sk + 4 * i][sr] = rb[i].y;
+    }
+  };
+  d4 acc_re[2][2], acc_im[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc_re[i][j] = acc_im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  load(0);
+  for (int k0 = 0; k0 < ktot; k0 += MMK) {
+    store();
+    __syncthreads();
+    if (k0 + MMK < ktot) load(k0 + MMK);
+#pragma unroll
+    for (int ks = 0; ks < MMK / 4; ++ks) {
+      const int kr = ks * 4 + lk;
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = As[0][kr][wr + i * 16 + li];
+        ai[i] = As[1][kr][wr + i * 16 + li];
+        br[i] = Bs[0][kr][wc + i * 16 + li];
+        bi[i] = Bs[1][kr][wc + i * 16 + li];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc_re[i][j], 0, 0, 0);
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acc_re[i][j], 0, 0, 1);
+          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acc_im[i][j], 0, 0, 0);
+          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acc_im[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // epilogue: element e of lane l sits at row l / 16 + 4 e, column l % 16 of its block
+  const int tgt = 2 * p + t, nf = a.nf, nz = a.nz, nl = a.nl;
+  const size_t nz2 = (size_t)nz * nz;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = col0 + wc + j * 16 + li;
+      if (n >= (int)nn) continue;
+      const int nu1 = n % nf, nu2 = n / nf;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = row0 + wr + i * 16 + lk + 4 * e;
+        if (r >= a.rr) continue;
+        const int l = r % nl2, m = r / nl2;
+        const int l1 = l % nl, l2 = l / nl;
+        const size_t q = (size_t)m * a.nw + w0;
+        const size_t idx = q * nz2 + (size_t)(l2 * nf + nu2) * nz + (size_t)(l1 * nf + nu1);
+        double2 o = a.phi[tgt][idx];
+        if (l == 0) {
+          const double2 lv = a.lam[tgt][(size_t)nu1 + (size_t)nf * ((size_t)nu2 + (size_t)nf * w0)];
+          o.x += lv.x * a.nc_aux;
+          o.y += lv.y * a.nc_aux;
+        }
+        o.x += acc_re[i][j][e] * a.inv_nx;
+        o.y += acc_im[i][j][e] * a.inv_nx;
+        a.F[tgt][idx] = o;
+      }
+    }
+}
+
+static int pae_gemm_path(tups_ctx* c) {
   const tups_sizes& s = c->sz;
-  PaeArgs a{};
+  PaeMmArgs a{};
   for (int r = 0; r < 4; ++r) {
     a.phi[r] = c->gamma[r];
     a.F[r] = c->fvert[r];
-    a.phir[r] = c->phir[r];
-    a.M[r] = c->mten[r];
     a.lam[r] = c->lam[r];
+    a.M[r] = c->mten[r];
+    a.phir[r] = c->phir[r];
   }
   a.signs = c->d_signs;
+  a.Y = c->pae_ym;
   a.nf = s.nf;
+  a.nl = s.nl;
   a.nw = s.nw;
   a.nibz = s.nibz;
   a.nr = s.nr;
+  a.nz = s.nz;
+  a.kk = s.nl * s.nl * s.nr;
+  a.rr = s.nl * s.nl * s.nibz;
+  a.tiles_r = (a.rr + 63) / 64;
+  a.tiles_n = (s.nf * s.nf + 63) / 64;
   a.inv_nx = 1.0 / s.nx;
   const double f0 = c->tab.ff[0];
   a.nc_aux = std::fabs(1.0 / f0 * 1.0 / f0);
-  if (s.nl == 1 && c->pae_y) {
-    int v = c->pae_variant;
-    if (!pae_variant_ok(c, v)) v = s.nr == 1 ? 1 : 3;
-    if (int rc = timer_begin(c)) return rc;
-    if (int rc = pae_pregather(c)) return rc;
-    if (int rc = timer_end(c, TUPS_K_PAE_GATHER)) return rc;
-    const PaeNl1Args b = nl1_args(c);
-    if (int rc = timer_begin(c)) return rc;
-    if (int rc = pae_nl1_launch(c, v, b)) return rc;
-    if (int rc = timer_end(c, TUPS_K_PAE)) return rc;
-    return TUPS_OK;
-  }
-  const long long nblk = (long long)((s.nz + 63) / 64) * ((s.nz + 4 * CPT - 1) / (4 * CPT)) * 2 * s.nq;
-  TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "crossing grid too large");
-  dim3 g((unsigned)nblk);
   if (int rc = timer_begin(c)) return rc;
-  if (s.nl == 1)
-    hipLaunchKernelGGL(pae_kernel<1>, g, dim3(256), 0, c->stream, a);
-  else if (s.nl == 5)
-    hipLaunchKernelGGL(pae_kernel<5>, g, dim3(256), 0, c->stream, a);
-  else
-    hipLaunchKernelGGL(pae_kernel<9>, g, dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(pae_ybuild, dim3((unsigned)((s.nf * s.nf + 255) / 256), (unsigned)a.kk, (unsigned)s.nw),
+                     dim3(256), 0, c->stream, a);
   TUPS_HIP(hipGetLastError());
+  if (int rc = timer_end(c, TUPS_K_PAE_GATHER)) return rc;
+  const long long nblk = (long long)a.tiles_r * a.tiles_n * s.nw * 4;
+  TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "crossing grid too large");
+  if (int rc = timer_begin(c)) return rc;
+  hipLaunchKernelGGL(pae_mm, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+  TUPS_HIP(hipGetLastError());
+  if (int rc = timer_end(c, TUPS_K_PAE)) return rc;
+  return TUPS_OK;
+}
+
+int launch_pae(tups_ctx* c) {
+  const tups_sizes& s = c->sz;
+  if (s.nl > 1) {
+    TUPS_CHECK(c->pae_ym, TUPS_ESTATE, "crossing operand buffer missing (not a full-IBZ context)");
+    return pae_gemm_path(c);
+  }
+  TUPS_CHECK(c->pae_y, TUPS_ESTATE, "crossing operand buffer missing (not a full-IBZ context)");
+  int v = c->pae_variant;
+  if (!pae_variant_ok(c, v)) v = s.nr == 1 ? 1 : 3;
+  if (int rc = timer_begin(c)) return rc;
+  if (int rc = pae_pregather(c)) return rc;
+  if (int rc = timer_end(c, TUPS_K_PAE_GATHER)) return rc;
+  const PaeNl1Args b = nl1_args(c);
+  if (int rc = timer_begin(c)) return rc;
+  if (int rc = pae_nl1_launch(c, v, b)) return rc;
   if (int rc = timer_end(c, TUPS_K_PAE)) return rc;
   return TUPS_OK;
 }
