@@ -247,6 +247,27 @@ def main():
     print(f"[bench] crossing Nl=5: {p5_ms / p5_n:.2f} ms per call", file=sys.stderr, flush=True)
     c5.close()
 
+    # batching form: 1024 q with Nf_q 64 -> 16 by bosonic frequency and Nl_q in {1, 5, 9}
+    from tups_amd.profile_workloads import batch_context, batch_flops
+    bt = batch_context()
+    for _ in range(args.warmup):
+        bt.BSE()
+    bt.sync()
+    bt_ms = sum(bt.BSE(timed=True) for _ in range(args.steps)) / args.steps
+    bfl = batch_flops(bt)
+    dims = bt.dims()
+    batching = {"value": round(bt.nq / (bt_ms / 1e3), 2), "unit": "q-points/s", "ms_per_call": round(bt_ms, 3),
+                "roofline": {"bound": "mfma", "achieved": round(bfl / (bt_ms / 1e3) / 1e12, 3),
+                             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": round(bfl / (bt_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                             "flops_per_launch": bfl},
+                "kernel": "bse_batch_kernel (one persistent launch, all q and channels)",
+                "items": bt.items,
+                "workload": f"Nx=8, Nq={bt.nq} (64 momenta x 16 bosonic freq), Nf_q 64->16 by frequency, "
+                            f"Nl_q 1/5/9 by momentum, D {min(dims)}..{max(dims)}, FP64"}
+    print(f"[bench] batching: {bt_ms:.2f} ms per launch", file=sys.stderr, flush=True)
+    bt.close()
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(nx, nf, nl, args.cpu_seconds)
@@ -277,6 +298,7 @@ def main():
         "roofline": roofline,
         "crossing_roofline": crossing,
         "crossing_nl5": crossing5,
+        "batching": batching,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

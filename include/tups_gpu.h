@@ -155,6 +155,23 @@ int tups_set_split(tups_ctx* ctx, int32_t moduli, int32_t qbatch);
  * it afterwards.  Needs a full-IBZ context with the M tensors built. */
 int tups_iteration(tups_ctx* ctx, int32_t mode, int32_t use_graph);
 
+/* --- batching form (no counterpart in the reference, which fixes Nf and Nl for
+ * every q): a batch of nq transfer points, each with its own even Nf_q and
+ * Nl_q in {1, 5, 9}, D_q = Nl_q Nf_q.  GAMMA and F (4 channels) are packed q
+ * after q as D_q x D_q column-major blocks, CHI (ph: 0, pp: 1) as (Nl_q, Nl_q,
+ * Nf_q) blocks.  tups_batch_bse runs the damped ladder of every (q, channel)
+ * in one persistent launch, largest D_q first; Phi replaces GAMMA as in
+ * tups_bse.  `ms` (may be NULL) receives the launch time. */
+typedef struct tups_batch tups_batch;
+int tups_batch_create(int32_t nx, int32_t nq, const int32_t* nf, const int32_t* nl, double beta,
+                      double f_damping, tups_batch** out);
+int tups_batch_destroy(tups_batch* batch);
+int tups_batch_query(const tups_batch* batch, int64_t* vertex_elems, int64_t* chi_elems, int32_t* items);
+int tups_batch_upload(tups_batch* batch, int32_t which, int32_t channel, const double* host);
+int tups_batch_download(tups_batch* batch, int32_t which, int32_t channel, double* host);
+int tups_batch_bse(tups_batch* batch, float* ms);
+int tups_batch_sync(tups_batch* batch);
+
 /* calc_PhiR(ite): PhiR_r from Gamma_r (holding Phi_r); SRC/parquet_PhiR.f90:35-109 */
 int tups_phir(tups_ctx* ctx);
 /* calc_M_*: the four transformation tensors; SRC/parquet_EQMatrix.f90:21-360 */

@@ -62,30 +62,26 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
 // P1 = Re A Re B, P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B),
 // Re = P1 - P2, Im = P3 - P1 - P2 -- three real MFMAs per complex block instead
 // of four, with a third LDS plane holding Re + Im of each operand.
+template <int NL, int G, bool DB, bool K3>
+constexpr int ladder_lds_doubles() {
+  return (DB ? 2 : 1) * (K3 ? 3 : 2) * NL * G * (BM + BN);
+}
+
+// One 64 x 64 tile of Phi for one (q, r) matrix: Gm, Fm, O are that matrix's
+// Gamma, F and output (D x D, column-major), chiq its bubble blocks, lds a
+// workgroup buffer of ladder_lds_doubles<...>() doubles.
 template <int NL, int G, bool DB, bool K3 = false>
-__global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
+__device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, const double2* __restrict__ Fm,
+                                            double2* __restrict__ O, const double2* __restrict__ chiq, int D,
+                                            int nf, double alpha, double cf, double omf, int row0, int col0,
+                                            double* lds) {
   constexpr int KC = NL * G;   // inner-dimension chunk: G frequencies x NL form factors
   constexpr int PP = G / 4;    // (row|col, nu) pairs per thread: 64 * G / 256
   constexpr int NB = DB ? 2 : 1;
   constexpr int NPL = K3 ? 3 : 2;  // LDS planes per operand: Re, Im (, Re + Im)
   static_assert(KC % 4 == 0, "chunk must be a multiple of the MFMA k");
-  __shared__ double As[NB][NPL][KC][BM];
-  __shared__ double Bs[NB][NPL][KC][BN];
-
-  const int tiles = a.tiles_n * a.tiles_n;
-  const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const int mat = logical / tiles;
-  const int t = logical - mat * tiles;
-  const int r = mat / a.nq;
-  const int q = mat - r * a.nq;
-  const int D = a.D, nf = a.nf;
-  const size_t DD = (size_t)D * D;
-  const int row0 = (t / a.tiles_n) * BM;
-  const int col0 = (t % a.tiles_n) * BN;
-  const double2* Gm = a.gam[r] + DD * q;
-  const double2* Fm = a.fv[r] + DD * q;
-  const double2* chiq = a.chi[r >> 1] + (size_t)nf * NL * NL * q;
-  const double alpha = a.alpha;
+  auto As = reinterpret_cast<double (*)[NPL][KC][BM]>(lds);
+  auto Bs = reinterpret_cast<double (*)[NPL][KC][BN]>(lds + NB * NPL * KC * BM);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -238,8 +234,6 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
 
   // epilogue: Phi = c f T + (1 - f)(F - Gamma).  v_mfma_f64_16x16x4f64 leaves
   // element e of lane l at block row (l / 16) + 4 e, column l % 16.
-  const double cf = a.cf[r];
-  double2* O = a.out[r] + DD * q;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -257,10 +251,25 @@ __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
             ti = ti - tr - acc_p2[i][j][e];
             tr = tr - acc_p2[i][j][e];
           }
-          O[idx] = make_double2(cf * tr + a.omf * (f.x - g.x), cf * ti + a.omf * (f.y - g.y));
+          O[idx] = make_double2(cf * tr + omf * (f.x - g.x), cf * ti + omf * (f.y - g.y));
         }
       }
     }
+}
+
+template <int NL, int G, bool DB, bool K3 = false>
+__global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
+  __shared__ double lds[ladder_lds_doubles<NL, G, DB, K3>()];
+  const int tiles = a.tiles_n * a.tiles_n;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int mat = logical / tiles;
+  const int t = logical - mat * tiles;
+  const int r = mat / a.nq;
+  const int q = mat - r * a.nq;
+  const size_t DD = (size_t)a.D * a.D;
+  ladder_tile<NL, G, DB, K3>(a.gam[r] + DD * q, a.fv[r] + DD * q, a.out[r] + DD * q,
+                             a.chi[r >> 1] + (size_t)a.nf * NL * NL * q, a.D, a.nf, a.alpha, a.cf[r], a.omf,
+                             (t / a.tiles_n) * BM, (t % a.tiles_n) * BN, lds);
 }
 
 typedef void (*BseKernel)(BseArgs);
@@ -372,3 +381,271 @@ const char* bse_variant_name(int v) {
 }
 
 }  // namespace tups
+
+namespace tups {
+
+// ---- batching form: q points with their own frequency box and form-factor count ----------
+//
+// No reference counterpart: the reference fixes Nf and Nl for every q.  A batch
+// holds nq points with Nf_q (even) and Nl_q in {1, 5, 9}, D_q = Nl_q Nf_q; the
+// vertex arrays are packed q after q (D_q x D_q column-major each) and the
+// bubbles as (Nl_q, Nl_q, Nf_q) blocks.  One persistent launch covers every
+// (q, channel, 64 x 64 tile): the work items are sorted by D_q, largest first,
+// and workgroup b takes items b, b + grid, ...; each item runs the ladder tile
+// of its Nl (the same code as the uniform path).
+
+namespace {
+struct BatchArgs {
+  const double2* gam[4];
+  const double2* fv[4];
+  double2* out[4];
+  const double2* chi[2];
+  const int4* items;        // (q, r, row0, col0)
+  const long long* voff;    // element offset of q's D_q x D_q block
+  const long long* coff;    // element offset of q's bubble blocks
+  const int* nf;
+  const int* nl;
+  int nitems;
+  double alpha;
+  double cf[4];
+  double omf;
+};
+}  // namespace
+
+constexpr int kBatchLds = ladder_lds_doubles<9, 4, false, false>();
+static_assert(ladder_lds_doubles<5, 4, false, true>() <= kBatchLds, "batch LDS too small");
+static_assert(ladder_lds_doubles<1, 16, false, false>() <= kBatchLds, "batch LDS too small");
+
+__global__ __launch_bounds__(256, 2) void bse_batch_kernel(BatchArgs a) {
+  __shared__ double lds[kBatchLds];
+  for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+    const int4 w = a.items[it];
+    const int q = w.x, r = w.y, nf = a.nf[q], nl = a.nl[q];
+    const long long vo = a.voff[q], co = a.coff[q];
+    const double2* g = a.gam[r] + vo;
+    const double2* f = a.fv[r] + vo;
+    double2* o = a.out[r] + vo;
+    const double2* ch = a.chi[r >> 1] + co;
+    if (nl == 1)
+      ladder_tile<1, 16, false, false>(g, f, o, ch, nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
+    else if (nl == 5)
+      ladder_tile<5, 4, false, true>(g, f, o, ch, 5 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
+    else
+      ladder_tile<9, 4, false, false>(g, f, o, ch, 9 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
+    __syncthreads();  // the next item reuses the LDS buffer
+  }
+}
+
+}  // namespace tups
+
+using namespace tups;
+
+struct tups_batch {
+  int nx = 0, nq = 0;
+  double alpha = 0, f_damping = 0;
+  std::vector<int> nf, nl;
+  std::vector<long long> voff, coff;  // offsets, with totals at [nq]
+  hipStream_t stream = nullptr;
+  double2* gamma[4] = {};
+  double2* fvert[4] = {};
+  double2* spare[4] = {};
+  double2* chi[2] = {};
+  int4* d_items = nullptr;
+  long long* d_voff = nullptr;
+  long long* d_coff = nullptr;
+  int* d_nf = nullptr;
+  int* d_nl = nullptr;
+  int nitems = 0;
+};
+
+extern "C" int tups_batch_destroy(tups_batch* b) {
+  if (!b) return TUPS_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  for (int r = 0; r < 4; ++r) {
+    f(b->gamma[r]);
+    f(b->fvert[r]);
+    f(b->spare[r]);
+  }
+  f(b->chi[0]);
+  f(b->chi[1]);
+  f(b->d_items);
+  f(b->d_voff);
+  f(b->d_coff);
+  f(b->d_nf);
+  f(b->d_nl);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return TUPS_OK;
+}
+
+extern "C" int tups_batch_create(int32_t nx, int32_t nq, const int32_t* nf, const int32_t* nl, double beta,
+                                 double f_damping, tups_batch** out) {
+  TUPS_CHECK(out && nf && nl, TUPS_EINVAL, "null argument");
+  TUPS_CHECK(nx >= 2 && nq >= 1 && beta > 0.0, TUPS_EINVAL, "bad batch dimensions");
+  for (int q = 0; q < nq; ++q) {
+    TUPS_CHECK(nf[q] >= 2 && nf[q] % 2 == 0, TUPS_EINVAL, "Nf_q must be even and >= 2");
+    TUPS_CHECK(nl[q] == 1 || nl[q] == 5 || nl[q] == 9, TUPS_EINVAL, "Nl_q must be 1, 5 or 9");
+  }
+  tups_batch* b = new tups_batch();
+  b->nx = nx;
+  b->nq = nq;
+  b->alpha = 1.0 / (beta * nx * nx);
+  b->f_damping = f_damping;
+  b->nf.assign(nf, nf + nq);
+  b->nl.assign(nl, nl + nq);
+  b->voff.assign(nq + 1, 0);
+  b->coff.assign(nq + 1, 0);
+  for (int q = 0; q < nq; ++q) {
+    const long long D = (long long)nl[q] * nf[q];
+    b->voff[q + 1] = b->voff[q] + D * D;
+    b->coff[q + 1] = b->coff[q] + (long long)nl[q] * nl[q] * nf[q];
+  }
+  // work items, largest D first
+  std::vector<int> order(nq);
+  for (int q = 0; q < nq; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return nl[x] * nf[x] > nl[y] * nf[y]; });
+  std::vector<int4> items;
+  for (int q : order) {
+    const int D = nl[q] * nf[q], tn = (D + BM - 1) / BM;
+    for (int r = 0; r < 4; ++r)
+      for (int t = 0; t < tn * tn; ++t) items.push_back(make_int4(q, r, (t / tn) * BM, (t % tn) * BN));
+  }
+  b->nitems = (int)items.size();
+  auto fail = [&](int rc) {
+    tups_batch_destroy(b);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(TUPS_EHIP);
+  }
+  auto alloc = [&](void** p, size_t bytes) {
+    if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) {
+      set_error("hipMalloc failed for the batch");
+      return TUPS_ENOMEM;
+    }
+    return (int)TUPS_OK;
+  };
+  const size_t vb = (size_t)b->voff[nq] * sizeof(double2), cb = (size_t)b->coff[nq] * sizeof(double2);
+  int rc = 0;
+  for (int r = 0; r < 4 && !rc; ++r) {
+    rc = alloc((void**)&b->gamma[r], vb);
+    if (!rc) rc = alloc((void**)&b->fvert[r], vb);
+    if (!rc) rc = alloc((void**)&b->spare[r], vb);
+  }
+  for (int p = 0; p < 2 && !rc; ++p) rc = alloc((void**)&b->chi[p], cb);
+  if (!rc) rc = alloc((void**)&b->d_items, items.size() * sizeof(int4));
+  if (!rc) rc = alloc((void**)&b->d_voff, (size_t)(nq + 1) * sizeof(long long));
+  if (!rc) rc = alloc((void**)&b->d_coff, (size_t)(nq + 1) * sizeof(long long));
+  if (!rc) rc = alloc((void**)&b->d_nf, (size_t)nq * sizeof(int));
+  if (!rc) rc = alloc((void**)&b->d_nl, (size_t)nq * sizeof(int));
+  if (rc) return fail(rc);
+  if (hipMemcpy(b->d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_voff, b->voff.data(), (nq + 1) * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_coff, b->coff.data(), (nq + 1) * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_nf, nf, nq * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_nl, nl, nq * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("batch table upload failed");
+    return fail(TUPS_EHIP);
+  }
+  *out = b;
+  return TUPS_OK;
+}
+
+extern "C" int tups_batch_query(const tups_batch* b, int64_t* vertex_elems, int64_t* chi_elems, int32_t* items) {
+  TUPS_CHECK(b, TUPS_EINVAL, "null batch");
+  if (vertex_elems) *vertex_elems = b->voff[b->nq];
+  if (chi_elems) *chi_elems = b->coff[b->nq];
+  if (items) *items = b->nitems;
+  return TUPS_OK;
+}
+
+static double2* batch_array(tups_batch* b, int which, int ch, size_t* elems) {
+  if (which == TUPS_GAMMA && ch >= 0 && ch < 4) {
+    *elems = b->voff[b->nq];
+    return b->gamma[ch];
+  }
+  if (which == TUPS_F && ch >= 0 && ch < 4) {
+    *elems = b->voff[b->nq];
+    return b->fvert[ch];
+  }
+  if (which == TUPS_CHI && (ch == 0 || ch == 1)) {
+    *elems = b->coff[b->nq];
+    return b->chi[ch];
+  }
+  return nullptr;
+}
+
+extern "C" int tups_batch_upload(tups_batch* b, int32_t which, int32_t ch, const double* host) {
+  TUPS_CHECK(b && host, TUPS_EINVAL, "null argument");
+  size_t n = 0;
+  double2* p = batch_array(b, which, ch, &n);
+  TUPS_CHECK(p, TUPS_EINVAL, "a batch holds GAMMA, F (4 channels) and CHI (2)");
+  TUPS_HIP(hipMemcpyAsync(p, host, n * sizeof(double2), hipMemcpyHostToDevice, b->stream));
+  TUPS_HIP(hipStreamSynchronize(b->stream));
+  return TUPS_OK;
+}
+
+extern "C" int tups_batch_download(tups_batch* b, int32_t which, int32_t ch, double* host) {
+  TUPS_CHECK(b && host, TUPS_EINVAL, "null argument");
+  size_t n = 0;
+  double2* p = batch_array(b, which, ch, &n);
+  TUPS_CHECK(p, TUPS_EINVAL, "a batch holds GAMMA, F (4 channels) and CHI (2)");
+  TUPS_HIP(hipMemcpyAsync(host, p, n * sizeof(double2), hipMemcpyDeviceToHost, b->stream));
+  TUPS_HIP(hipStreamSynchronize(b->stream));
+  return TUPS_OK;
+}
+
+extern "C" int tups_batch_bse(tups_batch* b, float* ms) {
+  TUPS_CHECK(b, TUPS_EINVAL, "null batch");
+  BatchArgs a{};
+  for (int r = 0; r < 4; ++r) {
+    a.gam[r] = b->gamma[r];
+    a.fv[r] = b->fvert[r];
+    a.out[r] = b->spare[r];
+  }
+  a.chi[0] = b->chi[0];
+  a.chi[1] = b->chi[1];
+  a.items = b->d_items;
+  a.voff = b->d_voff;
+  a.coff = b->d_coff;
+  a.nf = b->d_nf;
+  a.nl = b->d_nl;
+  a.nitems = b->nitems;
+  a.alpha = b->alpha;
+  const double cr[4] = {0.5, 0.5, -0.25, 0.25};
+  for (int r = 0; r < 4; ++r) a.cf[r] = cr[r] * b->f_damping;
+  a.omf = 1.0 - b->f_damping;
+  int dev = 0, ncu = 0;
+  TUPS_HIP(hipGetDevice(&dev));
+  TUPS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grid = std::max(1, std::min(b->nitems, 2 * ncu));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms) {
+    TUPS_HIP(hipEventCreate(&e0));
+    TUPS_HIP(hipEventCreate(&e1));
+    TUPS_HIP(hipEventRecord(e0, b->stream));
+  }
+  hipLaunchKernelGGL(bse_batch_kernel, dim3((unsigned)grid), dim3(256), 0, b->stream, a);
+  TUPS_HIP(hipGetLastError());
+  if (ms) {
+    TUPS_HIP(hipEventRecord(e1, b->stream));
+    TUPS_HIP(hipEventSynchronize(e1));
+    TUPS_HIP(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  for (int r = 0; r < 4; ++r) std::swap(b->gamma[r], b->spare[r]);
+  return TUPS_OK;
+}
+
+extern "C" int tups_batch_sync(tups_batch* b) {
+  TUPS_CHECK(b, TUPS_EINVAL, "null batch");
+  TUPS_HIP(hipStreamSynchronize(b->stream));
+  return TUPS_OK;
+}
+

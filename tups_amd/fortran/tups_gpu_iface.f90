@@ -133,6 +133,59 @@ module tups_gpu_iface
       integer(c_int) :: rc
     end function tups_bse
 
+    function tups_batch_create(nx, nq, nf, nl, beta, f_damping, batch) &
+        bind(C, name="tups_batch_create") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double
+      integer(c_int32_t), value :: nx, nq
+      integer(c_int32_t), intent(in) :: nf(*), nl(*)
+      real(c_double), value :: beta, f_damping
+      type(c_ptr), intent(out) :: batch
+      integer(c_int) :: rc
+    end function tups_batch_create
+
+    function tups_batch_destroy(batch) bind(C, name="tups_batch_destroy") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: batch
+      integer(c_int) :: rc
+    end function tups_batch_destroy
+
+    function tups_batch_query(batch, vertex_elems, chi_elems, items) bind(C, name="tups_batch_query") result(rc)
+      import :: c_ptr, c_int, c_int64_t, c_int32_t
+      type(c_ptr), value :: batch
+      integer(c_int64_t), intent(out) :: vertex_elems, chi_elems
+      integer(c_int32_t), intent(out) :: items
+      integer(c_int) :: rc
+    end function tups_batch_query
+
+    function tups_batch_upload(batch, which, channel, host) bind(C, name="tups_batch_upload") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double_complex
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: which, channel
+      complex(c_double_complex), intent(in) :: host(*)
+      integer(c_int) :: rc
+    end function tups_batch_upload
+
+    function tups_batch_download(batch, which, channel, host) bind(C, name="tups_batch_download") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double_complex
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: which, channel
+      complex(c_double_complex), intent(out) :: host(*)
+      integer(c_int) :: rc
+    end function tups_batch_download
+
+    function tups_batch_bse(batch, ms) bind(C, name="tups_batch_bse") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: batch
+      type(c_ptr), value :: ms
+      integer(c_int) :: rc
+    end function tups_batch_bse
+
+    function tups_batch_sync(batch) bind(C, name="tups_batch_sync") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: batch
+      integer(c_int) :: rc
+    end function tups_batch_sync
+
     function tups_iteration(ctx, mode, use_graph) bind(C, name="tups_iteration") result(rc)
       import :: c_ptr, c_int32_t, c_int
       type(c_ptr), value :: ctx

@@ -5,6 +5,7 @@
     python -m tups_amd.profile_workloads crossing --steps 5   # PAE crossing, Nl=1, Nx=16, Nf=64
     python -m tups_amd.profile_workloads phir     --steps 5   # PhiR, Nl=1, Nx=16, Nf=64
     python -m tups_amd.profile_workloads calib    --steps 3   # G = F - G streaming kernel
+    python -m tups_amd.profile_workloads batch    --steps 3   # uneven q batch, Nx=8, D 16..576
 
 `calib` runs a streaming kernel whose bytes are known exactly (per element:
 read 2 x 16 B, write 16 B) so hardware counters (FETCH_SIZE, WRITE_SIZE) can be
@@ -18,7 +19,7 @@ import sys
 
 import numpy as np
 
-from .runtime import BSE_LADDER_SPLIT, GAMMA, K_PAE, LAMBDA, TupsContext
+from .runtime import BSE_LADDER_SPLIT, CHI, F, GAMMA, K_PAE, LAMBDA, TupsBatch, TupsContext
 from .synth import batched_bse_inputs
 
 
@@ -67,15 +68,63 @@ def crossing_flops(c: TupsContext) -> float:
     return float(terms) * c.nibz * c.nl ** 2 * per_term
 
 
+def uneven_batch_shape(nx=8, nw=16, nf_max=64, nf_min=16):
+    """Per-q (Nf_q, Nl_q) of the uneven batch: q = m nw + w over nx^2 momenta m and nw
+    bosonic frequencies w; the fermionic box shrinks linearly with w from nf_max to
+    nf_min (kept even) and the form-factor count cycles 1, 5, 9 with m."""
+    nf, nl = [], []
+    for m in range(nx * nx):
+        for w in range(nw):
+            f = nf_max - (nf_max - nf_min) * w / max(1, nw - 1)
+            nf.append(int(2 * round(f / 2)))
+            nl.append((1, 5, 9)[m % 3])
+    return nf, nl
+
+
+def batch_context(nx=8, nw=16, U=2.0, seed=0):
+    """TupsBatch of the uneven shape with Gamma = F drawn per q from N(0, (0.3U)^2/D_q)
+    plus U on the diagonal, and random bubble blocks of unit scale."""
+    nf, nl = uneven_batch_shape(nx, nw)
+    b = TupsBatch(nx, nf, nl, beta=2.0, f_damping=0.5)
+    rng = np.random.default_rng(seed)
+    d = np.array([f * l for f, l in zip(nf, nl)], dtype=np.int64)
+    sizes = d * d
+    off = np.concatenate(([0], np.cumsum(sizes)))
+    scale = np.repeat(0.3 * U / np.sqrt(d), sizes)
+    diag = np.concatenate([off[q] + np.arange(d[q]) * (d[q] + 1) for q in range(len(d))])
+    for r in range(4):
+        v = (rng.standard_normal(b.vertex_elems) + 1j * rng.standard_normal(b.vertex_elems)) * scale
+        v[diag] += U
+        b.upload_packed(GAMMA, r, v)
+        b.upload_packed(F, r, v)
+    for p in range(2):
+        b.upload_packed(CHI, p, rng.standard_normal(b.chi_elems) + 1j * rng.standard_normal(b.chi_elems))
+    return b
+
+
+def batch_flops(b: TupsBatch) -> float:
+    """Algorithmic flops of one batch launch: 16 D_q^3 per (q, channel)."""
+    return float(sum(16.0 * float(d) ** 3 * 4 for d in b.dims()))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["bse", "split", "crossing", "phir", "calib"])
+    ap.add_argument("workload", choices=["bse", "split", "crossing", "phir", "calib", "batch"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--nx", type=int, default=16)
     ap.add_argument("--nf", type=int, default=64)
     ap.add_argument("--nl", type=int, default=1)
     ap.add_argument("--moduli", type=int, default=12)
     args = ap.parse_args(argv)
+    if args.workload == "batch":
+        b = batch_context()
+        for _ in range(args.steps):
+            b.BSE()
+        b.sync()
+        print(f"batch: {args.steps} launches, Nq={b.nq}, items={b.items}, flops/launch {batch_flops(b):.4e}")
+        b.close()
+        sys.stdout.flush()
+        return
     if args.workload in ("bse", "split", "calib"):
         c = TupsContext(nx=8, nf=64, nl=5, nq=1024, beta=2.0, f_damping=0.5)
         batched_bse_inputs(c)

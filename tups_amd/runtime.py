@@ -70,6 +70,13 @@ SIGNATURES = {
     "tups_set_split": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_calc_chi": ([_V, _D, _D, C.c_double, C.c_int32], C.c_int),
     "tups_iteration": ([_V, C.c_int32, C.c_int32], C.c_int),
+    "tups_batch_create": ([C.c_int32, C.c_int32, _I, _I, C.c_double, C.c_double, _P(_V)], C.c_int),
+    "tups_batch_destroy": ([_V], C.c_int),
+    "tups_batch_query": ([_V, _P(C.c_int64), _P(C.c_int64), _I], C.c_int),
+    "tups_batch_upload": ([_V, C.c_int32, C.c_int32, _D], C.c_int),
+    "tups_batch_download": ([_V, C.c_int32, C.c_int32, _D], C.c_int),
+    "tups_batch_bse": ([_V, _P(C.c_float)], C.c_int),
+    "tups_batch_sync": ([_V], C.c_int),
     "tups_phir": ([_V], C.c_int),
     "tups_build_m": ([_V], C.c_int),
     "tups_pae": ([_V], C.c_int),
@@ -317,3 +324,91 @@ class TupsContext:
 
     def copy(self, which_dst: int, which_src: int, channel: int):
         _check(self.lib.tups_copy(self._h, which_dst, which_src, channel), "tups_copy")
+
+
+class TupsBatch:
+    """Batching form: q points with their own (Nf_q, Nl_q); no reference counterpart.
+
+    Vertex arrays are lists of per-q (D_q, D_q) complex matrices, bubbles lists
+    of per-q (Nl_q, Nl_q, Nf_q) blocks; they are packed q after q for the C ABI.
+    """
+
+    def __init__(self, nx: int, nf, nl, beta: float = 2.0, f_damping: float = 0.5):
+        self.lib = load_library()
+        self.nf = [int(x) for x in nf]
+        self.nl = [int(x) for x in nl]
+        self.nq = len(self.nf)
+        self.nx = nx
+        self.alpha = 1.0 / (beta * nx * nx)
+        h = _V()
+        nfa = np.ascontiguousarray(self.nf, dtype=np.int32)
+        nla = np.ascontiguousarray(self.nl, dtype=np.int32)
+        _check(self.lib.tups_batch_create(nx, self.nq, _iptr(nfa), _iptr(nla), beta, f_damping, C.byref(h)),
+               "tups_batch_create")
+        self._h = h
+        ve, ce, it = C.c_int64(), C.c_int64(), C.c_int32()
+        _check(self.lib.tups_batch_query(self._h, C.byref(ve), C.byref(ce), C.byref(it)), "tups_batch_query")
+        self.vertex_elems, self.chi_elems, self.items = int(ve.value), int(ce.value), int(it.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.tups_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def dims(self):
+        return [l * f for l, f in zip(self.nl, self.nf)]
+
+    def _pack(self, blocks, n):
+        out = np.empty(n, dtype=np.complex128)
+        o = 0
+        for b in blocks:
+            flat = np.asarray(b, dtype=np.complex128).ravel(order="F")
+            out[o:o + flat.size] = flat
+            o += flat.size
+        assert o == n
+        return out
+
+    def upload(self, which: int, channel: int, blocks):
+        n = self.chi_elems if which == CHI else self.vertex_elems
+        self.upload_packed(which, channel, self._pack(blocks, n))
+
+    def upload_packed(self, which: int, channel: int, packed: np.ndarray):
+        """Upload an already packed array (q after q, each block column-major)."""
+        n = self.chi_elems if which == CHI else self.vertex_elems
+        packed = np.ascontiguousarray(packed, dtype=np.complex128)
+        if packed.size != n:
+            raise ValueError(f"packed array holds {packed.size} elements, the batch needs {n}")
+        _check(self.lib.tups_batch_upload(self._h, which, channel, _dptr(packed)), "tups_batch_upload")
+
+    def download(self, which: int, channel: int):
+        n = self.chi_elems if which == CHI else self.vertex_elems
+        packed = np.empty(n, dtype=np.complex128)
+        _check(self.lib.tups_batch_download(self._h, which, channel, _dptr(packed)), "tups_batch_download")
+        out, o = [], 0
+        for f, l in zip(self.nf, self.nl):
+            shape = (l, l, f) if which == CHI else (l * f, l * f)
+            size = int(np.prod(shape))
+            out.append(packed[o:o + size].reshape(shape, order="F"))
+            o += size
+        return out
+
+    def BSE(self, timed: bool = False):
+        """The damped ladder of every (q, channel) in one persistent launch."""
+        ms = C.c_float()
+        _check(self.lib.tups_batch_bse(self._h, C.byref(ms) if timed else None), "tups_batch_bse")
+        return float(ms.value) if timed else None
+
+    def sync(self):
+        _check(self.lib.tups_batch_sync(self._h), "tups_batch_sync")
