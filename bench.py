@@ -250,6 +250,7 @@ def main():
     # batching form: 1024 q with Nf_q 64 -> 16 by bosonic frequency and Nl_q in {1, 5, 9}
     from tups_amd.profile_workloads import batch_context, batch_flops
     bt = batch_context()
+    bvar, _ = bt.autotune()
     for _ in range(args.warmup):
         bt.BSE()
     bt.sync()
@@ -261,7 +262,7 @@ def main():
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": round(bfl / (bt_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
                              "flops_per_launch": bfl},
-                "kernel": "bse_batch_kernel (one persistent launch, all q and channels)",
+                "kernel": f"bse_batch_kernel variant {bvar} (one persistent launch, all q and channels)",
                 "items": bt.items,
                 "workload": f"Nx=8, Nq={bt.nq} (64 momenta x 16 bosonic freq), Nf_q 64->16 by frequency, "
                             f"Nl_q 1/5/9 by momentum, D {min(dims)}..{max(dims)}, FP64"}

@@ -171,6 +171,11 @@ int tups_batch_upload(tups_batch* batch, int32_t which, int32_t channel, const d
 int tups_batch_download(tups_batch* batch, int32_t which, int32_t channel, double* host);
 int tups_batch_bse(tups_batch* batch, float* ms);
 int tups_batch_sync(tups_batch* batch);
+/* Project autotuner for the batch: times every tile variant (tile choice per
+ * Nl, XCD-striped or plain item order) on the resident batch without changing
+ * it and keeps the fastest; tups_variant_name(TUPS_K_BSE_BATCH, v) names v. */
+int tups_batch_autotune(tups_batch* batch, int32_t* variant, float* ms);
+int tups_batch_set_variant(tups_batch* batch, int32_t variant);
 
 /* calc_PhiR(ite): PhiR_r from Gamma_r (holding Phi_r); SRC/parquet_PhiR.f90:35-109 */
 int tups_phir(tups_ctx* ctx);
@@ -212,6 +217,7 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 #define TUPS_K_SPLIT_GEMM 7   /* split ladder: INT8 MFMA residue products, CRT, damping */
 #define TUPS_K_BSE_CLOSED 8   /* closed form: build of [I - kappa Gamma chi | Gamma] and its solve */
 #define TUPS_K_CALC_CHI 9     /* bubbles chi_ph, chi_pp for every q slot */
+#define TUPS_K_BSE_BATCH 10   /* batching form: variant names only (tups_variant_name/count) */
 #define TUPS_K_COUNT 12
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);

@@ -50,15 +50,21 @@ def _oracle(nx, nf, nl, gam, fv, chi):
     return out
 
 
-def _run(nx, nf, nl, gam, fv, chi, calls=1):
+def _run(nx, nf, nl, gam, fv, chi, calls=1, variant=None, autotune=False):
     from tups_amd import CHI, F, GAMMA, TupsBatch
 
     with TupsBatch(nx, nf, nl, beta=BETA, f_damping=FD) as b:
+        if variant is not None:
+            b.set_variant(variant)
         for r in range(4):
             b.upload(GAMMA, r, gam[r])
             b.upload(F, r, fv[r])
         for p in range(2):
             b.upload(CHI, p, chi[p])
+        if autotune:
+            name, ms = b.autotune()
+            _say(f"  autotune picked {name} ({ms:.3f} ms)")
+            assert name and ms > 0
         for _ in range(calls):
             b.BSE()
         b.sync()
@@ -82,6 +88,44 @@ def test_batch_parity(nx, nf, nl):
             err = rel_fro(got[r][q], exp[r][q])
             assert err < TOL, (r, q, nf[q], nl[q], err)
     _say("  max rel err", max(rel_fro(got[r][q], exp[r][q]) for r in range(4) for q in range(len(nf))))
+
+
+def test_every_batch_variant_matches_oracle():
+    """Each tile choice and item order of the batch kernel against the oracle; the
+    two item orders of one tile choice agree bit for bit (same per-item code)."""
+    from tups_amd import TupsBatch
+
+    nx, nf, nl = 4, [8, 12, 6, 16, 4], [9, 5, 9, 1, 5]
+    rng = np.random.default_rng(23)
+    gam, fv, chi = _inputs(rng, nf, nl)
+    exp = _oracle(nx, nf, nl, gam, fv, chi)
+    with TupsBatch(nx, [2], [1]) as probe:
+        variants = probe.variants()
+    assert len(variants) >= 2
+    first = None
+    for v, name in variants:
+        _say(f"  variant {v} {name}")
+        got = _run(nx, nf, nl, gam, fv, chi, variant=v)
+        for r in range(4):
+            for q in range(len(nf)):
+                assert rel_fro(got[r][q], exp[r][q]) < TOL, (name, r, q)
+        if v % 5 == 0:
+            first = got
+        else:  # another item order of the same tile code
+            for r in range(4):
+                for q in range(len(nf)):
+                    assert np.array_equal(got[r][q], first[r][q]), (name, r, q)
+
+
+def test_autotune_leaves_the_batch_unchanged():
+    nx, nf, nl = 4, [8, 10, 4], [9, 5, 1]
+    rng = np.random.default_rng(5)
+    gam, fv, chi = _inputs(rng, nf, nl)
+    got = _run(nx, nf, nl, gam, fv, chi, autotune=True)
+    exp = _oracle(nx, nf, nl, gam, fv, chi)
+    for r in range(4):
+        for q in range(len(nf)):
+            assert rel_fro(got[r][q], exp[r][q]) < TOL, (r, q)
 
 
 def test_batch_two_calls_chain():

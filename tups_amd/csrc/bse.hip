@@ -146,16 +146,21 @@ __device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, cons
       // g = u >> 6 is the wave index (+ 4p): uniform, so chi_nu is a scalar load
       const int g = __builtin_amdgcn_readfirstlane(u >> 6);
       const int nu = nu0 + g < nf ? nu0 + g : nf - 1;
-      const double2* cb = chiq + (size_t)nu * NL * NL;
+      // chi is read-only for the whole launch: reading it through the constant
+      // address space keeps these uniform loads on the scalar path even where
+      // the compiler cannot rule out aliasing with earlier stores (a kernel
+      // that loops over several tiles)
+      const __attribute__((address_space(4))) double* cb =
+          (const __attribute__((address_space(4))) double*)(chiq + (size_t)nu * NL * NL);
 #pragma unroll
       for (int l2 = 0; l2 < NL; ++l2) {
         // (X alpha chi)(i, (l2, nu)) = alpha sum_l3 X(i, (l3, nu)) chi_nu(l3, l2)
         double sr = 0.0, si = 0.0;
 #pragma unroll
         for (int l3 = 0; l3 < NL; ++l3) {
-          const double2 x = cb[l3 + NL * l2];
-          sr += ra[p][l3].x * x.x - ra[p][l3].y * x.y;
-          si += ra[p][l3].x * x.y + ra[p][l3].y * x.x;
+          const double xr = cb[2 * (l3 + NL * l2)], xi = cb[2 * (l3 + NL * l2) + 1];
+          sr += ra[p][l3].x * xr - ra[p][l3].y * xi;
+          si += ra[p][l3].x * xi + ra[p][l3].y * xr;
         }
         As[sb][0][l2 * G + g][i] = alpha * sr;
         As[sb][1][l2 * G + g][i] = alpha * si;
@@ -400,28 +405,79 @@ struct BatchArgs {
   const double2* fv[4];
   double2* out[4];
   const double2* chi[2];
-  const int4* items;        // (q, r, row0, col0)
+  const int4* items;        // 8 lists of len entries (q, r, row0, col0); q < 0 pads
   const long long* voff;    // element offset of q's D_q x D_q block
   const long long* coff;    // element offset of q's bubble blocks
   const int* nf;
   const int* nl;
-  int nitems;
+  int* tickets;             // per-list counters of the dynamic order (zero at launch)
+  int len;                  // entries per list
+  int xcd;                  // 1: workgroups on XCD x take list x (grid a multiple of 8)
+  int dyn;                  // 1: entries handed out by the per-list counters
   double alpha;
   double cf[4];
   double omf;
 };
 }  // namespace
 
-constexpr int kBatchLds = ladder_lds_doubles<9, 4, false, false>();
-static_assert(ladder_lds_doubles<5, 4, false, true>() <= kBatchLds, "batch LDS too small");
-static_assert(ladder_lds_doubles<1, 16, false, false>() <= kBatchLds, "batch LDS too small");
+template <int G5, int G9, bool K9>
+constexpr int batch_lds_doubles() {
+  constexpr int a = ladder_lds_doubles<1, 16, false, false>();
+  constexpr int b = ladder_lds_doubles<5, G5, false, true>();
+  constexpr int c = ladder_lds_doubles<9, G9, false, K9>();
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
 
-__global__ __launch_bounds__(256, 2) void bse_batch_kernel(BatchArgs a) {
-  __shared__ double lds[kBatchLds];
-  for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
-    const int4 w = a.items[it];
-    const int q = w.x, r = w.y, nf = a.nf[q], nl = a.nl[q];
-    const long long vo = a.voff[q], co = a.coff[q];
+// workgroups per CU that the LDS footprint allows (160 KB per CU); the register
+// budget follows: 256 VGPRs per lane at two, the full 512 (AGPRs included) at one
+template <int G5, int G9, bool K9>
+constexpr int batch_occupancy() {
+  return 8 * batch_lds_doubles<G5, G9, K9>() <= 80 * 1024 ? 2 : 1;
+}
+
+__device__ __forceinline__ long long uniform64(long long v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(v & 0xffffffffLL));
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// The work items come as 8 lists, one per XCD, each holding whole (q, r)
+// matrices largest first, balanced by estimated work.  With xcd set, workgroup
+// b -- dispatched to XCD x = b % 8 as that XCD's j-th workgroup, j = b / 8 --
+// works on list x, so the tiles an XCD runs at one time belong to one or two
+// matrices and share that XCD's L2: statically entries j, j + grid/8, ..., or,
+// with dyn set, whichever entry the list's counter hands out next (a
+// workgroup that finishes early takes more).  Otherwise the lists are walked
+// as one array, entries b, b + grid, ...
+template <int G5, int G9, bool K9>
+__global__ __launch_bounds__(256, (batch_occupancy<G5, G9, K9>())) void bse_batch_kernel(BatchArgs a) {
+  extern __shared__ double lds[];
+  __shared__ int ticket;
+  const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
+  const int first = a.xcd ? x * a.len + j : blockIdx.x;
+  const int end = a.xcd ? (x + 1) * a.len : 8 * a.len;
+  const int step = a.xcd ? (int)(gridDim.x >> 3) : (int)gridDim.x;
+  for (int it = first;; it += step) {
+    if (a.dyn) {
+      if (threadIdx.x == 0) ticket = atomicAdd(a.tickets + x, 1);
+      __syncthreads();
+      it = x * a.len + ticket;
+      __syncthreads();  // every thread has read the ticket before the next one is drawn
+      if (it >= end) break;
+    } else if (it >= end) {
+      break;
+    }
+    // the item's values are workgroup-uniform: keep them (and the addresses
+    // built from them) in scalar registers, off the VALU of the MFMA loop
+    int4 w = a.items[it];
+    w.x = __builtin_amdgcn_readfirstlane(w.x);
+    if (w.x < 0) continue;  // list padding
+    w.y = __builtin_amdgcn_readfirstlane(w.y);
+    w.z = __builtin_amdgcn_readfirstlane(w.z);
+    w.w = __builtin_amdgcn_readfirstlane(w.w);
+    const int q = w.x, r = w.y;
+    const int nf = __builtin_amdgcn_readfirstlane(a.nf[q]), nl = __builtin_amdgcn_readfirstlane(a.nl[q]);
+    const long long vo = uniform64(a.voff[q]), co = uniform64(a.coff[q]);
     const double2* g = a.gam[r] + vo;
     const double2* f = a.fv[r] + vo;
     double2* o = a.out[r] + vo;
@@ -429,11 +485,47 @@ __global__ __launch_bounds__(256, 2) void bse_batch_kernel(BatchArgs a) {
     if (nl == 1)
       ladder_tile<1, 16, false, false>(g, f, o, ch, nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
     else if (nl == 5)
-      ladder_tile<5, 4, false, true>(g, f, o, ch, 5 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
+      ladder_tile<5, G5, false, true>(g, f, o, ch, 5 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
     else
-      ladder_tile<9, 4, false, false>(g, f, o, ch, 9 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
+      ladder_tile<9, G9, false, K9>(g, f, o, ch, 9 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
     __syncthreads();  // the next item reuses the LDS buffer
   }
+}
+
+__global__ void batch_tickets_clear(int* t) {
+  if (threadIdx.x < 8) t[threadIdx.x] = 0;
+}
+
+struct BatchVariant {
+  void (*fn)(BatchArgs);
+  int lds_bytes;
+  int per_cu;  // resident workgroups per CU
+  const char* name;
+};
+
+// tile choice per form-factor count (Nl = 1 always runs G = 16, four products)
+static const BatchVariant kBatchKernels[] = {
+#define TUPS_BATCH_VARIANT(G5, G9, K9, name) \
+  {bse_batch_kernel<G5, G9, K9>, 8 * batch_lds_doubles<G5, G9, K9>(), batch_occupancy<G5, G9, K9>(), name}
+    TUPS_BATCH_VARIANT(4, 4, false, "nl5_g4_3m+nl9_g4"),
+    TUPS_BATCH_VARIANT(4, 4, true, "nl5_g4_3m+nl9_g4_3m"),
+#undef TUPS_BATCH_VARIANT
+};
+constexpr int kNumBatchKernels = sizeof(kBatchKernels) / sizeof(kBatchKernels[0]);
+
+// variant id v = 5 * kernel + order: 0 resident grid, per-XCD lists, static
+// turns; 1 resident grid, lists walked as one array; 2, 3 the same with one
+// workgroup per entry (the hardware dispatcher balances); 4 resident grid,
+// per-XCD lists handed out by counters
+constexpr int kBatchOrders = 5;
+int batch_variant_count() { return kBatchOrders * kNumBatchKernels; }
+
+const char* batch_variant_name(int v) {
+  static const char* order[kBatchOrders] = {"_xcd", "_rr", "_xcd_flat", "_rr_flat", "_xcd_dyn"};
+  static std::string names[kBatchOrders * kNumBatchKernels];
+  if (v < 0 || v >= batch_variant_count()) return "";
+  if (names[v].empty()) names[v] = std::string(kBatchKernels[v / kBatchOrders].name) + order[v % kBatchOrders];
+  return names[v].c_str();
 }
 
 }  // namespace tups
@@ -455,7 +547,10 @@ struct tups_batch {
   long long* d_coff = nullptr;
   int* d_nf = nullptr;
   int* d_nl = nullptr;
-  int nitems = 0;
+  int* d_tickets = nullptr;
+  int nitems = 0;  // work items (tiles)
+  int len = 0;     // entries per XCD list, padding included
+  int variant = 0;
 };
 
 extern "C" int tups_batch_destroy(tups_batch* b) {
@@ -476,6 +571,7 @@ extern "C" int tups_batch_destroy(tups_batch* b) {
   f(b->d_coff);
   f(b->d_nf);
   f(b->d_nl);
+  f(b->d_tickets);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return TUPS_OK;
@@ -503,18 +599,39 @@ extern "C" int tups_batch_create(int32_t nx, int32_t nq, const int32_t* nf, cons
     b->voff[q + 1] = b->voff[q] + D * D;
     b->coff[q + 1] = b->coff[q] + (long long)nl[q] * nl[q] * nf[q];
   }
-  // work items, largest D first
+  // work lists: (q, r) matrices largest D first, each to the list with the
+  // least work so far (work of a matrix ~ tiles x inner length = tn^2 D)
   std::vector<int> order(nq);
   for (int q = 0; q < nq; ++q) order[q] = q;
   std::stable_sort(order.begin(), order.end(),
                    [&](int x, int y) { return nl[x] * nf[x] > nl[y] * nf[y]; });
-  std::vector<int4> items;
+  std::vector<int4> lists[8];
+  long long load[8] = {};
+  long long total = 0;
   for (int q : order) {
     const int D = nl[q] * nf[q], tn = (D + BM - 1) / BM;
-    for (int r = 0; r < 4; ++r)
-      for (int t = 0; t < tn * tn; ++t) items.push_back(make_int4(q, r, (t / tn) * BM, (t % tn) * BN));
+    for (int r = 0; r < 4; ++r) {
+      const int x = (int)(std::min_element(load, load + 8) - load);
+      load[x] += (long long)tn * tn * D;
+      for (int t = 0; t < tn * tn; ++t) lists[x].push_back(make_int4(q, r, (t / tn) * BM, (t % tn) * BN));
+      total += tn * tn;
+    }
   }
-  b->nitems = (int)items.size();
+  size_t len = 0;
+  for (auto& l : lists) len = std::max(len, l.size());
+  if (len * 8 >= (size_t)1 << 31) {
+    delete b;
+    set_error("batch too large for one launch");
+    return TUPS_EINVAL;
+  }
+  std::vector<int4> items;
+  items.reserve(8 * len);
+  for (auto& l : lists) {
+    items.insert(items.end(), l.begin(), l.end());
+    items.insert(items.end(), len - l.size(), make_int4(-1, 0, 0, 0));
+  }
+  b->nitems = (int)total;
+  b->len = (int)len;
   auto fail = [&](int rc) {
     tups_batch_destroy(b);
     return rc;
@@ -543,6 +660,7 @@ extern "C" int tups_batch_create(int32_t nx, int32_t nq, const int32_t* nf, cons
   if (!rc) rc = alloc((void**)&b->d_coff, (size_t)(nq + 1) * sizeof(long long));
   if (!rc) rc = alloc((void**)&b->d_nf, (size_t)nq * sizeof(int));
   if (!rc) rc = alloc((void**)&b->d_nl, (size_t)nq * sizeof(int));
+  if (!rc) rc = alloc((void**)&b->d_tickets, 8 * sizeof(int));
   if (rc) return fail(rc);
   if (hipMemcpy(b->d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_voff, b->voff.data(), (nq + 1) * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
@@ -600,8 +718,7 @@ extern "C" int tups_batch_download(tups_batch* b, int32_t which, int32_t ch, dou
   return TUPS_OK;
 }
 
-extern "C" int tups_batch_bse(tups_batch* b, float* ms) {
-  TUPS_CHECK(b, TUPS_EINVAL, "null batch");
+static BatchArgs batch_args(const tups_batch* b, int xcd, int dyn) {
   BatchArgs a{};
   for (int r = 0; r < 4; ++r) {
     a.gam[r] = b->gamma[r];
@@ -615,23 +732,50 @@ extern "C" int tups_batch_bse(tups_batch* b, float* ms) {
   a.coff = b->d_coff;
   a.nf = b->d_nf;
   a.nl = b->d_nl;
-  a.nitems = b->nitems;
+  a.tickets = b->d_tickets;
+  a.len = b->len;
+  a.xcd = xcd;
+  a.dyn = dyn;
   a.alpha = b->alpha;
   const double cr[4] = {0.5, 0.5, -0.25, 0.25};
   for (int r = 0; r < 4; ++r) a.cf[r] = cr[r] * b->f_damping;
   a.omf = 1.0 - b->f_damping;
+  return a;
+}
+
+// one launch of variant v into the spare buffers (no swap)
+static int batch_launch(tups_batch* b, int v) {
+  const BatchVariant& k = kBatchKernels[v / kBatchOrders];
+  const int order = v % kBatchOrders;
+  const int xcd = order != 1 && order != 3, flat = order == 2 || order == 3, dyn = order == 4;
   int dev = 0, ncu = 0;
   TUPS_HIP(hipGetDevice(&dev));
   TUPS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int grid = std::max(1, std::min(b->nitems, 2 * ncu));
+  // resident: as many workgroups as stay on the CUs at once (a multiple of 8);
+  // flat: one workgroup per list entry
+  const long long all = 8LL * b->len;
+  const long long grid = flat ? all : std::min<long long>(all, std::max(8, k.per_cu * ncu / 8 * 8));
+  TUPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               k.lds_bytes));
+  if (dyn) {  // a fill kernel, not a memset node, so the launch may be captured
+    hipLaunchKernelGGL(batch_tickets_clear, dim3(1), dim3(64), 0, b->stream, b->d_tickets);
+    TUPS_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k.fn, dim3((unsigned)grid), dim3(256), (size_t)k.lds_bytes, b->stream,
+                     batch_args(b, xcd, dyn));
+  TUPS_HIP(hipGetLastError());
+  return TUPS_OK;
+}
+
+extern "C" int tups_batch_bse(tups_batch* b, float* ms) {
+  TUPS_CHECK(b, TUPS_EINVAL, "null batch");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ms) {
     TUPS_HIP(hipEventCreate(&e0));
     TUPS_HIP(hipEventCreate(&e1));
     TUPS_HIP(hipEventRecord(e0, b->stream));
   }
-  hipLaunchKernelGGL(bse_batch_kernel, dim3((unsigned)grid), dim3(256), 0, b->stream, a);
-  TUPS_HIP(hipGetLastError());
+  if (int rc = batch_launch(b, b->variant)) return rc;
   if (ms) {
     TUPS_HIP(hipEventRecord(e1, b->stream));
     TUPS_HIP(hipEventSynchronize(e1));
@@ -640,6 +784,49 @@ extern "C" int tups_batch_bse(tups_batch* b, float* ms) {
     (void)hipEventDestroy(e1);
   }
   for (int r = 0; r < 4; ++r) std::swap(b->gamma[r], b->spare[r]);
+  return TUPS_OK;
+}
+
+extern "C" int tups_batch_set_variant(tups_batch* b, int32_t variant) {
+  TUPS_CHECK(b, TUPS_EINVAL, "null batch");
+  TUPS_CHECK(variant >= 0 && variant < batch_variant_count(), TUPS_EINVAL, "no such batch variant");
+  b->variant = variant;
+  return TUPS_OK;
+}
+
+// Times every variant on the resident batch (outputs go to the spare buffers
+// and are discarded, the vertex state is unchanged) and keeps the fastest.
+extern "C" int tups_batch_autotune(tups_batch* b, int32_t* variant, float* ms) {
+  TUPS_CHECK(b, TUPS_EINVAL, "null batch");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  TUPS_HIP(hipEventCreate(&e0));
+  TUPS_HIP(hipEventCreate(&e1));
+  float best = 0.f;
+  int arg = -1, rc = TUPS_OK;
+  for (int v = 0; v < batch_variant_count() && !rc; ++v) {
+    if ((rc = batch_launch(b, v))) break;  // warm-up
+    if (hipEventRecord(e0, b->stream) != hipSuccess) rc = TUPS_EHIP;
+    for (int i = 0; i < 2 && !rc; ++i) rc = batch_launch(b, v);
+    if (rc) break;
+    float t = 0.f;
+    if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+      set_error("batch autotune timing failed");
+      rc = TUPS_EHIP;
+      break;
+    }
+    t *= 0.5f;
+    if (arg < 0 || t < best) {
+      best = t;
+      arg = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  b->variant = arg;
+  if (variant) *variant = arg;
+  if (ms) *ms = best;
   return TUPS_OK;
 }
 

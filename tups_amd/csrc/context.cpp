@@ -513,6 +513,7 @@ int tups_autotune(tups_ctx* c, int32_t kernel, int32_t* variant, float* ms) {
 }
 
 const char* tups_variant_name(int32_t kernel, int32_t variant) {
+  if (kernel == TUPS_K_BSE_BATCH) return batch_variant_name(variant);
   return kernel == TUPS_K_BSE_LADDER ? bse_variant_name(variant)
                                      : (kernel == TUPS_K_PAE ? pae_variant_name(variant) : "");
 }
@@ -533,6 +534,7 @@ int tups_set_variant(tups_ctx* c, int32_t kernel, int32_t variant) {
 }
 
 int tups_variant_count(int32_t kernel) {
+  if (kernel == TUPS_K_BSE_BATCH) return batch_variant_count();
   return kernel == TUPS_K_BSE_LADDER ? bse_variant_count() : (kernel == TUPS_K_PAE ? pae_variant_count() : 0);
 }
 

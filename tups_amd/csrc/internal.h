@@ -134,6 +134,8 @@ int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms);
 const char* bse_variant_name(int v);
 int bse_variant_count();
 int bse_variant_nl(int v);
+const char* batch_variant_name(int v);
+int batch_variant_count();
 int autotune_pae(tups_ctx* c, int reps, int* chosen, float* best_ms);
 int pae_variant_count();
 const char* pae_variant_name(int v);

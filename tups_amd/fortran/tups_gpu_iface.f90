@@ -20,7 +20,7 @@ module tups_gpu_iface
   integer(c_int32_t), parameter :: TUPS_K_BSE_LADDER = 0, TUPS_K_PHIR = 1, TUPS_K_BUILD_M = 2, &
                                    TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4, TUPS_K_PAE_GATHER = 5, &
                                    TUPS_K_SPLIT_PREP = 6, TUPS_K_SPLIT_GEMM = 7, TUPS_K_BSE_CLOSED = 8, &
-                                   TUPS_K_CALC_CHI = 9
+                                   TUPS_K_CALC_CHI = 9, TUPS_K_BSE_BATCH = 10
 
   type, bind(C) :: tups_dims
     integer(c_int32_t) :: nx, nf, nl, nr, nq, reserved
@@ -185,6 +185,21 @@ module tups_gpu_iface
       type(c_ptr), value :: batch
       integer(c_int) :: rc
     end function tups_batch_sync
+
+    function tups_batch_autotune(batch, variant, ms) bind(C, name="tups_batch_autotune") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_float
+      type(c_ptr), value :: batch
+      integer(c_int32_t), intent(out) :: variant
+      real(c_float), intent(out) :: ms
+      integer(c_int) :: rc
+    end function tups_batch_autotune
+
+    function tups_batch_set_variant(batch, variant) bind(C, name="tups_batch_set_variant") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: variant
+      integer(c_int) :: rc
+    end function tups_batch_set_variant
 
     function tups_iteration(ctx, mode, use_graph) bind(C, name="tups_iteration") result(rc)
       import :: c_ptr, c_int32_t, c_int

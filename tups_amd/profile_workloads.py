@@ -115,9 +115,17 @@ def main(argv=None):
     ap.add_argument("--nf", type=int, default=64)
     ap.add_argument("--nl", type=int, default=1)
     ap.add_argument("--moduli", type=int, default=12)
+    ap.add_argument("--variant", default="", help="batch: tile variant by name (default: autotune)")
     args = ap.parse_args(argv)
     if args.workload == "batch":
         b = batch_context()
+        if args.variant:
+            ids = {n: v for v, n in b.variants()}
+            b.set_variant(ids[args.variant])
+            name = args.variant
+        else:
+            name, _ = b.autotune()
+        print(f"batch: variant {name}")
         for _ in range(args.steps):
             b.BSE()
         b.sync()

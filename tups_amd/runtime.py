@@ -21,7 +21,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
 
 GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
 (K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER, K_SPLIT_PREP, K_SPLIT_GEMM,
- K_BSE_CLOSED, K_CALC_CHI) = range(10)
+ K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH) = range(11)
 CH_D, CH_M, CH_S, CH_T = range(4)
 BSE_LADDER_FP64 = 0
 BSE_LADDER_SPLIT = 1
@@ -77,6 +77,8 @@ SIGNATURES = {
     "tups_batch_download": ([_V, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_batch_bse": ([_V, _P(C.c_float)], C.c_int),
     "tups_batch_sync": ([_V], C.c_int),
+    "tups_batch_autotune": ([_V, _I, _P(C.c_float)], C.c_int),
+    "tups_batch_set_variant": ([_V, C.c_int32], C.c_int),
     "tups_phir": ([_V], C.c_int),
     "tups_build_m": ([_V], C.c_int),
     "tups_pae": ([_V], C.c_int),
@@ -412,3 +414,17 @@ class TupsBatch:
 
     def sync(self):
         _check(self.lib.tups_batch_sync(self._h), "tups_batch_sync")
+
+    def autotune(self):
+        """Pick the fastest batch variant on the resident inputs; returns (name, ms per launch)."""
+        v = C.c_int32()
+        ms = C.c_float()
+        _check(self.lib.tups_batch_autotune(self._h, C.byref(v), C.byref(ms)), "tups_batch_autotune")
+        return self.lib.tups_variant_name(K_BSE_BATCH, v.value).decode(), float(ms.value)
+
+    def variants(self):
+        return [(v, self.lib.tups_variant_name(K_BSE_BATCH, v).decode())
+                for v in range(self.lib.tups_variant_count(K_BSE_BATCH))]
+
+    def set_variant(self, variant: int):
+        _check(self.lib.tups_batch_set_variant(self._h, variant), "tups_batch_set_variant")
