@@ -109,7 +109,7 @@ def test_every_batch_variant_matches_oracle():
         for r in range(4):
             for q in range(len(nf)):
                 assert rel_fro(got[r][q], exp[r][q]) < TOL, (name, r, q)
-        if v % 5 == 0:
+        if v % 3 == 0:
             first = got
         else:  # another item order of the same tile code
             for r in range(4):

@@ -58,19 +58,21 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-// K3: the three-multiplication complex product (Re A + Im A)(Re B + Im B) - ...:
-// P1 = Re A Re B, P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B),
-// Re = P1 - P2, Im = P3 - P1 - P2 -- three real MFMAs per complex block instead
-// of four, with a third LDS plane holding Re + Im of each operand.
-template <int NL, int G, bool DB, bool K3>
+// K3: the complex product.  0: four real MFMAs per complex block.  1 and 2:
+// the three-multiplication form P1 = Re A Re B, P2 = Im A Im B,
+// P3 = (Re A + Im A)(Re B + Im B), Re = P1 - P2, Im = P3 - P1 - P2, with the
+// sums Re + Im held in a third LDS plane (1) or formed in registers from the
+// Re and Im planes as the operands are read (2: two VALU adds per operand
+// instead of LDS space, which keeps large Nl at two workgroups per CU).
+template <int NL, int G, bool DB, int K3>
 constexpr int ladder_lds_doubles() {
-  return (DB ? 2 : 1) * (K3 ? 3 : 2) * NL * G * (BM + BN);
+  return (DB ? 2 : 1) * (K3 == 1 ? 3 : 2) * NL * G * (BM + BN);
 }
 
 // One 64 x 64 tile of Phi for one (q, r) matrix: Gm, Fm, O are that matrix's
 // Gamma, F and output (D x D, column-major), chiq its bubble blocks, lds a
 // workgroup buffer of ladder_lds_doubles<...>() doubles.
-template <int NL, int G, bool DB, bool K3 = false>
+template <int NL, int G, bool DB, int K3 = 0>
 __device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, const double2* __restrict__ Fm,
                                             double2* __restrict__ O, const double2* __restrict__ chiq, int D,
                                             int nf, double alpha, double cf, double omf, int row0, int col0,
@@ -78,7 +80,7 @@ __device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, cons
   constexpr int KC = NL * G;   // inner-dimension chunk: G frequencies x NL form factors
   constexpr int PP = G / 4;    // (row|col, nu) pairs per thread: 64 * G / 256
   constexpr int NB = DB ? 2 : 1;
-  constexpr int NPL = K3 ? 3 : 2;  // LDS planes per operand: Re, Im (, Re + Im)
+  constexpr int NPL = K3 == 1 ? 3 : 2;  // LDS planes per operand: Re, Im (, Re + Im)
   static_assert(KC % 4 == 0, "chunk must be a multiple of the MFMA k");
   auto As = reinterpret_cast<double (*)[NPL][KC][BM]>(lds);
   auto Bs = reinterpret_cast<double (*)[NPL][KC][BN]>(lds + NB * NPL * KC * BM);
@@ -164,7 +166,7 @@ __device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, cons
         }
         As[sb][0][l2 * G + g][i] = alpha * sr;
         As[sb][1][l2 * G + g][i] = alpha * si;
-        if (K3) As[sb][NPL - 1][l2 * G + g][i] = alpha * sr + alpha * si;
+        if (K3 == 1) As[sb][NPL - 1][l2 * G + g][i] = alpha * sr + alpha * si;
       }
     }
 #pragma unroll
@@ -175,7 +177,7 @@ __device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, cons
       for (int l1 = 0; l1 < NL; ++l1) {
         Bs[sb][0][l1 * G + g][j] = rb[p][l1].x;
         Bs[sb][1][l1 * G + g][j] = rb[p][l1].y;
-        if (K3) Bs[sb][NPL - 1][l1 * G + g][j] = rb[p][l1].x + rb[p][l1].y;
+        if (K3 == 1) Bs[sb][NPL - 1][l1 * G + g][j] = rb[p][l1].x + rb[p][l1].y;
       }
     }
   };
@@ -191,9 +193,12 @@ __device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, cons
         ai[i] = As[buf][1][kr][wr + i * 16 + li];
         br[i] = Bs[buf][0][kr][wc + i * 16 + li];
         bi[i] = Bs[buf][1][kr][wc + i * 16 + li];
-        if (K3) {
+        if (K3 == 1) {
           as[i] = As[buf][NPL - 1][kr][wr + i * 16 + li];
           bs[i] = Bs[buf][NPL - 1][kr][wc + i * 16 + li];
+        } else if (K3 == 2) {
+          as[i] = ar[i] + ai[i];
+          bs[i] = br[i] + bi[i];
         }
       }
 #pragma unroll
@@ -262,7 +267,7 @@ __device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, cons
     }
 }
 
-template <int NL, int G, bool DB, bool K3 = false>
+template <int NL, int G, bool DB, int K3 = 0>
 __global__ __launch_bounds__(256, 2) void bse_ladder_kernel(BseArgs a) {
   __shared__ double lds[ladder_lds_doubles<NL, G, DB, K3>()];
   const int tiles = a.tiles_n * a.tiles_n;
@@ -299,6 +304,9 @@ static const BseVariant kBseVariants[] = {
     {5, bse_ladder_kernel<5, 4, false, true>, "nl5_g4_3m"},
     {5, bse_ladder_kernel<5, 8, false, true>, "nl5_g8_3m"},
     {9, bse_ladder_kernel<9, 4, false, true>, "nl9_g4_3m"},
+    {1, bse_ladder_kernel<1, 16, false, 2>, "nl1_g16_3r"},
+    {5, bse_ladder_kernel<5, 4, false, 2>, "nl5_g4_3r"},
+    {9, bse_ladder_kernel<9, 4, false, 2>, "nl9_g4_3r"},
 };
 constexpr int kNumBseVariants = sizeof(kBseVariants) / sizeof(kBseVariants[0]);
 
@@ -412,7 +420,6 @@ struct BatchArgs {
   const int* nl;
   int* tickets;             // per-list counters of the dynamic order (zero at launch)
   int len;                  // entries per list
-  int xcd;                  // 1: workgroups on XCD x take list x (grid a multiple of 8)
   int dyn;                  // 1: entries handed out by the per-list counters
   double alpha;
   double cf[4];
@@ -420,19 +427,19 @@ struct BatchArgs {
 };
 }  // namespace
 
-template <int G5, int G9, bool K9>
+template <int K1, int K5, int K9>
 constexpr int batch_lds_doubles() {
-  constexpr int a = ladder_lds_doubles<1, 16, false, false>();
-  constexpr int b = ladder_lds_doubles<5, G5, false, true>();
-  constexpr int c = ladder_lds_doubles<9, G9, false, K9>();
+  constexpr int a = ladder_lds_doubles<1, 16, false, K1>();
+  constexpr int b = ladder_lds_doubles<5, 4, false, K5>();
+  constexpr int c = ladder_lds_doubles<9, 4, false, K9>();
   return a > b ? (a > c ? a : c) : (b > c ? b : c);
 }
 
 // workgroups per CU that the LDS footprint allows (160 KB per CU); the register
 // budget follows: 256 VGPRs per lane at two, the full 512 (AGPRs included) at one
-template <int G5, int G9, bool K9>
+template <int K1, int K5, int K9>
 constexpr int batch_occupancy() {
-  return 8 * batch_lds_doubles<G5, G9, K9>() <= 80 * 1024 ? 2 : 1;
+  return 8 * batch_lds_doubles<K1, K5, K9>() <= 80 * 1024 ? 2 : 1;
 }
 
 __device__ __forceinline__ long long uniform64(long long v) {
@@ -442,21 +449,22 @@ __device__ __forceinline__ long long uniform64(long long v) {
 }
 
 // The work items come as 8 lists, one per XCD, each holding whole (q, r)
-// matrices largest first, balanced by estimated work.  With xcd set, workgroup
-// b -- dispatched to XCD x = b % 8 as that XCD's j-th workgroup, j = b / 8 --
-// works on list x, so the tiles an XCD runs at one time belong to one or two
-// matrices and share that XCD's L2: statically entries j, j + grid/8, ..., or,
-// with dyn set, whichever entry the list's counter hands out next (a
-// workgroup that finishes early takes more).  Otherwise the lists are walked
-// as one array, entries b, b + grid, ...
-template <int G5, int G9, bool K9>
-__global__ __launch_bounds__(256, (batch_occupancy<G5, G9, K9>())) void bse_batch_kernel(BatchArgs a) {
+// matrices largest first, balanced by estimated work.  Workgroup b --
+// dispatched to XCD x = b % 8 as that XCD's j-th workgroup, j = b / 8 -- works
+// on list x, so the tiles an XCD runs at one time belong to one or two
+// matrices and share that XCD's L2: entries j, j + grid/8, ... (grid/8 = len
+// gives one entry per workgroup and leaves the balancing to the hardware
+// dispatcher), or, with dyn set, whichever entry the list's counter hands out
+// next (a resident workgroup that finishes early takes more).  The grid is a
+// multiple of 8.  K1, K5, K9: the complex-product form per form-factor count.
+template <int K1, int K5, int K9>
+__global__ __launch_bounds__(256, (batch_occupancy<K1, K5, K9>())) void bse_batch_kernel(BatchArgs a) {
   extern __shared__ double lds[];
   __shared__ int ticket;
   const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
-  const int first = a.xcd ? x * a.len + j : blockIdx.x;
-  const int end = a.xcd ? (x + 1) * a.len : 8 * a.len;
-  const int step = a.xcd ? (int)(gridDim.x >> 3) : (int)gridDim.x;
+  const int first = x * a.len + j;
+  const int end = (x + 1) * a.len;
+  const int step = (int)(gridDim.x >> 3);
   for (int it = first;; it += step) {
     if (a.dyn) {
       if (threadIdx.x == 0) ticket = atomicAdd(a.tickets + x, 1);
@@ -483,11 +491,11 @@ __global__ __launch_bounds__(256, (batch_occupancy<G5, G9, K9>())) void bse_batc
     double2* o = a.out[r] + vo;
     const double2* ch = a.chi[r >> 1] + co;
     if (nl == 1)
-      ladder_tile<1, 16, false, false>(g, f, o, ch, nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
+      ladder_tile<1, 16, false, K1>(g, f, o, ch, nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
     else if (nl == 5)
-      ladder_tile<5, G5, false, true>(g, f, o, ch, 5 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
+      ladder_tile<5, 4, false, K5>(g, f, o, ch, 5 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
     else
-      ladder_tile<9, G9, false, K9>(g, f, o, ch, 9 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
+      ladder_tile<9, 4, false, K9>(g, f, o, ch, 9 * nf, nf, a.alpha, a.cf[r], a.omf, w.z, w.w, lds);
     __syncthreads();  // the next item reuses the LDS buffer
   }
 }
@@ -505,23 +513,24 @@ struct BatchVariant {
 
 // tile choice per form-factor count (Nl = 1 always runs G = 16, four products)
 static const BatchVariant kBatchKernels[] = {
-#define TUPS_BATCH_VARIANT(G5, G9, K9, name) \
-  {bse_batch_kernel<G5, G9, K9>, 8 * batch_lds_doubles<G5, G9, K9>(), batch_occupancy<G5, G9, K9>(), name}
-    TUPS_BATCH_VARIANT(4, 4, false, "nl5_g4_3m+nl9_g4"),
-    TUPS_BATCH_VARIANT(4, 4, true, "nl5_g4_3m+nl9_g4_3m"),
+#define TUPS_BATCH_VARIANT(K1, K5, K9, name) \
+  {bse_batch_kernel<K1, K5, K9>, 8 * batch_lds_doubles<K1, K5, K9>(), batch_occupancy<K1, K5, K9>(), name}
+    TUPS_BATCH_VARIANT(0, 1, 0, "nl1_4m+nl5_3m+nl9_4m"),
+    TUPS_BATCH_VARIANT(0, 1, 2, "nl1_4m+nl5_3m+nl9_3r"),
+    TUPS_BATCH_VARIANT(2, 2, 2, "nl1_3r+nl5_3r+nl9_3r"),
+    TUPS_BATCH_VARIANT(0, 1, 1, "nl1_4m+nl5_3m+nl9_3m"),
 #undef TUPS_BATCH_VARIANT
 };
 constexpr int kNumBatchKernels = sizeof(kBatchKernels) / sizeof(kBatchKernels[0]);
 
-// variant id v = 5 * kernel + order: 0 resident grid, per-XCD lists, static
-// turns; 1 resident grid, lists walked as one array; 2, 3 the same with one
-// workgroup per entry (the hardware dispatcher balances); 4 resident grid,
-// per-XCD lists handed out by counters
-constexpr int kBatchOrders = 5;
+// variant id v = 3 * kernel + order: 0 resident grid taking its list's
+// entries in static turns; 1 one workgroup per entry; 2 resident grid drawing
+// entries from the per-list counters
+constexpr int kBatchOrders = 3;
 int batch_variant_count() { return kBatchOrders * kNumBatchKernels; }
 
 const char* batch_variant_name(int v) {
-  static const char* order[kBatchOrders] = {"_xcd", "_rr", "_xcd_flat", "_rr_flat", "_xcd_dyn"};
+  static const char* order[kBatchOrders] = {"_static", "_flat", "_dyn"};
   static std::string names[kBatchOrders * kNumBatchKernels];
   if (v < 0 || v >= batch_variant_count()) return "";
   if (names[v].empty()) names[v] = std::string(kBatchKernels[v / kBatchOrders].name) + order[v % kBatchOrders];
@@ -718,7 +727,7 @@ extern "C" int tups_batch_download(tups_batch* b, int32_t which, int32_t ch, dou
   return TUPS_OK;
 }
 
-static BatchArgs batch_args(const tups_batch* b, int xcd, int dyn) {
+static BatchArgs batch_args(const tups_batch* b, int dyn) {
   BatchArgs a{};
   for (int r = 0; r < 4; ++r) {
     a.gam[r] = b->gamma[r];
@@ -734,7 +743,6 @@ static BatchArgs batch_args(const tups_batch* b, int xcd, int dyn) {
   a.nl = b->d_nl;
   a.tickets = b->d_tickets;
   a.len = b->len;
-  a.xcd = xcd;
   a.dyn = dyn;
   a.alpha = b->alpha;
   const double cr[4] = {0.5, 0.5, -0.25, 0.25};
@@ -747,7 +755,7 @@ static BatchArgs batch_args(const tups_batch* b, int xcd, int dyn) {
 static int batch_launch(tups_batch* b, int v) {
   const BatchVariant& k = kBatchKernels[v / kBatchOrders];
   const int order = v % kBatchOrders;
-  const int xcd = order != 1 && order != 3, flat = order == 2 || order == 3, dyn = order == 4;
+  const int flat = order == 1, dyn = order == 2;
   int dev = 0, ncu = 0;
   TUPS_HIP(hipGetDevice(&dev));
   TUPS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -762,7 +770,7 @@ static int batch_launch(tups_batch* b, int v) {
     TUPS_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k.fn, dim3((unsigned)grid), dim3(256), (size_t)k.lds_bytes, b->stream,
-                     batch_args(b, xcd, dyn));
+                     batch_args(b, dyn));
   TUPS_HIP(hipGetLastError());
   return TUPS_OK;
 }
