@@ -242,6 +242,7 @@ def main():
     crossing5 = {"bound": "mfma", "achieved": round(fl5 / t5 / 1e12, 3), "peak": FP64_PEAK_TFLOPS,
                  "unit": "TFLOP/s", "frac": round(fl5 / t5 / 1e12 / FP64_PEAK_TFLOPS, 4),
                  "avg_launch_ms": round(p5_ms / p5_n, 3), "flops_per_call": fl5,
+                 "traffic": profile_traffic("crossing_pae_nl5_nx8_nf64.md"),
                  "bytes_per_call": crossing_bytes(c5), "kernel": "pae_ybuild + pae_mm (FP64 MFMA GEMM per bosonic frequency)",
                  "workload": f"calculate_PAE Nx=8 Nf=64 Nl=5 NR={c5.nr}, Nq={c5.nq} (all IBZ q), 4 channels"}
     print(f"[bench] crossing Nl=5: {p5_ms / p5_n:.2f} ms per call", file=sys.stderr, flush=True)
@@ -261,7 +262,8 @@ def main():
                 "roofline": {"bound": "mfma", "achieved": round(bfl / (bt_ms / 1e3) / 1e12, 3),
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": round(bfl / (bt_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
-                             "flops_per_launch": bfl},
+                             "traffic": profile_traffic("bse_batch_uneven_nx8_nq1024.md"),
+                             "avg_launch_ms": round(bt_ms, 4), "flops_per_launch": bfl},
                 "kernel": f"bse_batch_kernel variant {bvar} (one persistent launch, all q and channels)",
                 "items": bt.items,
                 "workload": f"Nx=8, Nq={bt.nq} (64 momenta x 16 bosonic freq), Nf_q 64->16 by frequency, "
