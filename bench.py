@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--nq", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the D=576, Nq=4096 (261 GB) share")
     ap.add_argument("--moduli", type=int, default=12, help="CRT moduli of the split form")
     ap.add_argument("--qbatch", type=int, default=0, help="q points per split pass (0: library default)")
     return ap.parse_args()
@@ -89,6 +90,61 @@ def cpu_baseline(nx, nf, nl, budget_s):
     return {"value": done / el, "unit": "q-points/s", "cores": int(threads), "kind": "port",
             "sample": f"{done} q-points of the D={D} ladder (4 channels x 4 ZGEMM + mixing each, "
                       f"oracle restatement of SRC/parquet_BSE.f90:62-132 on host BLAS), {el:.1f} s"}
+
+
+def large_share(moduli):
+    """Nx=16, Nl=9, Nf=64 (D=576), Nq=4096 = 256 momenta x 16 bosonic frequencies:
+    12 vertex arrays of 21.7 GB resident.  One FP64 and one split step (after a
+    warm-up each), and the split error on 16 q of the same inputs."""
+    import numpy as np
+    from tups_amd import BSE_LADDER_FP64, BSE_LADDER_SPLIT, GAMMA, TupsContext, TupsError
+    from tups_amd.synth import batched_bse_inputs
+
+    out = {"workload": "batched BSE ladder, Nx=16, D=576 (Nl=9 x Nf=64), Nq=4096 (256 momenta x 16 bosonic "
+                       "freq), 4 channels; Gamma, F, Phi of every channel resident (12 x 21.7 GB)"}
+    try:
+        c = TupsContext(nx=16, nf=64, nl=9, nq=4096, beta=2.0, f_damping=0.5)
+    except TupsError as e:
+        out["skipped"] = str(e)
+        return out
+    try:
+        batched_bse_inputs(c, seed=0, nmom=256, nomega=16)
+        ids = {n: v for v, n in c.variants()}
+        c.set_variant(ids["nl9_g4_3r"])
+        c.set_split(moduli, 0)
+        fl = 16.0 * c.nz ** 3 * 4 * c.nq
+        res = {}
+        for mode, key in ((BSE_LADDER_FP64, "fp64"), (BSE_LADDER_SPLIT, "split")):
+            c.BSE(0, mode=mode)
+            c.sync()
+            c.record(0)
+            c.BSE(0, mode=mode)
+            c.record(1)
+            c.sync()
+            ms = c.elapsed_ms(0, 1)
+            res[key] = {"q_points_per_s": round(c.nq / (ms / 1e3), 1), "ms_per_step": round(ms, 1),
+                        "algorithmic_tflops": round(fl / (ms / 1e3) / 1e12, 2)}
+            print(f"[bench] large share {key}: {ms:.0f} ms per step", file=sys.stderr, flush=True)
+        out.update(res)
+        out["fp64"]["frac_fp64_peak"] = round(res["fp64"]["algorithmic_tflops"] / FP64_PEAK_TFLOPS, 4)
+        out["split_vs_fp64"] = round(res["split"]["q_points_per_s"] / res["fp64"]["q_points_per_s"], 3)
+    except TupsError as e:
+        out["failed"] = str(e)
+        return out
+    finally:
+        c.close()
+    # split error on 16 q of the same inputs (the device generator is counter-based per element)
+    e = TupsContext(nx=16, nf=64, nl=9, nq=16, beta=2.0, f_damping=0.5)
+    batched_bse_inputs(e, seed=0, nmom=256, nomega=16)
+    e.BSE(0)
+    ref = [e.download(GAMMA, r) for r in range(4)]
+    batched_bse_inputs(e, seed=0, nmom=256, nomega=16)
+    e.set_split(moduli, 0)
+    e.BSE(0, mode=BSE_LADDER_SPLIT)
+    errs = [float(np.linalg.norm(e.download(GAMMA, r) - ref[r]) / np.linalg.norm(ref[r])) for r in range(4)]
+    e.close()
+    out["split_error"] = {"max": max(errs), "per_channel": errs, "vs": "FP64 form, 16 q of the same inputs"}
+    return out
 
 
 def main():
@@ -271,6 +327,8 @@ def main():
     print(f"[bench] batching: {bt_ms:.2f} ms per launch", file=sys.stderr, flush=True)
     bt.close()
 
+    large = None if args.no_large else large_share(args.moduli)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(nx, nf, nl, args.cpu_seconds)
@@ -302,6 +360,7 @@ def main():
         "crossing_roofline": crossing,
         "crossing_nl5": crossing5,
         "batching": batching,
+        "large_share": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
