@@ -144,9 +144,12 @@ int tups_calc_chi(tups_ctx* ctx, const double* sigma, const double* sigma_h, dou
 
 /* Split ladder settings (no counterpart in the reference): `moduli` CRT moduli
  * (8..16, default 12: relative error ~2e-11 at Nz = 320; 13 ~1e-12; 16 FP64
- * rounding), `qbatch` q points per residue+GEMM pass (0 = library default).
- * The workspace holds 8 N P^2 + 8 P bytes per (q, channel) of one pass,
- * P = Nz rounded up to a multiple of 64; the split mode needs Nz <= 1024. */
+ * rounding), `qbatch` q points per residue+GEMM pass (0 = library default:
+ * about 4 GiB of residues, at most an eighth of the free device memory, fixed
+ * at the first split call).  The workspace holds about 8 N P^2 + 72 P bytes per
+ * (q, channel) of one pass, P = Nz rounded up to a multiple of 64 (of 192 when
+ * the 192 x 192 GEMM tile is used), plus 2 N P^2 bytes of GEMM scratch in that
+ * case; the split mode needs Nz <= 1024. */
 int tups_set_split(tups_ctx* ctx, int32_t moduli, int32_t qbatch);
 /* One iteration of the path (no counterpart as one call; SRC/loop.f90:78-139,
  * 219-222 calls the steps one by one): BSE(mode) -> calc_PhiR -> calculate_PAE

@@ -32,6 +32,9 @@
 #include "internal.h"
 
 #include <cmath>
+#include <cstdint>
+#include <numeric>
+#include <type_traits>
 
 namespace tups {
 
@@ -54,6 +57,7 @@ struct CrtArgs {
   int* escale;                  // [mat][Dp] row exponents of A
   double* rmax;                 // [mat][8][Dp] partial row maxima of A
   int* fscale;                  // [mat][Dp] column exponents of B
+  int* crt;                     // large-tile GEMM: per-modulus (Re, Im) residue bytes of each tile
   int D, Dp, Kp, nf, nl;
   int q0, nqb, tiles_n;
   int nmod, ta, tb;
@@ -408,6 +412,239 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
     }
 }
 
+// ---- large-tile CRT GEMM --------------------------------------------------------------
+//
+// Workgroup tile T = 32 TB rows x 32 TB columns; four waves 2 x 2, each a
+// 16 TB x 16 TB tile = TB x TB blocks of v_mfma_i32_16x16x64_i8, so a wave
+// reads 2 TB fragments from LDS per TB^2 MFMAs (the 64 x 64 kernel above reads
+// 4 per 4, which leaves it bound by LDS bandwidth).  The steps run over
+// (modulus, sign, 64-byte K chunk): a step stages the T rows of A_sign and the
+// T columns of B_sign (2 x T x 64 bytes, 16-byte columns XOR-swizzled by
+// row / 4) into one of two LDS stages, from registers loaded three steps
+// earlier.  The INT32 products of the + sign are reduced to residues and kept
+// packed as bytes; after the - sign each element's (Re, Im) residue pair goes
+// to a global scratch (two bytes per element and modulus) instead of CRT sums
+// held in registers, and the CRT, scaling and damping run once at the end over
+// the scratch, a pair of elements at a time.
+template <int TB>
+struct CrtBig {
+  static constexpr int WT = 16 * TB;          // wave tile
+  static constexpr int T = 2 * WT;            // workgroup tile
+  static constexpr int VPT = T / 32;          // 16-byte pieces per thread per step (2 T rows x 4 / 256)
+  static constexpr int DEPTH = 3;             // steps in flight in registers
+  static constexpr int DW = 2 * TB * TB;      // scratch dwords per lane per modulus
+};
+
+template <int TB>
+__global__ __launch_bounds__(256, 1) void crt_gemm_big(CrtArgs a) {
+  using P = CrtBig<TB>;
+  constexpr int T = P::T, WT = P::WT, VPT = P::VPT, DEPTH = P::DEPTH, DW = P::DW;
+  __shared__ __attribute__((aligned(16))) int8_t S[2][2][T][64];  // [stage][A|B][row][byte]
+  const int tiles = a.tiles_n * a.tiles_n;
+  const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int mat = bid / tiles;
+  const int t = bid - mat * tiles;
+  const int row0 = (t / a.tiles_n) * T, col0 = (t % a.tiles_n) * T;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * WT, wc = (wave & 1) * WT;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nk = a.Kp / 64;
+  const int nsteps = a.nmod * 2 * nk;
+
+  // piece v of a thread: u = tid + 256 v -> (A|B, row, 16-byte column).  Its
+  // workspace address is a per-thread base plus a step offset common to A and
+  // B ((modulus, sign, K chunk) planes have the same strides in both); both are
+  // formed once, so the loop issues loads with no address arithmetic that
+  // could make it wait on earlier loads
+  const int8_t* pbase[VPT];
+  int lofs[VPT];
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) {
+    const int u = tid + 256 * v;
+    const int which = u / (4 * T), row = (u >> 2) % T, ch = u & 3;
+    const size_t off = ws_off(a, mat, 0, 0, (which ? col0 : row0) + row, 16 * ch);
+    pbase[v] = (which ? a.B : a.A) + off;
+    lofs[v] = (which * T + row) * 64 + 16 * (ch ^ ((row >> 2) & 3));
+  }
+  const size_t plane = (size_t)a.Dp * a.Kp;
+  auto step_off = [&](int step) __attribute__((always_inline)) -> size_t {
+    const int md = step / (2 * nk), rem = step - md * 2 * nk;
+    const int sign = rem / nk, kc = rem - sign * nk;
+    return ((size_t)md * 2 + sign) * plane + (size_t)kc * a.Dp * 64;
+  };
+  auto load = [&](int step, v4i* r) __attribute__((always_inline)) {
+    const size_t o = step_off(step);
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) r[v] = *reinterpret_cast<const v4i*>(pbase[v] + o);
+  };
+  auto put = [&](int st, const v4i* r) __attribute__((always_inline)) {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) *reinterpret_cast<v4i*>(&S[st][0][0][0] + lofs[v]) = r[v];
+  };
+  auto frag = [&](int st, int which, int row) __attribute__((always_inline)) -> v4i {
+    return *reinterpret_cast<const v4i*>(&S[st][which][row][16 * (lk ^ ((row >> 2) & 3))]);
+  };
+
+  v4i acc[TB][TB];
+  int pk[TB][TB];  // residues of the + product, 4 bytes (e = 0..3) per block
+#pragma unroll
+  for (int i = 0; i < TB; ++i)
+#pragma unroll
+    for (int j = 0; j < TB; ++j) {
+      acc[i][j] = v4i{0, 0, 0, 0};
+      pk[i][j] = 0;
+    }
+  int* scr = a.crt + (size_t)bid * a.nmod * DW * 256 + tid;  // dword w of modulus md: scr[(md DW + w) 256]
+
+  auto compute = [&](int st, int step) __attribute__((always_inline)) {
+    v4i bf[TB];
+#pragma unroll
+    for (int j = 0; j < TB; ++j) bf[j] = frag(st, 1, wc + 16 * j + li);
+#pragma unroll
+    for (int i = 0; i < TB; ++i) {
+      const v4i af = frag(st, 0, wr + 16 * i + li);
+#pragma unroll
+      for (int j = 0; j < TB; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[j], acc[i][j], 0, 0, 0);
+    }
+    const int md = step / (2 * nk), rem = step - md * 2 * nk;
+    if (rem == nk - 1) {  // + product complete: keep its residues
+      const double m = a.m[md], im = a.inv_m[md];
+#pragma unroll
+      for (int i = 0; i < TB; ++i)
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+          int w = 0;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) w |= ((int)sym_mod((double)acc[i][j][e], m, im) & 0xff) << (8 * e);
+          pk[i][j] = w;
+          acc[i][j] = v4i{0, 0, 0, 0};
+        }
+    } else if (rem == 2 * nk - 1) {  // - product complete: (Re, Im) residues to the scratch
+      const double m = a.m[md], im = a.inv_m[md], h2 = a.inv2[md], h2j = a.inv2j[md];
+#pragma unroll
+      for (int i = 0; i < TB; ++i)
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+          int w0 = 0, w1 = 0;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const double p = (double)(signed char)(pk[i][j] >> (8 * e)), n = (double)acc[i][j][e];
+            const int ur = (int)sym_mod((p + n) * h2, m, im) & 0xff;
+            const int ui = (int)sym_mod((p - n) * h2j, m, im) & 0xff;
+            const int pair = ur | (ui << 8);
+            if (e < 2)
+              w0 |= pair << (16 * e);
+            else
+              w1 |= pair << (16 * (e - 2));
+          }
+          const int x = i * TB + j;  // block index; dwords 2x, 2x + 1 hold elements e = 0,1 and 2,3
+          scr[((size_t)md * DW + 2 * x) * 256] = w0;
+          scr[((size_t)md * DW + 2 * x + 1) * 256] = w1;
+          acc[i][j] = v4i{0, 0, 0, 0};
+        }
+    }
+  };
+
+  // prologue: steps 0 and 1 staged in LDS; steps 2, 3, 4 in flight in registers
+  // (nsteps >= 16: at least 8 moduli x 2 signs).  Loads past the last step
+  // re-read the last step, so the loop issues the same loads every time.
+  v4i R[DEPTH][VPT];
+  load(0, R[0]);
+  put(0, R[0]);
+  load(1, R[1]);
+  put(1, R[1]);
+#pragma unroll
+  for (int d = 0; d < DEPTH; ++d) load(2 + d, R[(2 + d) % DEPTH]);
+  __syncthreads();
+  // step s (s % DEPTH = d, a compile-time register set): compute from stage
+  // s & 1; after the barrier that stage takes step s + 2 from registers, whose
+  // set then loads step s + 2 + DEPTH
+  auto step_body = [&](int s, auto dc) __attribute__((always_inline)) {
+    constexpr int d = decltype(dc)::value;
+    compute(s & 1, s);
+    __syncthreads();
+    put(s & 1, R[(d + 2) % DEPTH]);
+    load(min(s + 2 + DEPTH, nsteps - 1), R[(d + 2) % DEPTH]);
+  };
+  int s0 = 0;
+  for (; s0 + DEPTH <= nsteps; s0 += DEPTH) {
+    step_body(s0, std::integral_constant<int, 0>{});
+    step_body(s0 + 1, std::integral_constant<int, 1>{});
+    step_body(s0 + 2, std::integral_constant<int, 2>{});
+  }
+  static_assert(DEPTH == 3, "the loop above is unrolled for three register sets");
+  if (s0 < nsteps) step_body(s0, std::integral_constant<int, 0>{});
+  if (s0 + 1 < nsteps) step_body(s0 + 1, std::integral_constant<int, 1>{});
+
+  // CRT over the scratch, two elements (one dword) at a time; element e of
+  // block (i, j) sits at row 4 (lane / 16) + e, column lane % 16
+  const int D = a.D;
+  const size_t DD = (size_t)D * D;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  const double2* Gm = a.gam[r] + DD * q;
+  const double2* Fm = a.fv[r] + DD * q;
+  double2* O = a.out[r] + DD * q;
+  const double cf = a.cf[r];
+  // groups of TB dwords (2 TB elements); the scratch is read four moduli at a
+  // time, all 4 TB loads of a chunk issued before any is used (a chunk past
+  // nmod re-reads the last modulus with weight 0)
+  for (int w0 = 0; w0 < DW; w0 += TB) {
+    double shr[TB][2], slr[TB][2], shi[TB][2], sli[TB][2];
+#pragma unroll
+    for (int g = 0; g < TB; ++g)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) shr[g][h] = slr[g][h] = shi[g][h] = sli[g][h] = 0.0;
+    for (int m0 = 0; m0 < a.nmod; m0 += 4) {
+      int dv[4][TB];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int mc = min(m0 + k, a.nmod - 1);
+#pragma unroll
+        for (int g = 0; g < TB; ++g) dv[k][g] = scr[((size_t)mc * DW + w0 + g) * 256];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const bool live = m0 + k < a.nmod;
+        const int mk = min(m0 + k, CRT_MAX - 1);
+        const double wh = live ? a.whi[mk] : 0.0, wl = live ? a.wlo[mk] : 0.0;
+#pragma unroll
+        for (int g = 0; g < TB; ++g)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int d = dv[k][g];
+            const double ur = (double)(signed char)(d >> (16 * h)), ui = (double)(signed char)(d >> (16 * h + 8));
+            shr[g][h] = fma(ur, wh, shr[g][h]);
+            slr[g][h] = fma(ur, wl, slr[g][h]);
+            shi[g][h] = fma(ui, wh, shi[g][h]);
+            sli[g][h] = fma(ui, wl, sli[g][h]);
+          }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < TB; ++g) {
+      const int w = w0 + g;
+      const int x = w >> 1, i = x / TB, j = x - i * TB, e0 = 2 * (w & 1);
+      const int gj = col0 + wc + 16 * j + li;
+      const int gi0 = row0 + wr + 16 * i + 4 * lk + e0;
+      if (gj >= D) continue;
+      const int fj = a.fscale[(size_t)mat * a.Dp + gj];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int gi = gi0 + h;
+        if (gi >= D) continue;
+        const double qr = rint((shr[g][h] + slr[g][h]) * a.inv_mm);
+        const double qi = rint((shi[g][h] + sli[g][h]) * a.inv_mm);
+        const double cr = fma(-qr, a.mhi, shr[g][h]) + fma(-qr, a.mlo, slr[g][h]);
+        const double ci = fma(-qi, a.mhi, shi[g][h]) + fma(-qi, a.mlo, sli[g][h]);
+        const double s = ldexp(cf, a.escale[(size_t)mat * a.Dp + gi] - a.ta + fj - a.tb);
+        const size_t idx = (size_t)gj * D + gi;
+        const double2 fv = Fm[idx], gv = Gm[idx];
+        O[idx] = make_double2(fma(s, cr, a.omf * (fv.x - gv.x)), fma(s, ci, a.omf * (fv.y - gv.y)));
+      }
+    }
+  }
+}
+
 // ---- host: moduli constants ---------------------------------------------------------
 
 typedef unsigned __int128 u128;
@@ -470,6 +707,33 @@ static void crt_constants(CrtArgs& a, int nmod, int D) {
   a.tb = std::min(52, tsum - tsum / 2);
 }
 
+// GEMM tile for an order D: 6 selects crt_gemm_big<6> (192 x 192 tiles) when
+// padding D to 192 costs at most 7% over the 64-row padding, else 0 (the
+// 64 x 64 kernel).  Measured per 256 q: D = 576 24.2 ms (big) vs 28.7 ms;
+// at D = 320 the 160-row variant (TB = 5) lost to the 64 x 64 kernel
+// (24.0 vs 22.6 ms: its per-modulus reductions and scratch epilogue weigh
+// more against the shorter inner dimension), so it is not selected.  The
+// padded order Dp is a multiple of the tile and of 64.
+static int crt_big_tb(int D) {
+  const long long d64 = (D + 63) / 64 * 64;
+  const long long L = std::lcm(32 * 6, 64);
+  return (D + L - 1) / L * L * 100 <= d64 * 107 ? 6 : 0;
+}
+
+static size_t crt_padded(int D) {
+  const int tb = crt_big_tb(D);
+  const size_t L = tb ? std::lcm(32 * tb, 64) : 64;
+  return ((size_t)D + L - 1) / L * L;
+}
+
+// scratch bytes per matrix of the large-tile GEMM
+static size_t crt_scratch_per_mat(int D, int nmod) {
+  const int tb = crt_big_tb(D);
+  if (!tb) return 0;
+  const size_t tn = crt_padded(D) / (32 * tb);
+  return tn * tn * (size_t)nmod * (2 * tb * tb) * 256 * sizeof(int);
+}
+
 // q points per pass and workspace of the split ladder; allocates (outside any
 // stream capture) when the workspace is too small
 int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
@@ -477,12 +741,23 @@ int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
   const int nmod = c->split_moduli;
   TUPS_CHECK(nmod >= 8 && nmod <= CRT_MAX, TUPS_EINVAL, "split moduli must be in 8..16");
   TUPS_CHECK(s.nz <= 1024, TUPS_EINVAL, "split path needs Nz <= 1024 (exact INT32 sums)");
-  const size_t Dp = ((size_t)s.nz + 63) / 64 * 64;
+  const size_t Dp = crt_padded(s.nz);
   const size_t Kp = (2 * (size_t)s.nz + 63) / 64 * 64;
-  // per q point: 4 channels x (A and B) x 2 signs x nmod residue planes + exponents
-  const size_t per_q = 4 * (4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int) + CRT_KB * Dp * sizeof(double));
-  // default pass: about 1 GiB of residues
-  const size_t want = c->split_qbatch > 0 ? (size_t)c->split_qbatch : ((size_t)1 << 30) / per_q;
+  // per q point: 4 channels x (A and B) x 2 signs x nmod residue planes + exponents + GEMM scratch
+  const size_t per_q = 4 * (4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int) + CRT_KB * Dp * sizeof(double) +
+                            crt_scratch_per_mat(s.nz, nmod) + 16);
+  // default pass: about 4 GiB of residues, at most an eighth of the free memory
+  // (fewer, larger passes measured 3-5% faster than 1 GiB ones); fixed at
+  // first use, so a later call (possibly under stream capture) finds the same
+  // pass size and the workspace already in place
+  if (c->split_qbatch == 0 && c->split_auto_q == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+    const size_t budget =
+        std::max<size_t>(per_q, std::min<size_t>((size_t)4 << 30, (free_b + c->split_ws_bytes) / 8));
+    c->split_auto_q = (int)std::max<size_t>(1, std::min<size_t>(s.nq, budget / per_q));
+  }
+  const size_t want = c->split_qbatch > 0 ? (size_t)c->split_qbatch : (size_t)c->split_auto_q;
   const int qbatch = (int)std::max<size_t>(1, std::min<size_t>(s.nq, want));
   const size_t need = per_q * qbatch;
   if (c->split_ws_bytes < need) {
@@ -506,8 +781,9 @@ int launch_bse_split(tups_ctx* c) {
   const int nmod = c->split_moduli;
   int qbatch = 0;
   if (int rc = bse_split_reserve(c, &qbatch)) return rc;
-  const size_t Dp = ((size_t)s.nz + 63) / 64 * 64;
+  const size_t Dp = crt_padded(s.nz);
   const size_t Kp = (2 * (size_t)s.nz + 63) / 64 * 64;
+  const int tb = crt_big_tb(s.nz);
   CrtArgs a{};
   for (int r = 0; r < 4; ++r) {
     a.gam[r] = c->gamma[r];
@@ -521,7 +797,7 @@ int launch_bse_split(tups_ctx* c) {
   a.Kp = (int)Kp;
   a.nf = s.nf;
   a.nl = s.nl;
-  a.tiles_n = (int)(Dp / CRT_TILE);
+  a.tiles_n = (int)(Dp / (tb ? 32 * tb : CRT_TILE));
   a.alpha = s.alpha;
   const double cr[4] = {0.5, 0.5, -0.25, 0.25};
   for (int r = 0; r < 4; ++r) a.cf[r] = cr[r] * c->f_damping;
@@ -538,6 +814,7 @@ int launch_bse_split(tups_ctx* c) {
     a.rmax = reinterpret_cast<double*>(a.B + mats * 2 * nmod * Dp * Kp);  // 8-byte aligned: Kp % 64 == 0
     a.escale = reinterpret_cast<int*>(a.rmax + mats * CRT_KB * Dp);
     a.fscale = a.escale + mats * Dp;
+    a.crt = reinterpret_cast<int*>((reinterpret_cast<uintptr_t>(a.fscale + mats * Dp) + 15) / 16 * 16);
     if (int rc = timer_begin(c)) return rc;
     hipLaunchKernelGGL(crt_rowmax_a, dim3((unsigned)(Dp / 64 * CRT_KB), (unsigned)mats), dim3(256), 0, c->stream,
                        a);
@@ -552,7 +829,8 @@ int launch_bse_split(tups_ctx* c) {
     const long long nblk = (long long)a.tiles_n * a.tiles_n * mats;
     TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "split grid too large");
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL(crt_gemm, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+    void (*gemm)(CrtArgs) = tb == 6 ? crt_gemm_big<6> : crt_gemm;
+    hipLaunchKernelGGL(gemm, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_SPLIT_GEMM)) return rc;
   }

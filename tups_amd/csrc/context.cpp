@@ -453,6 +453,7 @@ int tups_set_split(tups_ctx* c, int32_t moduli, int32_t qbatch) {
   drop_graphs(c);
   c->split_moduli = moduli;
   c->split_qbatch = qbatch;
+  c->split_auto_q = 0;
   return TUPS_OK;
 }
 

@@ -103,6 +103,7 @@ struct tups_ctx {
   int pae_variant = -1;
   int split_moduli = 12;         // split ladder: CRT moduli (8..16)
   int split_qbatch = 0;          // split ladder: q points per pass (0: default)
+  int split_auto_q = 0;          // split ladder: the default, fixed at first use until set_split
   void* split_ws = nullptr;      // split ladder: residues and exponents of one pass
   size_t split_ws_bytes = 0;
   void* closed_ws = nullptr;     // closed form: augmented matrices of one pass
