@@ -136,9 +136,13 @@ __global__ __launch_bounds__(256) void crt_rowmax_a(CrtArgs a) {
 }
 
 // A = [Gamma | F] residues: one workgroup per 64 rows x 64 inner indices; lane
-// = row, wave w converts inner indices 16 w .. 16 w + 15 of the chunk and
-// stores 16 bytes per (modulus, sign).  grid: ((Dp / 64) * (Kp / 64), 4 * nqb)
+// = row, wave w converts inner indices 16 w .. 16 w + 15 of the chunk.  Per
+// modulus the (+, -) residue bytes of the block go through LDS (rows padded to
+// 80 bytes) so the global stores are whole 64-byte rows, 1 KiB per wave and
+// instruction.  grid: ((Dp / 64) * (Kp / 64), 4 * nqb)
+constexpr int PREP_A_ROW = 80;
 __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
+  __shared__ __attribute__((aligned(16))) int8_t T[2][2][64 * PREP_A_ROW];  // [buffer][sign][row][80]
   const int mat = blockIdx.y;
   const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
   const int D = a.D;
@@ -147,7 +151,7 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
   const double2* F = a.fv[r] + DD * q;
   const int nkc = a.Kp / 64;
   const int rb = blockIdx.x / nkc, kc = blockIdx.x - rb * nkc;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int row = rb * 64 + lane;
   const bool rok = row < D;
   const int k0 = kc * 64 + wave * 16;
@@ -167,7 +171,10 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
     limbs(rint(v.x * sc), r1[u], r0[u]);
     limbs(rint(v.y * sc), i1[u], i0[u]);
   }
+  // write-out: thread tid moves 16 bytes (column tid & 3) of row tid / 4 per sign
+  const int orow = tid >> 2, och = tid & 3;
   for (int mod = 0; mod < a.nmod; ++mod) {
+    const int buf = mod & 1;
     int wp[4] = {0, 0, 0, 0}, wm[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
@@ -176,8 +183,14 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
       wp[u >> 2] |= bp << (8 * (u & 3));
       wm[u >> 2] |= bm << (8 * (u & 3));
     }
-    *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 0, row, k0)) = v4i{wp[0], wp[1], wp[2], wp[3]};
-    *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, 1, row, k0)) = v4i{wm[0], wm[1], wm[2], wm[3]};
+    // buffer buf was last read two moduli ago, before the barrier of the previous one
+    *reinterpret_cast<v4i*>(&T[buf][0][lane * PREP_A_ROW + 16 * wave]) = v4i{wp[0], wp[1], wp[2], wp[3]};
+    *reinterpret_cast<v4i*>(&T[buf][1][lane * PREP_A_ROW + 16 * wave]) = v4i{wm[0], wm[1], wm[2], wm[3]};
+    __syncthreads();
+#pragma unroll
+    for (int sg = 0; sg < 2; ++sg)
+      *reinterpret_cast<v4i*>(a.A + ws_off(a, mat, mod, sg, rb * 64 + orow, kc * 64 + 16 * och)) =
+          *reinterpret_cast<const v4i*>(&T[buf][sg][orow * PREP_A_ROW + 16 * och]);
   }
 }
 
