@@ -86,6 +86,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
 // time-reversed form (conj) selected and invalid entries zero.  The crossing
 // then reads Y contiguously: F = Phi + Lambda Nc + (1/Nx) sum_R M(Q,R) (Y_a + Y_b).
 
+// grid: ((per + 255) / 256, 8): blockIdx.y = k, one output entry per thread
 __global__ __launch_bounds__(256) void pae_pregather_nl1(const double2* __restrict__ pd,
                                                          const double2* __restrict__ pm,
                                                          const double2* __restrict__ ps,
@@ -96,41 +97,39 @@ __global__ __launch_bounds__(256) void pae_pregather_nl1(const double2* __restri
   const size_t per = nf2 * nw * nr;  // one k slice
   const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= per) return;
+  const int k = blockIdx.y;  // 2 * routine + target of the pair
   const int nu1 = (int)(gid % nf) + 1;
   const int nu2 = (int)((gid / nf) % nf) + 1;
   const int qw = (int)((gid / nf2) % nw) + 1;
   const int R = (int)(gid / (nf2 * nw));
   const size_t rb = nf2 * (size_t)nw * R;  // PhiR(nu_a, nu_b, w2, R) = rb + (w2-1)*nf2 + (nub-1)*nf + nua-1
-  auto at = [&](const Term& t) { return rb + (size_t)(t.w2 - 1) * nf2 + (size_t)(t.nub - 1) * nf + (t.nua - 1); };
-  auto comb = [&](const double2* p0, const double2* p1, const Term& t, double c0, double c1) {
-    const size_t i = at(t);
-    const double2 x0 = p0[i], x1 = p1[i];
-    const double im = t.tr ? -1.0 : 1.0;
-    return make_double2(t.wt * (c0 * x0.x + c1 * x1.x), t.wt * im * (c0 * x0.y + c1 * x1.y));
-  };
   const int dn = nu2 - nu1;
   const bool pb_plain = dn >= 0 && dn + 1 <= nw;
   const bool pb_tr = dn < 0 && 1 - dn <= nw;
-  const Term t0 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(nu1 + qw - 1, nf), 1 - dn,
-                            nf + 1 - nu1, clampnu(nf + 1 - (nu1 + qw - 1), nf));
-  int w2 = nu1 + nu2 + qw - nf - 1, w2t = nf + 3 - qw - nu1 - nu2;
-  const Term t1 = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
-                            nf + 1 - nu1, nu1 + qw + w2t - 2);
-  w2 = qw - nu1 - nu2 + nf + 1;
-  w2t = nu1 + nu2 - qw - nf + 1;
-  const Term t2 = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t,
-                            nf + 1 - nu1, nu1 - qw - w2t + 2);
-  const Term t3 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(qw - nu1 - dn + nf, nf), 1 - dn,
-                            nf + 1 - nu1, clampnu(nu1 - qw - (1 - dn) + 2, nf));
-  const size_t o = gid;  // same (nu1, nu2, qw, R) ordering as the k slices
-  Y[0 * per + o] = comb(pd, pm, t0, -0.5, -1.5);  // phbar->ph  -> d
-  Y[1 * per + o] = comb(pd, pm, t0, -0.5, 0.5);   //            -> m
-  Y[2 * per + o] = comb(ps, pt, t1, 0.5, 1.5);    // pp->ph     -> d
-  Y[3 * per + o] = comb(ps, pt, t1, -0.5, 0.5);   //            -> m
-  Y[4 * per + o] = comb(pd, pm, t2, 0.5, -1.5);   // ph->pp     -> s
-  Y[5 * per + o] = comb(pd, pm, t2, 0.5, 0.5);    //            -> t
-  Y[6 * per + o] = comb(pd, pm, t3, 0.5, -1.5);   // phbar->pp  -> s
-  Y[7 * per + o] = comb(pd, pm, t3, -0.5, -0.5);  //            -> t
+  Term t;
+  if (k < 2) {  // phbar->ph
+    t = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(nu1 + qw - 1, nf), 1 - dn, nf + 1 - nu1,
+                  clampnu(nf + 1 - (nu1 + qw - 1), nf));
+  } else if (k < 4) {  // pp->ph
+    const int w2 = nu1 + nu2 + qw - nf - 1, w2t = nf + 3 - qw - nu1 - nu2;
+    t = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t, nf + 1 - nu1, nu1 + qw + w2t - 2);
+  } else if (k < 6) {  // ph->pp
+    const int w2 = qw - nu1 - nu2 + nf + 1, w2t = nu1 + nu2 - qw - nf + 1;
+    t = make_term(w2 >= 1 && w2 <= nw, w2t >= 2 && w2t <= nw, w2, nu1, nu2, w2t, nf + 1 - nu1, nu1 - qw - w2t + 2);
+  } else {  // phbar->pp
+    t = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(qw - nu1 - dn + nf, nf), 1 - dn, nf + 1 - nu1,
+                  clampnu(nu1 - qw - (1 - dn) + 2, nf));
+  }
+  // source channels and factors per k (SRC/parquet_EQOuter.f90:85-112)
+  const double2* p0 = k == 2 || k == 3 ? ps : pd;
+  const double2* p1 = k == 2 || k == 3 ? pt : pm;
+  constexpr double c0s[8] = {-0.5, -0.5, 0.5, -0.5, 0.5, 0.5, 0.5, -0.5};
+  constexpr double c1s[8] = {-1.5, 0.5, 1.5, 0.5, -1.5, 0.5, -1.5, -0.5};
+  const double c0 = c0s[k], c1 = c1s[k];
+  const size_t i = rb + (size_t)(t.w2 - 1) * nf2 + (size_t)(t.nub - 1) * nf + (t.nua - 1);
+  const double2 x0 = p0[i], x1 = p1[i];
+  const double im = t.tr ? -1.0 : 1.0;
+  Y[(size_t)k * per + gid] = make_double2(t.wt * (c0 * x0.x + c1 * x1.x), t.wt * im * (c0 * x0.y + c1 * x1.y));
 }
 
 struct PaeNl1Args {
@@ -288,6 +287,7 @@ static const PaeVariant kPaeVariants[] = {
     {4, 0, true, pae_stream_nl1<4, true>, "nl1_c4"},
     {8, 0, true, pae_stream_nl1<8, true>, "nl1_c8"},
     {4, 0, false, pae_stream_nl1<4, false>, "nl1_c4_nr"},
+    {1, 1, true, pae_stream_nl1_mom<1>, "nl1_m1"},
     {1, 2, true, pae_stream_nl1_mom<2>, "nl1_m2"},
     {1, 4, true, pae_stream_nl1_mom<4>, "nl1_m4"},
     {1, 8, true, pae_stream_nl1_mom<8>, "nl1_m8"},
@@ -336,7 +336,7 @@ static PaeNl1Args nl1_args(tups_ctx* c) {
 static int pae_pregather(tups_ctx* c) {
   const tups_sizes& s = c->sz;
   const size_t per = (size_t)s.nf * s.nf * s.nw * s.nr;
-  hipLaunchKernelGGL(pae_pregather_nl1, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, c->stream,
+  hipLaunchKernelGGL(pae_pregather_nl1, dim3((unsigned)((per + 255) / 256), 8), dim3(256), 0, c->stream,
                      c->phir[0], c->phir[1], c->phir[2], c->phir[3], c->pae_y, s.nf, s.nw, s.nr);
   TUPS_HIP(hipGetLastError());
   return TUPS_OK;
