@@ -52,7 +52,8 @@ def _run(c, gam, fv, chic, moduli, qbatch=0):
      (4, 8, 9, 0, 0.3, 7),      # Nz=72: two tiles, several passes
      (8, 64, 5, 3, 0.5, 0),     # Nz=320: the batched-BSE shape
      (16, 64, 9, 2, 1.0, 1),    # Nz=576, one q per pass
-     (8, 64, 1, 5, 0.5, 2)],    # Nz=64: exact tile, ragged last pass
+     (8, 64, 1, 5, 0.5, 2),     # Nz=64: exact tile, ragged last pass
+     (4, 112, 9, 1, 0.5, 0)],   # Nz=1008: bubble blocks too large for LDS beside the columns
 )
 def test_split_ladder_parity(nx, nf, nl, nq, f, qbatch):
     _say(f"split parity nx={nx} nf={nf} nl={nl} nq={nq} qbatch={qbatch}")

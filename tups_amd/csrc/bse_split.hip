@@ -194,79 +194,99 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
   }
 }
 
-// B = [alpha chi F ; alpha chi Gamma]: one wave per column.  The wave first
-// forms the column's 2D mixed values with lanes along nu (coalesced reads of
-// X and chi) into LDS, then lane l converts inner indices 4 l + 256 n .. + 3
-// and stores one dword per (modulus, sign).  grid: (Dp / 4, 4 * nqb);
-// dynamic LDS 4 * Kp * 16 bytes.
-template <int NL>
+// B = [alpha chi F ; alpha chi Gamma]: each wave does CPW columns in turn.
+// With CL the q's bubble blocks are first staged in LDS with nu fastest
+// ([l1][l3][nu]), so the mixing reads them conflict-free instead of every
+// column re-reading them from the cache.  Per column the wave forms its 2D
+// mixed values with lanes along nu (coalesced reads of X) into LDS, then lane
+// l converts inner indices 4 l + 256 n .. + 3 and stores one dword per
+// (modulus, sign).  grid: (Dp / (4 CPW), 4 * nqb); dynamic LDS
+// 4 Kp 16 bytes (+ nl^2 nf 16 with CL).
+constexpr int PREP_B_CPW = 4;
+template <int NL, bool CL>
 __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
-  extern __shared__ double2 colv[];  // [wave][Kp]
+  extern __shared__ double2 colv[];  // [wave][Kp], then with CL the bubble blocks [l1][l3][nu]
   const int mat = blockIdx.y;
   const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
   const int D = a.D, nf = a.nf;
   constexpr int nl = NL;
   const size_t DD = (size_t)D * D;
-  const double2* chiq = a.chi[r >> 1] + (size_t)nf * nl * nl * q;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int col = blockIdx.x * 4 + wave;
-  const bool cok = col < D;
-  double2* v = colv + (size_t)wave * a.Kp;
-  // B((part, l1, nu), col) = alpha sum_l3 chi_nu(l1, l3) X((l3, nu), col), X = F (part 0), Gamma (part 1)
-  double mx = 0.0;
-  for (int nu = lane; nu < nf; nu += 64) {
-    const double2* cb = chiq + (size_t)nu * nl * nl;
-    double2 xf[NL], xg[NL];
-#pragma unroll
-    for (int l3 = 0; l3 < NL; ++l3) {
-      xf[l3] = cok ? a.fv[r][DD * q + (size_t)col * D + l3 * nf + nu] : make_double2(0.0, 0.0);
-      xg[l3] = cok ? a.gam[r][DD * q + (size_t)col * D + l3 * nf + nu] : make_double2(0.0, 0.0);
+  const double2* chig = a.chi[r >> 1] + (size_t)nf * nl * nl * q;  // [nu][l3][l1]
+  double2* chil = colv + 4 * (size_t)a.Kp;
+  if (CL) {
+    for (int u = threadIdx.x; u < nl * nl * nf; u += 256) {
+      const int nu = u % nf, l = u / nf;  // l = l1 + nl l3
+      chil[u] = chig[(size_t)nu * nl * nl + l];
     }
-#pragma unroll
-    for (int l1 = 0; l1 < NL; ++l1) {
-      double fr = 0.0, fi = 0.0, gr = 0.0, gi = 0.0;
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double2* v = colv + (size_t)wave * a.Kp;
+  for (int cc = 0; cc < PREP_B_CPW; ++cc) {
+    const int col = (blockIdx.x * 4 + wave) * PREP_B_CPW + cc;
+    const bool cok = col < D;
+    // B((part, l1, nu), col) = alpha sum_l3 chi_nu(l1, l3) X((l3, nu), col), X = F (part 0), Gamma (part 1)
+    double mx = 0.0;
+    for (int nu = lane; nu < nf; nu += 64) {
+      double2 xf[NL], xg[NL];
 #pragma unroll
       for (int l3 = 0; l3 < NL; ++l3) {
-        const double2 c = cb[l1 + nl * l3];
-        fr = fma(c.x, xf[l3].x, fma(-c.y, xf[l3].y, fr));
-        fi = fma(c.x, xf[l3].y, fma(c.y, xf[l3].x, fi));
-        gr = fma(c.x, xg[l3].x, fma(-c.y, xg[l3].y, gr));
-        gi = fma(c.x, xg[l3].y, fma(c.y, xg[l3].x, gi));
+        xf[l3] = cok ? a.fv[r][DD * q + (size_t)col * D + l3 * nf + nu] : make_double2(0.0, 0.0);
+        xg[l3] = cok ? a.gam[r][DD * q + (size_t)col * D + l3 * nf + nu] : make_double2(0.0, 0.0);
       }
-      const double2 mf = make_double2(a.alpha * fr, a.alpha * fi);
-      const double2 mg = make_double2(a.alpha * gr, a.alpha * gi);
-      v[l1 * nf + nu] = mf;
-      v[D + l1 * nf + nu] = mg;
-      mx = fmax(mx, fmax(fmax(fabs(mf.x), fabs(mf.y)), fmax(fabs(mg.x), fabs(mg.y))));
-    }
-  }
-  for (int k = 2 * D + lane; k < a.Kp; k += 64) v[k] = make_double2(0.0, 0.0);
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-  int f = 0;
-  if (mx > 0.0) frexp(mx, &f);
-  if (lane == 0) a.fscale[(size_t)mat * a.Dp + col] = f;
-  const double sc = ldexp(1.0, a.tb - f);
-  __syncthreads();  // the column in LDS is complete (each wave reads only its own)
-  for (int k0 = 4 * lane; k0 < a.Kp; k0 += 256) {
-    double r1[4], r0[4], i1[4], i0[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const double2 x = v[k0 + u];
-      limbs(rint(x.x * sc), r1[u], r0[u]);
-      limbs(rint(x.y * sc), i1[u], i0[u]);
+      for (int l1 = 0; l1 < NL; ++l1) {
+        double fr = 0.0, fi = 0.0, gr = 0.0, gi = 0.0;
+#pragma unroll
+        for (int l3 = 0; l3 < NL; ++l3) {
+          const double2 c = CL ? chil[(size_t)(l1 + nl * l3) * nf + nu] : chig[(size_t)nu * nl * nl + l1 + nl * l3];
+          fr = fma(c.x, xf[l3].x, fma(-c.y, xf[l3].y, fr));
+          fi = fma(c.x, xf[l3].y, fma(c.y, xf[l3].x, fi));
+          gr = fma(c.x, xg[l3].x, fma(-c.y, xg[l3].y, gr));
+          gi = fma(c.x, xg[l3].y, fma(c.y, xg[l3].x, gi));
+        }
+        const double2 mf = make_double2(a.alpha * fr, a.alpha * fi);
+        const double2 mg = make_double2(a.alpha * gr, a.alpha * gi);
+        v[l1 * nf + nu] = mf;
+        v[D + l1 * nf + nu] = mg;
+        mx = fmax(mx, fmax(fmax(fabs(mf.x), fabs(mf.y)), fmax(fabs(mg.x), fabs(mg.y))));
+      }
     }
-    for (int mod = 0; mod < a.nmod; ++mod) {
-      int wp = 0, wm = 0;
+    for (int k = 2 * D + lane; k < a.Kp; k += 64) v[k] = make_double2(0.0, 0.0);
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    int f = 0;
+    if (mx > 0.0) frexp(mx, &f);
+    if (lane == 0) a.fscale[(size_t)mat * a.Dp + col] = f;
+    const double sc = ldexp(1.0, a.tb - f);
+    // the wave's column in LDS is complete (each wave reads only its own)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int k0 = 4 * lane; k0 < a.Kp; k0 += 256) {
+      double r1[4], r0[4], i1[4], i0[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        int bp, bm;
-        gauss_residues(a, mod, r1[u], r0[u], i1[u], i0[u], bp, bm);
-        wp |= bp << (8 * u);
-        wm |= bm << (8 * u);
+        const double2 x = v[k0 + u];
+        limbs(rint(x.x * sc), r1[u], r0[u]);
+        limbs(rint(x.y * sc), i1[u], i0[u]);
       }
-      *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 0, col, k0)) = wp;
-      *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 1, col, k0)) = wm;
+      for (int mod = 0; mod < a.nmod; ++mod) {
+        int wp = 0, wm = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          int bp, bm;
+          gauss_residues(a, mod, r1[u], r0[u], i1[u], i0[u], bp, bm);
+          wp |= bp << (8 * u);
+          wm |= bm << (8 * u);
+        }
+        *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 0, col, k0)) = wp;
+        *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 1, col, k0)) = wm;
+      }
     }
+    // the next column overwrites v only after every lane has read it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
 }
 
@@ -747,6 +767,17 @@ static size_t crt_scratch_per_mat(int D, int nmod) {
   return tn * tn * (size_t)nmod * (2 * tb * tb) * 256 * sizeof(int);
 }
 
+// prep_b for Nl, with the bubble blocks staged in LDS when they fit beside the
+// four columns (160 KiB per workgroup); *lds receives the dynamic LDS bytes
+static void (*prep_b_kernel(int nl, size_t Kp, int nf, size_t* lds))(CrtArgs) {
+  const size_t cols = 4 * Kp * sizeof(double2), chi = (size_t)nl * nl * nf * sizeof(double2);
+  const bool cl = cols + chi <= 160 * 1024;
+  *lds = cols + (cl ? chi : 0);
+  if (nl == 1) return cl ? crt_prep_b<1, true> : crt_prep_b<1, false>;
+  if (nl == 5) return cl ? crt_prep_b<5, true> : crt_prep_b<5, false>;
+  return cl ? crt_prep_b<9, true> : crt_prep_b<9, false>;
+}
+
 // q points per pass and workspace of the split ladder; allocates (outside any
 // stream capture) when the workspace is too small
 int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
@@ -780,11 +811,11 @@ int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
     TUPS_HIP(hipMalloc(&c->split_ws, need));
     c->split_ws_bytes = need;
   }
-  const size_t lds_b = 4 * Kp * sizeof(double2);  // <= 128 KiB for Nz <= 1024
+  size_t lds_b = 0;
+  void (*prep_b)(CrtArgs) = prep_b_kernel(s.nl, Kp, s.nf, &lds_b);
   if (lds_b > 65536)
-    for (const void* f : {reinterpret_cast<const void*>(crt_prep_b<1>), reinterpret_cast<const void*>(crt_prep_b<5>),
-                          reinterpret_cast<const void*>(crt_prep_b<9>)})
-      TUPS_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    TUPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(prep_b), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds_b));
   if (qbatch_out) *qbatch_out = qbatch;
   return TUPS_OK;
 }
@@ -816,7 +847,8 @@ int launch_bse_split(tups_ctx* c) {
   for (int r = 0; r < 4; ++r) a.cf[r] = cr[r] * c->f_damping;
   a.omf = 1.0 - c->f_damping;
   crt_constants(a, nmod, s.nz);
-  const size_t lds_b = 4 * Kp * sizeof(double2);
+  size_t lds_b = 0;
+  void (*prep_b)(CrtArgs) = prep_b_kernel(s.nl, Kp, s.nf, &lds_b);
   for (int q0 = 0; q0 < s.nq; q0 += qbatch) {
     a.q0 = q0;
     a.nqb = std::min(qbatch, s.nq - q0);
@@ -835,8 +867,8 @@ int launch_bse_split(tups_ctx* c) {
     hipLaunchKernelGGL(crt_prep_a, dim3((unsigned)((Dp / 64) * (Kp / 64)), (unsigned)mats), dim3(256), 0,
                        c->stream, a);
     TUPS_HIP(hipGetLastError());
-    void (*prep_b)(CrtArgs) = s.nl == 1 ? crt_prep_b<1> : (s.nl == 5 ? crt_prep_b<5> : crt_prep_b<9>);
-    hipLaunchKernelGGL(prep_b, dim3((unsigned)(Dp / 4), (unsigned)mats), dim3(256), lds_b, c->stream, a);
+    hipLaunchKernelGGL(prep_b, dim3((unsigned)(Dp / (4 * PREP_B_CPW)), (unsigned)mats), dim3(256), lds_b, c->stream,
+                       a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_SPLIT_PREP)) return rc;
     const long long nblk = (long long)a.tiles_n * a.tiles_n * mats;
