@@ -142,6 +142,18 @@ int tups_bse(tups_ctx* ctx, int32_t mode);
  * coarse-graining Ngrain.  Synchronous w.r.t. the host buffers. */
 int tups_calc_chi(tups_ctx* ctx, const double* sigma, const double* sigma_h, double mu, int32_t ngrain);
 
+/* calc_SDE (SRC/parquet_SDE.f90:27-258) without its use_U2 second-order term
+ * calc_order_2nd (:263-340): the self-energy over the whole zone from the
+ * resident reducible vertices G_r_LL (the GAMMA arrays, holding Phi after the
+ * crossing as at SRC/loop.f90:154) and Lambda (L_d, L_m), in FP64.  sigma,
+ * sigma_h, mu, ngrain as for tups_calc_chi (they build G); f_range the
+ * enlarged frequency box of the auxiliary bubbles (0..8); u the Hubbard U;
+ * use_u2 selects the reference's use_U2 form of LChi (the caller then adds
+ * the second-order term).  sigma_out receives Sigma (Nx Ny Nf complex, the
+ * reference's Sigma layout).  Needs a full-IBZ context. */
+int tups_sde(tups_ctx* ctx, const double* sigma, const double* sigma_h, double mu, int32_t ngrain, int32_t f_range,
+             double u, int32_t use_u2, double* sigma_out);
+
 /* Split ladder settings (no counterpart in the reference): `moduli` CRT moduli
  * (8..16, default 12: relative error ~2e-11 at Nz = 320; 13 ~1e-12; 16 FP64
  * rounding), `qbatch` q points per residue+GEMM pass (0 = library default:
@@ -221,6 +233,7 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 #define TUPS_K_BSE_CLOSED 8   /* closed form: build of [I - kappa Gamma chi | Gamma] and its solve */
 #define TUPS_K_CALC_CHI 9     /* bubbles chi_ph, chi_pp for every q slot */
 #define TUPS_K_BSE_BATCH 10   /* batching form: variant names only (tups_variant_name/count) */
+#define TUPS_K_SDE 11         /* self-energy: auxiliary bubbles, vertex contraction, star sums */
 #define TUPS_K_COUNT 12
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);

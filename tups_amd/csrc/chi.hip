@@ -13,6 +13,7 @@
 // One workgroup per q slot; the frequency range is walked in chunks: the
 // k-space bubble of a chunk is formed in LDS, then projected on the l pairs
 // with k summed in the reference's order (kx fastest).
+#include "green.h"
 #include "internal.h"
 
 #include <cmath>
@@ -22,39 +23,13 @@ namespace tups {
 struct ChiArgs {
   double2* chi_ph;
   double2* chi_pp;
-  const double2* sigma;   // (Nx Ny Nf): ((kx-1) Ny + ky-1) Nf + nu-1
-  const double2* sigh;    // Sigma_H(Nx, Ny), column-major
+  GreenParams g;
   const double* ff;       // FF[l Nc + ky Nx + kx]
   const int* qm;          // IBZ momentum slot of each q
   const int* qw;          // bosonic index of each q (1-based)
-  const int* ibz;         // IBZ momenta: x[0..nibz), y[0..nibz) (1-based)
+  const int* ibz;         // IBZ momenta: x[0..nibz), then y[0..nibz) (1-based)
   int nx, nf, nl, nibz, ngrain, chunk;
-  double mu, beta;
 };
-
-__device__ __forceinline__ double2 green(const ChiArgs& a, int gx, int gy, int kx, int ky, int nu) {
-  // 1-based grain (gx, gy), momentum (kx, ky) and frequency nu (may lie outside 1..Nf)
-  const int nx = a.nx, ng = a.ngrain;
-  const double pi = 3.14159265358979323846;
-  const double dk = 2.0 * pi / nx;
-  const int half = (ng + 1) / 2;
-  const double px = dk * (kx - 1) + (gx - half) * dk / (double)ng;
-  const double py = dk * (ky - 1) + (gy - half) * dk / (double)ng;
-  const double ek = -2.0 * cos(px) - 2.0 * cos(py);
-  const double wn = pi / a.beta * (2.0 * (nu - a.nf / 2 - 1) + 1.0);
-  double2 s;
-  if (nu < 1 || nu > a.nf)
-    s = a.sigh[(kx - 1) + (size_t)nx * (ky - 1)];
-  else
-    s = a.sigma[((size_t)(kx - 1) * nx + (ky - 1)) * a.nf + (nu - 1)];
-  const double re = -ek + a.mu - s.x, im = wn - s.y;  // z = re + i im
-  const double d = re * re + im * im;
-  return make_double2(re / d, -im / d);
-}
-
-__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
 
 // grid: nq; dynamic LDS 2 * chunk * Nc * 16 bytes
 __global__ __launch_bounds__(256) void chi_kernel(ChiArgs a) {
@@ -78,11 +53,11 @@ __global__ __launch_bounds__(256) void chi_kernel(ChiArgs a) {
       double2 sph = make_double2(0.0, 0.0), spp = make_double2(0.0, 0.0);
       for (int gx = 1; gx <= ng; ++gx)
         for (int gy = 1; gy <= ng; ++gy) {
-          const double2 gk = green(a, gx, gy, kx, ky, nu);
-          const double2 p = cmul2(gk, green(a, gx, gy, kqx, kqy, nu_kq));
+          const double2 gk = green_coarse(a.g, gx, gy, kx, ky, nu);
+          const double2 p = cmul(gk, green_coarse(a.g, gx, gy, kqx, kqy, nu_kq));
           sph.x += p.x * inv_g2;
           sph.y += p.y * inv_g2;
-          const double2 t = cmul2(gk, green(a, ng - gx + 1, ng - gy + 1, qmkx, qmky, nu_qmk));
+          const double2 t = cmul(gk, green_coarse(a.g, ng - gx + 1, ng - gy + 1, qmkx, qmky, nu_qmk));
           spp.x += t.x * inv_g2;
           spp.y += t.y * inv_g2;
         }
@@ -112,10 +87,13 @@ __global__ __launch_bounds__(256) void chi_kernel(ChiArgs a) {
   }
 }
 
-int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain) {
+// Sigma and Sigma_H to the device (Sigma_H defaults to Sigma at the last box
+// frequency, SRC/parquet_chi.f90:103-110); synchronous w.r.t. the host buffers
+int upload_sigma(tups_ctx* c, const double* sigma, const double* sigma_h, GreenParams* g, double mu, int ngrain) {
   const tups_sizes& s = c->sz;
   const int nc = s.nx * s.ny;
   const size_t nt = (size_t)nc * s.nf;
+  TUPS_CHECK(sigma, TUPS_EINVAL, "null Sigma");
   TUPS_CHECK(ngrain >= 1 && ngrain <= 64, TUPS_EINVAL, "ngrain must be in 1..64");
   if (!c->d_sigma) TUPS_HIP(hipMalloc((void**)&c->d_sigma, nt * sizeof(double2)));
   if (!c->d_sigh) TUPS_HIP(hipMalloc((void**)&c->d_sigh, (size_t)nc * sizeof(double2)));
@@ -123,7 +101,6 @@ int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, dou
   if (sigma_h) {
     std::copy(sigma_h, sigma_h + 2 * (size_t)nc, sh.begin());
   } else {
-    // Sigma_H(kx, ky) = Sigma at the last box frequency (SRC/parquet_chi.f90:103-110)
     for (int kx = 1; kx <= s.nx; ++kx)
       for (int ky = 1; ky <= s.ny; ++ky) {
         const size_t src = ((size_t)(kx - 1) * s.ny + (ky - 1)) * s.nf + (s.nf - 1);
@@ -134,11 +111,24 @@ int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, dou
   }
   TUPS_HIP(hipMemcpyAsync(c->d_sigma, sigma, nt * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   TUPS_HIP(hipMemcpyAsync(c->d_sigh, sh.data(), (size_t)nc * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  TUPS_HIP(hipStreamSynchronize(c->stream));
+  g->sigma = c->d_sigma;
+  g->sigh = c->d_sigh;
+  g->nx = s.nx;
+  g->nf = s.nf;
+  g->ngrain = ngrain;
+  g->mu = mu;
+  g->beta = c->beta;
+  return TUPS_OK;
+}
+
+int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain) {
+  const tups_sizes& s = c->sz;
+  const int nc = s.nx * s.ny;
   ChiArgs a{};
+  if (int rc = upload_sigma(c, sigma, sigma_h, &a.g, mu, ngrain)) return rc;
   a.chi_ph = c->chi[0];
   a.chi_pp = c->chi[1];
-  a.sigma = c->d_sigma;
-  a.sigh = c->d_sigh;
   a.ff = c->d_ffk;
   a.qm = c->d_qm;
   a.qw = c->d_qw;
@@ -149,8 +139,6 @@ int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, dou
   a.nibz = s.nibz;
   a.ngrain = ngrain;
   a.chunk = std::max(1, std::min(s.nf, 2048 / nc));
-  a.mu = mu;
-  a.beta = c->beta;
   const size_t lds = 2 * (size_t)a.chunk * nc * sizeof(double2);
   TUPS_CHECK(lds <= 160 * 1024, TUPS_EINVAL, "calc_chi supports Nx <= 64");
   if (lds > 65536)
@@ -160,8 +148,6 @@ int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, dou
   hipLaunchKernelGGL(chi_kernel, dim3((unsigned)s.nq), dim3(256), lds, c->stream, a);
   TUPS_HIP(hipGetLastError());
   if (int rc = timer_end(c, TUPS_K_CALC_CHI)) return rc;
-  // the host buffers may be reused as soon as the call returns
-  TUPS_HIP(hipStreamSynchronize(c->stream));
   return TUPS_OK;
 }
 

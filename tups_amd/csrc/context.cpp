@@ -290,6 +290,7 @@ int tups_destroy(tups_ctx* c) {
   f(c->chi[0]);
   f(c->chi[1]);
   f(c->split_ws);
+  f(c->sde_ws);
   f(c->closed_ws);
   f(c->d_sigma);
   f(c->d_sigh);
@@ -444,6 +445,15 @@ int tups_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, doubl
   int rc = ensure_tables(c);
   if (rc) return rc;
   return launch_calc_chi(c, sigma, sigma_h, mu, ngrain);
+}
+
+int tups_sde(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int32_t ngrain, int32_t f_range,
+             double u, int32_t use_u2, double* sigma_out) {
+  TUPS_CHECK(c, TUPS_EINVAL, "null context");
+  TUPS_CHECK(full_ibz(c), TUPS_EINVAL, "the SDE sums over every IBZ q of the box (nq = 0 at create)");
+  int rc = ensure_tables(c);
+  if (rc) return rc;
+  return launch_sde(c, sigma, sigma_h, mu, ngrain, f_range, u, use_u2, sigma_out);
 }
 
 int tups_set_split(tups_ctx* c, int32_t moduli, int32_t qbatch) {

@@ -105,6 +105,8 @@ struct tups_ctx {
   int split_qbatch = 0;          // split ladder: q points per pass (0: default)
   int split_auto_q = 0;          // split ladder: the default, fixed at first use until set_split
   void* split_ws = nullptr;      // split ladder: residues and exponents of one pass
+  void* sde_ws = nullptr;        // SDE: auxiliary bubbles, contracted vertices, Sigma
+  size_t sde_ws_bytes = 0;
   size_t split_ws_bytes = 0;
   void* closed_ws = nullptr;     // closed form: augmented matrices of one pass
   size_t closed_ws_bytes = 0;
@@ -131,6 +133,10 @@ int ensure_tables(tups_ctx* c);  // rebuild and upload the tables when stale
 void drop_graphs(tups_ctx* c);
 int launch_bse_closed(tups_ctx* c);
 int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain);
+struct GreenParams;
+int upload_sigma(tups_ctx* c, const double* sigma, const double* sigma_h, GreenParams* g, double mu, int ngrain);
+int launch_sde(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain, int f_range, double u,
+               int use_u2, double* sigma_out);
 int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms);
 const char* bse_variant_name(int v);
 int bse_variant_count();

@@ -20,7 +20,7 @@ module tups_gpu_iface
   integer(c_int32_t), parameter :: TUPS_K_BSE_LADDER = 0, TUPS_K_PHIR = 1, TUPS_K_BUILD_M = 2, &
                                    TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4, TUPS_K_PAE_GATHER = 5, &
                                    TUPS_K_SPLIT_PREP = 6, TUPS_K_SPLIT_GEMM = 7, TUPS_K_BSE_CLOSED = 8, &
-                                   TUPS_K_CALC_CHI = 9, TUPS_K_BSE_BATCH = 10
+                                   TUPS_K_CALC_CHI = 9, TUPS_K_BSE_BATCH = 10, TUPS_K_SDE = 11
 
   type, bind(C) :: tups_dims
     integer(c_int32_t) :: nx, nf, nl, nr, nq, reserved
@@ -200,6 +200,18 @@ module tups_gpu_iface
       integer(c_int32_t), value :: variant
       integer(c_int) :: rc
     end function tups_batch_set_variant
+
+    function tups_sde(ctx, sigma, sigma_h, mu, ngrain, f_range, u, use_u2, sigma_out) &
+        bind(C, name="tups_sde") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: sigma(*)
+      type(c_ptr), value :: sigma_h
+      real(c_double), value :: mu, u
+      integer(c_int32_t), value :: ngrain, f_range, use_u2
+      complex(c_double_complex), intent(out) :: sigma_out(*)
+      integer(c_int) :: rc
+    end function tups_sde
 
     function tups_iteration(ctx, mode, use_graph) bind(C, name="tups_iteration") result(rc)
       import :: c_ptr, c_int32_t, c_int

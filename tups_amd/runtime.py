@@ -21,7 +21,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
 
 GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
 (K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER, K_SPLIT_PREP, K_SPLIT_GEMM,
- K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH) = range(11)
+ K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH, K_SDE) = range(12)
 CH_D, CH_M, CH_S, CH_T = range(4)
 BSE_LADDER_FP64 = 0
 BSE_LADDER_SPLIT = 1
@@ -69,6 +69,7 @@ SIGNATURES = {
     "tups_bse": ([_V, C.c_int32], C.c_int),
     "tups_set_split": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_calc_chi": ([_V, _D, _D, C.c_double, C.c_int32], C.c_int),
+    "tups_sde": ([_V, _D, _D, C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_int32, _D], C.c_int),
     "tups_iteration": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_batch_create": ([C.c_int32, C.c_int32, _I, _I, C.c_double, C.c_double, _P(_V)], C.c_int),
     "tups_batch_destroy": ([_V], C.c_int),
@@ -249,6 +250,21 @@ class TupsContext:
             sh = np.asfortranarray(np.asarray(sigma_h, dtype=np.complex128).reshape(self.nx, self.nx))
         _check(self.lib.tups_calc_chi(self._h, _dptr(s), _dptr(sh) if sh is not None else None, float(mu),
                                       int(ngrain)), "tups_calc_chi")
+
+    def sde(self, sigma, mu: float, U: float, ngrain: int = 1, f_range: int = 0, use_u2: bool = False,
+            sigma_h=None) -> np.ndarray:
+        """calc_SDE without its use_U2 second-order term (SRC/parquet_SDE.f90:27-258):
+        Sigma (Nx*Ny*Nf,) over the whole zone from the resident G_r_LL and Lambda.
+        sigma / sigma_h / mu / ngrain build G as in calc_chi."""
+        s = np.ascontiguousarray(np.asarray(sigma, dtype=np.complex128).ravel())
+        assert s.size == self.nx * self.nx * self.nf
+        sh = None
+        if sigma_h is not None:
+            sh = np.asfortranarray(np.asarray(sigma_h, dtype=np.complex128).reshape(self.nx, self.nx))
+        out = np.zeros(self.nx * self.nx * self.nf, dtype=np.complex128)
+        _check(self.lib.tups_sde(self._h, _dptr(s), _dptr(sh) if sh is not None else None, float(mu), int(ngrain),
+                                 int(f_range), float(U), 1 if use_u2 else 0, _dptr(out)), "tups_sde")
+        return out
 
     def iteration(self, mode: int = BSE_LADDER_FP64, graph: bool = True):
         """BSE -> calc_PhiR -> calculate_PAE -> G = F - G (SRC/loop.f90:78-139,219-222);
