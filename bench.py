@@ -327,6 +327,32 @@ def main():
     print(f"[bench] batching: {bt_ms:.2f} ms per launch", file=sys.stderr, flush=True)
     bt.close()
 
+    # self-energy (SDE without the use_U2 term) at Nx=8, Nf=64, Nl=5, all 480 IBZ q
+    from tups_amd import LAMBDA
+    from tups_amd.runtime import K_SDE
+    sc = TupsContext(nx=8, nf=64, nl=5, beta=2.0, f_damping=0.5)
+    for r in range(4):
+        sc.synth_vertex(GAMMA, r, 40 + r, 0.0, 0.1)
+    lam = np.zeros((64, 64, sc.nw), dtype=np.complex128)
+    lam[:] = U
+    sc.upload(LAMBDA, 0, lam)
+    lam[:] = -U
+    sc.upload(LAMBDA, 1, lam)
+    sig = np.zeros(8 * 8 * 64, dtype=np.complex128)
+    sc.sde(sig, 0.0, U)
+    sc.set_timing(True)
+    for _ in range(3):
+        sc.sde(sig, 0.0, U)
+    sde_ms, sde_n = sc.kernel_time(K_SDE)
+    sc.set_timing(False)
+    vbytes = 4.0 * sc.nz * sc.nz * sc.nq * 16
+    sde = {"ms_per_call": round(sde_ms / sde_n, 3), "vertex_bytes": vbytes,
+           "vertex_read_gbs": round(vbytes / (sde_ms / sde_n / 1e3) / 1e9, 1),
+           "kernel": "sde_chi_aux + sde_vchi + sde_sigma (FP64)",
+           "workload": f"calc_SDE without use_U2, Nx=8 Nf=64 Nl=5, Nq={sc.nq} (all IBZ q), f_range=0"}
+    print(f"[bench] sde: {sde_ms / sde_n:.2f} ms per call", file=sys.stderr, flush=True)
+    sc.close()
+
     large = None if args.no_large else large_share(args.moduli)
 
     cpu = None
@@ -361,6 +387,7 @@ def main():
         "crossing_nl5": crossing5,
         "batching": batching,
         "large_share": large,
+        "sde": sde,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
