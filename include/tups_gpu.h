@@ -169,6 +169,19 @@ int tups_set_split(tups_ctx* ctx, int32_t moduli, int32_t qbatch);
  * into a HIP graph on first use (per mode and vertex-buffer parity) and replays
  * it afterwards.  Needs a full-IBZ context with the M tensors built. */
 int tups_iteration(tups_ctx* ctx, int32_t mode, int32_t use_graph);
+/* Sigma on the device (no counterpart: the reference keeps Sigma in host
+ * module arrays): tups_set_sigma uploads Sigma (Nx Ny Nf complex), Sigma_H
+ * (NULL: Sigma(k, Nf), also after every device update), mu and ngrain for
+ * tups_iteration_full; tups_get_sigma downloads the current Sigma. */
+int tups_set_sigma(tups_ctx* ctx, const double* sigma, const double* sigma_h, double mu, int32_t ngrain);
+int tups_get_sigma(tups_ctx* ctx, double* sigma);
+/* The loop body of SRC/loop.f90:67-222 on the device (no counterpart as one
+ * call): calc_chi from the device Sigma -> BSE(mode) -> calc_PhiR ->
+ * calculate_PAE -> SDE (tups_sde's arguments f_range, u, use_u2) -> the new
+ * Sigma replaces the old -> G = F - G.  mu stays as set (the reference's
+ * bisection in calc_Gkw_Grt is not part of it).  use_graph != 0 captures the
+ * sequence once per (mode, buffer parity, SDE settings) as a HIP graph. */
+int tups_iteration_full(tups_ctx* ctx, int32_t mode, int32_t use_graph, int32_t f_range, double u, int32_t use_u2);
 
 /* --- batching form (no counterpart in the reference, which fixes Nf and Nl for
  * every q): a batch of nq transfer points, each with its own even Nf_q and

@@ -68,3 +68,52 @@ def test_iteration_needs_m_tensors():
     with TupsContext(nx=4, nf=8, nl=1, beta=2.0, f_damping=0.5) as c:
         with pytest.raises(TupsError):
             c.iteration(graph=True)
+
+
+@pytest.mark.parametrize("nx,nf,nl,split,f_range", [(4, 8, 1, False, 0), (4, 8, 5, False, 1), (6, 8, 1, True, 0)])
+def test_full_iteration_graph_equals_steps(nx, nf, nl, split, f_range):
+    """calc_chi -> BSE -> PhiR -> PAE -> SDE -> Sigma -> G = F - G with Sigma on the
+    device, as a graph and eagerly, against the same library calls made one by one
+    with Sigma passed through the host (SRC/loop.f90:67-222 without the mu bisection)."""
+    from tups_amd import BSE_LADDER_FP64, BSE_LADDER_SPLIT, F, GAMMA, PHIR, TupsContext
+
+    mode = BSE_LADDER_SPLIT if split else BSE_LADDER_FP64
+    U, mu = 2.0, 0.1
+    _say(f"full iteration nx={nx} nf={nf} nl={nl} mode={mode} f_range={f_range}")
+    rng = np.random.default_rng(23)
+    sigma0 = crandn(rng, (nx * nx * nf,), 0.1).ravel()
+    with TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as a, \
+            TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as b, \
+            TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as e:
+        for c in (a, b, e):
+            _setup(c, 29)
+        b.set_sigma(sigma0, mu)
+        e.set_sigma(sigma0, mu)
+        sigma = sigma0.copy()
+        for it in range(3):
+            a.calc_chi(sigma, mu)
+            a.BSE(it, mode=mode)
+            a.calc_PhiR(it)
+            a.calculate_PAE()
+            new_sigma = a.sde(sigma, mu, U, f_range=f_range)
+            a.gamma_update()
+            sigma = new_sigma
+            b.iteration_full(U, mode, graph=True, f_range=f_range)
+            e.iteration_full(U, mode, graph=False, f_range=f_range)
+            assert np.array_equal(sigma, b.get_sigma()), it
+            assert np.array_equal(sigma, e.get_sigma()), it
+            for which in (GAMMA, F, PHIR):
+                for r in range(4):
+                    x = a.download(which, r)
+                    assert np.array_equal(x, b.download(which, r)), (it, which, r)
+                    assert np.array_equal(x, e.download(which, r)), (it, which, r)
+            assert np.isfinite(sigma).all()
+
+
+def test_full_iteration_needs_sigma():
+    from tups_amd import TupsContext, TupsError
+
+    with TupsContext(nx=4, nf=8, nl=1, beta=2.0, f_damping=0.5) as c:
+        _setup(c, 3)
+        with pytest.raises(TupsError):
+            c.iteration_full(2.0, graph=True)

@@ -119,14 +119,19 @@ int upload_sigma(tups_ctx* c, const double* sigma, const double* sigma_h, GreenP
   g->ngrain = ngrain;
   g->mu = mu;
   g->beta = c->beta;
+  c->green_mu = mu;
+  c->green_ngrain = ngrain;
+  c->sigh_default = sigma_h == nullptr;
+  c->green_set = true;
   return TUPS_OK;
 }
 
-int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain) {
+// calc_chi from the Sigma already on the device (g)
+int launch_chi_device(tups_ctx* c, const GreenParams& g) {
   const tups_sizes& s = c->sz;
   const int nc = s.nx * s.ny;
   ChiArgs a{};
-  if (int rc = upload_sigma(c, sigma, sigma_h, &a.g, mu, ngrain)) return rc;
+  a.g = g;
   a.chi_ph = c->chi[0];
   a.chi_pp = c->chi[1];
   a.ff = c->d_ffk;
@@ -137,7 +142,7 @@ int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, dou
   a.nf = s.nf;
   a.nl = s.nl;
   a.nibz = s.nibz;
-  a.ngrain = ngrain;
+  a.ngrain = g.ngrain;
   a.chunk = std::max(1, std::min(s.nf, 2048 / nc));
   const size_t lds = 2 * (size_t)a.chunk * nc * sizeof(double2);
   TUPS_CHECK(lds <= 160 * 1024, TUPS_EINVAL, "calc_chi supports Nx <= 64");
@@ -149,6 +154,12 @@ int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, dou
   TUPS_HIP(hipGetLastError());
   if (int rc = timer_end(c, TUPS_K_CALC_CHI)) return rc;
   return TUPS_OK;
+}
+
+int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain) {
+  GreenParams g{};
+  if (int rc = upload_sigma(c, sigma, sigma_h, &g, mu, ngrain)) return rc;
+  return launch_chi_device(c, g);
 }
 
 }  // namespace tups

@@ -107,6 +107,13 @@ struct tups_ctx {
   void* split_ws = nullptr;      // split ladder: residues and exponents of one pass
   void* sde_ws = nullptr;        // SDE: auxiliary bubbles, contracted vertices, Sigma
   size_t sde_ws_bytes = 0;
+  bool green_set = false;        // d_sigma / d_sigh hold a Sigma (calc_chi, sde or set_sigma)
+  bool sigh_default = true;      // Sigma_H follows Sigma(k, Nf) when Sigma is replaced on the device
+  double green_mu = 0.0;
+  int green_ngrain = 1;
+  hipGraphExec_t graph_full[4] = {};  // full iteration, per (ladder mode, buffer parity)
+  int full_f_range = -1, full_use_u2 = 0;  // settings the full-iteration graphs were captured with
+  double full_u = 0.0;
   size_t split_ws_bytes = 0;
   void* closed_ws = nullptr;     // closed form: augmented matrices of one pass
   size_t closed_ws_bytes = 0;
@@ -137,6 +144,11 @@ struct GreenParams;
 int upload_sigma(tups_ctx* c, const double* sigma, const double* sigma_h, GreenParams* g, double mu, int ngrain);
 int launch_sde(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain, int f_range, double u,
                int use_u2, double* sigma_out);
+int launch_chi_device(tups_ctx* c, const GreenParams& g);
+int sde_reserve(tups_ctx* c, int f_range);
+int launch_sde_device(tups_ctx* c, const GreenParams& g, int f_range, double u, int use_u2);
+int launch_sigma_commit(tups_ctx* c, int f_range);
+int green_params(const tups_ctx* c, GreenParams* g);
 int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms);
 const char* bse_variant_name(int v);
 int bse_variant_count();

@@ -221,16 +221,13 @@ __global__ __launch_bounds__(256) void sde_sigma(SdeArgs a) {
   }
 }
 
-int launch_sde(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain, int f_range, double u,
-               int use_u2, double* sigma_out) {
+// workspace of the SDE for f_range (allocated outside any stream capture)
+int sde_reserve(tups_ctx* c, int f_range) {
   const tups_sizes& s = c->sz;
-  TUPS_CHECK(sigma_out, TUPS_EINVAL, "null output");
   TUPS_CHECK(f_range >= 0 && f_range <= 8, TUPS_EINVAL, "f_range must be in 0..8");
-  SdeArgs a{};
-  if (int rc = upload_sigma(c, sigma, sigma_h, &a.g, mu, ngrain)) return rc;
   const int nc = s.nx * s.ny;
-  a.ext = (2 * f_range + 1) * s.nf;
-  const size_t naux = (size_t)s.nq * s.nl * a.ext, nv = (size_t)s.nq * s.nz, nt = (size_t)nc * s.nf;
+  const size_t ext = (2 * (size_t)f_range + 1) * s.nf;
+  const size_t naux = (size_t)s.nq * s.nl * ext, nv = (size_t)s.nq * s.nz, nt = (size_t)nc * s.nf;
   const size_t need = (2 * naux + 2 * nv + nt) * sizeof(double2);
   if (c->sde_ws_bytes < need) {
     if (c->sde_ws) TUPS_HIP(hipFree(c->sde_ws));
@@ -239,6 +236,25 @@ int launch_sde(tups_ctx* c, const double* sigma, const double* sigma_h, double m
     TUPS_HIP(hipMalloc(&c->sde_ws, need));
     c->sde_ws_bytes = need;
   }
+  return TUPS_OK;
+}
+
+// Sigma output of the last SDE launch with this f_range (inside the workspace)
+double2* sde_output(tups_ctx* c, int f_range) {
+  const tups_sizes& s = c->sz;
+  const size_t ext = (2 * (size_t)f_range + 1) * s.nf;
+  const size_t naux = (size_t)s.nq * s.nl * ext, nv = (size_t)s.nq * s.nz;
+  return static_cast<double2*>(c->sde_ws) + 2 * naux + 2 * nv;
+}
+
+// the SDE from the Sigma on the device (g); the workspace must be reserved
+int launch_sde_device(tups_ctx* c, const GreenParams& g, int f_range, double u, int use_u2) {
+  const tups_sizes& s = c->sz;
+  SdeArgs a{};
+  a.g = g;
+  const int nc = s.nx * s.ny;
+  a.ext = (2 * f_range + 1) * s.nf;
+  const size_t naux = (size_t)s.nq * s.nl * a.ext, nv = (size_t)s.nq * s.nz;
   double2* ws = static_cast<double2*>(c->sde_ws);
   a.caux_ph = ws;
   a.caux_pp = ws + naux;
@@ -275,7 +291,44 @@ int launch_sde(tups_ctx* c, const double* sigma, const double* sigma_h, double m
   hipLaunchKernelGGL(sde_sigma, dim3((unsigned)(s.nibz * s.nf)), dim3(256), 0, c->stream, a);
   TUPS_HIP(hipGetLastError());
   if (int rc = timer_end(c, TUPS_K_SDE)) return rc;
-  TUPS_HIP(hipMemcpyAsync(sigma_out, a.sigma_out, nt * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  return TUPS_OK;
+}
+
+// Sigma <- the SDE output, and with the default rule Sigma_H(k) = Sigma(k, Nf)
+// (SRC/parquet_chi.f90:103-110): the device state for the next iteration
+__global__ __launch_bounds__(256) void sigma_commit(const double2* __restrict__ src, double2* __restrict__ sigma,
+                                                    double2* __restrict__ sigh, int nx, int nf, int sigh_default) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nt = (size_t)nx * nx * nf;
+  if (i >= nt) return;
+  const double2 v = src[i];
+  sigma[i] = v;
+  if (sigh_default && (int)(i % nf) == nf - 1) {
+    const size_t k = i / nf;  // (kx - 1) Ny + ky - 1
+    const size_t kx = k / nx, ky = k % nx;
+    sigh[kx + (size_t)nx * ky] = v;
+  }
+}
+
+int launch_sigma_commit(tups_ctx* c, int f_range) {
+  const tups_sizes& s = c->sz;
+  const size_t nt = (size_t)s.nx * s.ny * s.nf;
+  hipLaunchKernelGGL(sigma_commit, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream,
+                     sde_output(c, f_range), c->d_sigma, c->d_sigh, s.nx, s.nf, c->sigh_default ? 1 : 0);
+  TUPS_HIP(hipGetLastError());
+  return TUPS_OK;
+}
+
+int launch_sde(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain, int f_range, double u,
+               int use_u2, double* sigma_out) {
+  const tups_sizes& s = c->sz;
+  TUPS_CHECK(sigma_out, TUPS_EINVAL, "null output");
+  if (int rc = sde_reserve(c, f_range)) return rc;
+  GreenParams g{};
+  if (int rc = upload_sigma(c, sigma, sigma_h, &g, mu, ngrain)) return rc;
+  if (int rc = launch_sde_device(c, g, f_range, u, use_u2)) return rc;
+  const size_t nt = (size_t)s.nx * s.ny * s.nf;
+  TUPS_HIP(hipMemcpyAsync(sigma_out, sde_output(c, f_range), nt * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
   TUPS_HIP(hipStreamSynchronize(c->stream));
   return TUPS_OK;
 }

@@ -213,6 +213,32 @@ module tups_gpu_iface
       integer(c_int) :: rc
     end function tups_sde
 
+    function tups_set_sigma(ctx, sigma, sigma_h, mu, ngrain) bind(C, name="tups_set_sigma") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: sigma(*)
+      type(c_ptr), value :: sigma_h
+      real(c_double), value :: mu
+      integer(c_int32_t), value :: ngrain
+      integer(c_int) :: rc
+    end function tups_set_sigma
+
+    function tups_get_sigma(ctx, sigma) bind(C, name="tups_get_sigma") result(rc)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: sigma(*)
+      integer(c_int) :: rc
+    end function tups_get_sigma
+
+    function tups_iteration_full(ctx, mode, use_graph, f_range, u, use_u2) &
+        bind(C, name="tups_iteration_full") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode, use_graph, f_range, use_u2
+      real(c_double), value :: u
+      integer(c_int) :: rc
+    end function tups_iteration_full
+
     function tups_iteration(ctx, mode, use_graph) bind(C, name="tups_iteration") result(rc)
       import :: c_ptr, c_int32_t, c_int
       type(c_ptr), value :: ctx

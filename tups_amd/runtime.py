@@ -70,6 +70,9 @@ SIGNATURES = {
     "tups_set_split": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_calc_chi": ([_V, _D, _D, C.c_double, C.c_int32], C.c_int),
     "tups_sde": ([_V, _D, _D, C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_int32, _D], C.c_int),
+    "tups_set_sigma": ([_V, _D, _D, C.c_double, C.c_int32], C.c_int),
+    "tups_get_sigma": ([_V, _D], C.c_int),
+    "tups_iteration_full": ([_V, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32], C.c_int),
     "tups_iteration": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_batch_create": ([C.c_int32, C.c_int32, _I, _I, C.c_double, C.c_double, _P(_V)], C.c_int),
     "tups_batch_destroy": ([_V], C.c_int),
@@ -265,6 +268,28 @@ class TupsContext:
         _check(self.lib.tups_sde(self._h, _dptr(s), _dptr(sh) if sh is not None else None, float(mu), int(ngrain),
                                  int(f_range), float(U), 1 if use_u2 else 0, _dptr(out)), "tups_sde")
         return out
+
+    def set_sigma(self, sigma, mu: float, ngrain: int = 1, sigma_h=None):
+        """Sigma, Sigma_H (None: Sigma(k, Nf)), mu and ngrain on the device for iteration_full."""
+        s = np.ascontiguousarray(np.asarray(sigma, dtype=np.complex128).ravel())
+        assert s.size == self.nx * self.nx * self.nf
+        sh = None
+        if sigma_h is not None:
+            sh = np.asfortranarray(np.asarray(sigma_h, dtype=np.complex128).reshape(self.nx, self.nx))
+        _check(self.lib.tups_set_sigma(self._h, _dptr(s), _dptr(sh) if sh is not None else None, float(mu),
+                                       int(ngrain)), "tups_set_sigma")
+
+    def get_sigma(self) -> np.ndarray:
+        out = np.zeros(self.nx * self.nx * self.nf, dtype=np.complex128)
+        _check(self.lib.tups_get_sigma(self._h, _dptr(out)), "tups_get_sigma")
+        return out
+
+    def iteration_full(self, U: float, mode: int = BSE_LADDER_FP64, graph: bool = True, f_range: int = 0,
+                       use_u2: bool = False):
+        """calc_chi -> BSE -> calc_PhiR -> calculate_PAE -> SDE -> Sigma update -> G = F - G
+        on the device (SRC/loop.f90:67-222 without the mu bisection)."""
+        _check(self.lib.tups_iteration_full(self._h, mode, 1 if graph else 0, int(f_range), float(U),
+                                            1 if use_u2 else 0), "tups_iteration_full")
 
     def iteration(self, mode: int = BSE_LADDER_FP64, graph: bool = True):
         """BSE -> calc_PhiR -> calculate_PAE -> G = F - G (SRC/loop.f90:78-139,219-222);
