@@ -87,9 +87,57 @@ def cpu_baseline(nx, nf, nl, budget_s):
         el = time.perf_counter() - t0
         if el >= budget_s:
             break
-    return {"value": done / el, "unit": "q-points/s", "cores": int(threads), "kind": "port",
-            "sample": f"{done} q-points of the D={D} ladder (4 channels x 4 ZGEMM + mixing each, "
-                      f"oracle restatement of SRC/parquet_BSE.f90:62-132 on host BLAS), {el:.1f} s"}
+    out = {"value": done / el, "unit": "q-points/s", "cores": int(threads), "kind": "port",
+           "sample": f"{done} q-points of the D={D} ladder (4 channels x 4 ZGEMM + mixing each, "
+                     f"oracle restatement of SRC/parquet_BSE.f90:62-132 on host BLAS), {el:.1f} s"}
+    out["config1_iteration"] = config1_cpu()
+    return out
+
+
+def config1_inputs(lat_or_ctx_nq, nz, nf, nw, seed=7):
+    """Vertices, Lambda (parquet approximation, U = 2) and Sigma = 0 of the 4x4 / Nf=16 / Nl=1 case."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    mk = lambda d: np.asfortranarray(0.05 * (rng.standard_normal((nz, nz, lat_or_ctx_nq))
+                                             + 1j * rng.standard_normal((nz, nz, lat_or_ctx_nq))) + d * np.eye(nz)[:, :, None])
+    gam = [mk(0.5) for _ in range(4)]
+    fv = [mk(0.5) for _ in range(4)]
+    U = 2.0
+    lam = [np.full((nf, nf, nw), v, dtype=np.complex128, order="F") for v in (U, -U, 2.0 * U, 0.0)]
+    return gam, fv, lam, np.zeros(4 * 4 * nf, dtype=np.complex128)
+
+
+def config1_cpu():
+    """One full iteration of the 4x4 / Nf=16 / Nl=1 / beta=2 / U=2 case on the oracle:
+    calc_chi (every q) -> BSE -> calc_PhiR -> calculate_PAE -> SDE (no use_U2 term) -> G = F - G
+    (SRC/loop.f90:67-222 without the mu bisection; M tensors built beforehand, not timed)."""
+    import numpy as np
+
+    from oracle import bse as B
+    from oracle import parquet as P
+    from oracle import sde as S
+    from oracle.lattice import Lattice
+
+    lat = Lattice(4, 16, 1)
+    gam, fv, lam, sigma = config1_inputs(lat.nq, lat.nz, lat.nf, lat.nw)
+    sh = np.zeros((4, 4), dtype=np.complex128)
+    mt = P.calc_m_tensors(lat)
+    t0 = time.perf_counter()
+    ph = np.zeros((lat.nz, lat.nz, lat.nq), dtype=np.complex128, order="F")
+    pp = np.zeros_like(ph)
+    for q in range(lat.nq):
+        ph[:, :, q], pp[:, :, q] = B.calc_chi(lat, int(lat.qx[q]), int(lat.qy[q]), int(lat.qw[q]), sigma, 0.0, 2.0,
+                                              sigma_h=sh)
+    phi = B.bse_ladder(lat, gam, fv, (ph, pp), 2.0, 0.5)
+    phir = [P.calc_phir(lat, p) for p in phi]
+    F = P.calculate_pae(lat, phi, lam, phir, mt)
+    S.calc_sde(lat, phi, (lam[0], lam[1]), sigma, sh, 0.0, 2.0, 2.0)
+    [B.gamma_update(f, p) for f, p in zip(F, phi)]
+    el = time.perf_counter() - t0
+    return {"ms_per_iteration": round(el * 1e3, 1), "cores": 1, "kind": "port",
+            "workload": "one full iteration, Nx=4 Nf=16 Nl=1 Nq=48, beta=2, U=2: calc_chi, BSE, calc_PhiR, "
+                        "calculate_PAE, SDE (no use_U2 term), G = F - G; oracle restatement in numpy"}
 
 
 def large_share(moduli):
@@ -353,6 +401,30 @@ def main():
     print(f"[bench] sde: {sde_ms / sde_n:.2f} ms per call", file=sys.stderr, flush=True)
     sc.close()
 
+    # the reference's CPU case (4x4, Nf=16, Nl=1) as one full iteration on the device, graph replayed
+    from tups_amd import F as FV
+    c1 = TupsContext(nx=4, nf=16, nl=1, beta=2.0, f_damping=0.5)
+    gam1, fv1, lam1, sig1 = config1_inputs(c1.nq, c1.nz, c1.nf, c1.nw)
+    for r in range(4):
+        c1.upload(GAMMA, r, gam1[r])
+        c1.upload(FV, r, fv1[r])
+        c1.upload(LAMBDA, r, lam1[r])
+    c1.calc_M()
+    c1.set_sigma(sig1, 0.0, sigma_h=np.zeros((4, 4), dtype=np.complex128))
+    c1.iteration_full(U, graph=True)
+    c1.sync()
+    c1.record(0)
+    for _ in range(20):
+        c1.iteration_full(U, graph=True)
+    c1.record(1)
+    c1.sync()
+    c1_ms = c1.elapsed_ms(0, 1) / 20
+    config1 = {"gpu_ms_per_iteration": round(c1_ms, 4), "iterations": 20,
+               "workload": "one full iteration, Nx=4 Nf=16 Nl=1 Nq=48, beta=2, U=2, replayed as one HIP graph: "
+                           "calc_chi, BSE, calc_PhiR, calculate_PAE, SDE (no use_U2 term), Sigma update, G = F - G"}
+    print(f"[bench] config-1 iteration: {c1_ms:.3f} ms", file=sys.stderr, flush=True)
+    c1.close()
+
     large = None if args.no_large else large_share(args.moduli)
 
     cpu = None
@@ -388,6 +460,7 @@ def main():
         "batching": batching,
         "large_share": large,
         "sde": sde,
+        "config1_iteration": config1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
