@@ -50,6 +50,7 @@ extern "C" {
 #define TUPS_ENOMEM -3    /* device allocation failed */
 #define TUPS_ESTATE -4    /* call order violated (e.g. PAE before M) */
 #define TUPS_ENONFINITE -5/* non-finite value found by tups_check_finite */
+#define TUPS_ENOCONV -6   /* the mu bisection did not reach |n - nParticle| <= 1e-6 in 64 steps */
 
 /* channel numbering of the reference: d, m, s, t (SRC/parquet_BSE.f90:69-128) */
 #define TUPS_CH_D 0
@@ -142,15 +143,16 @@ int tups_bse(tups_ctx* ctx, int32_t mode);
  * coarse-graining Ngrain.  Synchronous w.r.t. the host buffers. */
 int tups_calc_chi(tups_ctx* ctx, const double* sigma, const double* sigma_h, double mu, int32_t ngrain);
 
-/* calc_SDE (SRC/parquet_SDE.f90:27-258) without its use_U2 second-order term
- * calc_order_2nd (:263-340): the self-energy over the whole zone from the
- * resident reducible vertices G_r_LL (the GAMMA arrays, holding Phi after the
- * crossing as at SRC/loop.f90:154) and Lambda (L_d, L_m), in FP64.  sigma,
- * sigma_h, mu, ngrain as for tups_calc_chi (they build G); f_range the
+/* calc_SDE (SRC/parquet_SDE.f90:27-258): the self-energy over the whole zone
+ * from the resident reducible vertices G_r_LL (the GAMMA arrays, holding Phi
+ * after the crossing as at SRC/loop.f90:154) and Lambda (L_d, L_m), in FP64.
+ * sigma, sigma_h, mu, ngrain as for tups_calc_chi (they build G); f_range the
  * enlarged frequency box of the auxiliary bubbles (0..8); u the Hubbard U;
- * use_u2 selects the reference's use_U2 form of LChi (the caller then adds
- * the second-order term).  sigma_out receives Sigma (Nx Ny Nf complex, the
- * reference's Sigma layout).  Needs a full-IBZ context. */
+ * use_u2 selects the reference's use_U2 form of LChi and adds the second-order
+ * term calc_order_2nd (:263-340) from the Grt of the last tups_gkw_grt (the
+ * reference passes calc_Gkw_Grt's Grt to calc_SDE; TUPS_ESTATE without one for
+ * this ngrain).  sigma_out receives Sigma (Nx Ny Nf complex, the reference's
+ * Sigma layout).  Needs a full-IBZ context. */
 int tups_sde(tups_ctx* ctx, const double* sigma, const double* sigma_h, double mu, int32_t ngrain, int32_t f_range,
              double u, int32_t use_u2, double* sigma_out);
 
@@ -175,13 +177,36 @@ int tups_iteration(tups_ctx* ctx, int32_t mode, int32_t use_graph);
  * tups_iteration_full; tups_get_sigma downloads the current Sigma. */
 int tups_set_sigma(tups_ctx* ctx, const double* sigma, const double* sigma_h, double mu, int32_t ngrain);
 int tups_get_sigma(tups_ctx* ctx, double* sigma);
+/* calc_Gkw_Grt (SRC/parquet_chi.f90:18-210) from the Sigma on the device
+ * (tups_set_sigma, tups_calc_chi or tups_sde): Sigma_H(k) <- Sigma(k, Nf) on the
+ * device; mu bisected from *mu_inout between mu - u and mu + u until the
+ * particle number n = -2 Grt(r = 0, tau = beta) is within 1e-6 of n_particle
+ * (cal_chi != 0: one evaluation at the midpoint and no search, as the
+ * reference's early exit); G(k, tau) on the Nx Ngrain x Ny Ngrain grid and its
+ * 2D transform Grt.  *mu_inout receives the new mu, which also becomes the
+ * device mu; steps_out (may be NULL) the number of evaluations; grt_out (may
+ * be NULL) Grt(Nx Ngrain, Ny Ngrain, Nf) complex, column-major.  Grt stays on
+ * the device for tups_sde's use_u2 term.  TUPS_ENOCONV when 64 evaluations do
+ * not reach the tolerance (the reference loops on), TUPS_ENONFINITE when n is
+ * not finite.  Needs Nx = Ny, 6 <= Nf <= 1024, Nx Ngrain <= 1024.
+ * Synchronous. */
+int tups_gkw_grt(tups_ctx* ctx, double u, double n_particle, int32_t cal_chi, double* mu_inout, int32_t* steps_out,
+                 double* grt_out);
+/* The device mu (no counterpart: the reference keeps mu in a module variable)
+ * and the number of evaluations of the last search; returns the search's
+ * failure, if any, as tups_gkw_grt would.  Synchronous. */
+int tups_get_mu(tups_ctx* ctx, double* mu, int32_t* steps);
 /* The loop body of SRC/loop.f90:67-222 on the device (no counterpart as one
- * call): calc_chi from the device Sigma -> BSE(mode) -> calc_PhiR ->
- * calculate_PAE -> SDE (tups_sde's arguments f_range, u, use_u2) -> the new
- * Sigma replaces the old -> G = F - G.  mu stays as set (the reference's
- * bisection in calc_Gkw_Grt is not part of it).  use_graph != 0 captures the
- * sequence once per (mode, buffer parity, SDE settings) as a HIP graph. */
-int tups_iteration_full(tups_ctx* ctx, int32_t mode, int32_t use_graph, int32_t f_range, double u, int32_t use_u2);
+ * call): calc_Gkw_Grt's mu search when n_particle >= 0 (else mu stays as set)
+ * -> calc_chi from the device Sigma -> BSE(mode) -> calc_PhiR -> calculate_PAE
+ * -> SDE (tups_sde's arguments f_range, u, use_u2; with use_u2 the
+ * second-order term from this iteration's Grt) -> the new Sigma replaces the
+ * old -> G = F - G.  use_graph != 0 captures the sequence once per (mode,
+ * buffer parity, SDE and search settings, Ngrain) as a HIP graph; the search
+ * then runs a fixed number of launches, so a failed search is reported by
+ * tups_get_mu. */
+int tups_iteration_full(tups_ctx* ctx, int32_t mode, int32_t use_graph, int32_t f_range, double u, int32_t use_u2,
+                        double n_particle);
 
 /* --- batching form (no counterpart in the reference, which fixes Nf and Nl for
  * every q): a batch of nq transfer points, each with its own even Nf_q and
@@ -247,7 +272,8 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 #define TUPS_K_CALC_CHI 9     /* bubbles chi_ph, chi_pp for every q slot */
 #define TUPS_K_BSE_BATCH 10   /* batching form: variant names only (tups_variant_name/count) */
 #define TUPS_K_SDE 11         /* self-energy: auxiliary bubbles, vertex contraction, star sums */
-#define TUPS_K_COUNT 12
+#define TUPS_K_GTAU 12        /* G(tau): mu bisection, fine-grid G(k, tau), 2D transform */
+#define TUPS_K_COUNT 13
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);
 /* scans the resident array for NaN/Inf; returns TUPS_ENONFINITE if any */

@@ -51,29 +51,27 @@ def fine_index(lat: Lattice, ngrain: int, kx: int, ky: int, gx: int, gy: int):
 
 
 def g_tau_k(lat: Lattice, sigma, mu, beta, ngrain=1):
-    """G(k_fine, tau) before the FFT: (Nx Ng, Ny Ng, Nf) real, SRC/parquet_chi.f90:124-170."""
+    """G(k_fine, tau) before the FFT: (Nx Ng, Ny Ng, Nf) real, SRC/parquet_chi.f90:124-170
+    (the nu and tau loops as numpy vectors)."""
     nf = lat.nf
     ekg = ek_grain(lat, ngrain)
     out = np.zeros((lat.nx * ngrain, lat.ny * ngrain, nf))
     tau = beta / (nf - 1) * np.arange(nf)
+    nus = np.arange(1, nf + 1)
+    z = np.array([matsubara(int(nu), nf, beta) for nu in nus])
+    # e^{-i pi/beta (2 (nu - Nf/2) - 1) tau}: rows tau, columns nu
+    ph = -math.pi / beta * (2.0 * (nus[None, :] - nf // 2) - 1.0) * tau[:, None]
+    phase = np.cos(ph) + 1j * np.sin(ph)
     for ky in range(1, lat.ny + 1):
         for kx in range(1, lat.nx + 1):
+            s = sigma[((kx - 1) * lat.ny + ky - 1) * nf:((kx - 1) * lat.ny + ky) * nf]
             for gy in range(1, ngrain + 1):
                 for gx in range(1, ngrain + 1):
                     i1, i2 = fine_index(lat, ngrain, kx, ky, gx, gy)
                     e = ekg[gx - 1, gy - 1, kx - 1, ky - 1]
-                    d = np.zeros(nf, dtype=np.complex128)
-                    for nu in range(1, nf + 1):
-                        z = matsubara(nu, nf, beta)
-                        s = sigma[((kx - 1) * lat.ny + ky - 1) * nf + nu - 1]
-                        d[nu - 1] = 1.0 / (z + mu - e - s) - 1.0 / (z + mu - e)
-                    for it in range(nf):
-                        acc = 0.0
-                        for nu in range(1, nf + 1):
-                            ph = -math.pi / beta * (2.0 * (nu - nf // 2) - 1.0) * tau[it]
-                            acc += (complex(math.cos(ph), math.sin(ph)) * d[nu - 1]).real / beta
-                        out[i1 - 1, i2 - 1, it] = acc - math.exp((beta - tau[it]) * (e - mu)) / (
-                            1.0 + math.exp(beta * (e - mu)))
+                    d = 1.0 / (z + mu - e - s) - 1.0 / (z + mu - e)
+                    acc = (phase * d[None, :]).real.sum(axis=1) / beta
+                    out[i1 - 1, i2 - 1, :] = acc - np.exp((beta - tau) * (e - mu)) / (1.0 + math.exp(beta * (e - mu)))
     return out
 
 

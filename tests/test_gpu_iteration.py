@@ -117,3 +117,48 @@ def test_full_iteration_needs_sigma():
         _setup(c, 3)
         with pytest.raises(TupsError):
             c.iteration_full(2.0, graph=True)
+
+
+@pytest.mark.parametrize("nx,nf,nl,split,ngrain", [(4, 8, 1, False, 1), (4, 8, 5, False, 2), (6, 8, 1, True, 1)])
+def test_full_iteration_with_mu_search_equals_steps(nx, nf, nl, split, ngrain):
+    """The whole loop body of SRC/loop.f90:67-222: calc_Gkw_Grt's mu bisection ->
+    calc_chi -> BSE -> PhiR -> PAE -> SDE with use_U2 (its second-order term from
+    this iteration's Grt) -> Sigma -> G = F - G, as a graph and eagerly, against the
+    library calls made one by one with Sigma and mu passed through the host."""
+    from tups_amd import BSE_LADDER_FP64, BSE_LADDER_SPLIT, F, GAMMA, PHIR, TupsContext
+
+    mode = BSE_LADDER_SPLIT if split else BSE_LADDER_FP64
+    U, mu0, n_target = 2.0, 0.4, 0.95
+    _say(f"full iteration with mu search nx={nx} nf={nf} nl={nl} mode={mode} ngrain={ngrain}")
+    rng = np.random.default_rng(41)
+    sigma0 = crandn(rng, (nx * nx * nf,), 0.1).ravel()
+    with TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as a, \
+            TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as b, \
+            TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as e:
+        for c in (a, b, e):
+            _setup(c, 31)
+        b.set_sigma(sigma0, mu0, ngrain=ngrain)
+        e.set_sigma(sigma0, mu0, ngrain=ngrain)
+        sigma, mu = sigma0.copy(), mu0
+        for it in range(3):
+            a.set_sigma(sigma, mu, ngrain=ngrain)
+            mu, steps, _ = a.gkw_grt(U, n_target, mu)
+            a.calc_chi(sigma, mu, ngrain=ngrain)
+            a.BSE(it, mode=mode)
+            a.calc_PhiR(it)
+            a.calculate_PAE()
+            new_sigma = a.sde(sigma, mu, U, ngrain=ngrain, use_u2=True)
+            a.gamma_update()
+            sigma = new_sigma
+            b.iteration_full(U, mode, graph=True, use_u2=True, n_particle=n_target)
+            e.iteration_full(U, mode, graph=False, use_u2=True, n_particle=n_target)
+            for x in (b, e):
+                assert x.get_mu() == (mu, steps), it
+                assert np.array_equal(sigma, x.get_sigma()), it
+            for which in (GAMMA, F, PHIR):
+                for r in range(4):
+                    x = a.download(which, r)
+                    assert np.array_equal(x, b.download(which, r)), (it, which, r)
+                    assert np.array_equal(x, e.download(which, r)), (it, which, r)
+            assert np.isfinite(sigma).all()
+            _say(f"  it {it}: mu {mu:.6f} after {steps} evaluations")

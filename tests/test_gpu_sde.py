@@ -1,6 +1,7 @@
 """GPU parity of the self-energy (tups_sde) against the oracle.
 
-calc_SDE (SRC/parquet_SDE.f90:27-258) without calc_order_2nd: the oracle
+calc_SDE (SRC/parquet_SDE.f90:27-258; with use_U2 also calc_order_2nd from the
+device Grt, whose own parity is in test_gpu_gtau.py): the oracle
 (oracle/sde.py) is pinned by the brute-force second-order diagram and the
 full-zone star sum; here the GPU kernels must reproduce it on random vertices,
 Lambda and Sigma at 1e-12 relative Frobenius error, across Nl, the enlarged
@@ -11,6 +12,7 @@ import sys
 import numpy as np
 import pytest
 
+from oracle import gtau as T
 from oracle import sde as S
 from oracle.lattice import Lattice
 from tests.helpers import crandn, rel_fro
@@ -46,13 +48,18 @@ def test_sde_parity(nx, nf, nl, f_range, ngrain, use_u2, with_sh):
         for r in range(4):
             c.upload(GAMMA, r, gv[r])
             c.upload(LAMBDA, r, lam[r])
+        u2 = 0.0
+        if use_u2:  # Grt of calc_Gkw_Grt at this mu: one evaluation at 0.5 ((mu + U) + (mu - U))
+            c.set_sigma(sigma, mu, ngrain=ngrain)
+            mu_g, _, _ = c.gkw_grt(U, 1.0, mu, cal_chi=True)
+            u2 = T.order_2nd(lat, T.grt(lat, sigma, mu_g, BETA, ngrain), U, BETA, ngrain)
         got = c.sde(sigma, mu, U, ngrain=ngrain, f_range=f_range, use_u2=use_u2, sigma_h=sigma_h)
         again = c.sde(sigma, mu, U, ngrain=ngrain, f_range=f_range, use_u2=use_u2, sigma_h=sigma_h)
     sh = sigma_h
     if sh is None:  # the default rule Sigma_H(k) = Sigma(k, Nf)
         sh = sigma.reshape(nx, nx, nf)[:, :, nf - 1].copy()
     exp = S.calc_sde(lat, gv, (lam[0], lam[1]), sigma, sh, mu, BETA, U, ngrain=ngrain, f_range=f_range,
-                     use_u2=use_u2)
+                     use_u2=use_u2) + u2
     err = rel_fro(got, exp)
     _say(f"  rel err {err:.2e}")
     assert err < TOL

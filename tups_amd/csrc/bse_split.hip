@@ -805,6 +805,7 @@ int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
   const int qbatch = (int)std::max<size_t>(1, std::min<size_t>(s.nq, want));
   const size_t need = per_q * qbatch;
   if (c->split_ws_bytes < need) {
+    drop_graphs(c);  // captured graphs hold the old workspace
     if (c->split_ws) TUPS_HIP(hipFree(c->split_ws));
     c->split_ws = nullptr;
     c->split_ws_bytes = 0;

@@ -34,6 +34,7 @@ struct ChiArgs {
 // grid: nq; dynamic LDS 2 * chunk * Nc * 16 bytes
 __global__ __launch_bounds__(256) void chi_kernel(ChiArgs a) {
   extern __shared__ double2 ck[];  // [2][chunk][Nc]
+  const GreenParams g = green_resolve(a.g);
   const int q = blockIdx.x;
   const int nx = a.nx, nc = nx * nx, nf = a.nf, nl = a.nl, ng = a.ngrain;
   const int m = a.qm[q];
@@ -53,11 +54,11 @@ __global__ __launch_bounds__(256) void chi_kernel(ChiArgs a) {
       double2 sph = make_double2(0.0, 0.0), spp = make_double2(0.0, 0.0);
       for (int gx = 1; gx <= ng; ++gx)
         for (int gy = 1; gy <= ng; ++gy) {
-          const double2 gk = green_coarse(a.g, gx, gy, kx, ky, nu);
-          const double2 p = cmul(gk, green_coarse(a.g, gx, gy, kqx, kqy, nu_kq));
+          const double2 gk = green_coarse(g, gx, gy, kx, ky, nu);
+          const double2 p = cmul(gk, green_coarse(g, gx, gy, kqx, kqy, nu_kq));
           sph.x += p.x * inv_g2;
           sph.y += p.y * inv_g2;
-          const double2 t = cmul(gk, green_coarse(a.g, ng - gx + 1, ng - gy + 1, qmkx, qmky, nu_qmk));
+          const double2 t = cmul(gk, green_coarse(g, ng - gx + 1, ng - gy + 1, qmkx, qmky, nu_qmk));
           spp.x += t.x * inv_g2;
           spp.y += t.y * inv_g2;
         }
@@ -111,6 +112,8 @@ int upload_sigma(tups_ctx* c, const double* sigma, const double* sigma_h, GreenP
   }
   TUPS_HIP(hipMemcpyAsync(c->d_sigma, sigma, nt * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   TUPS_HIP(hipMemcpyAsync(c->d_sigh, sh.data(), (size_t)nc * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  if (!c->d_gstate) TUPS_HIP(hipMalloc((void**)&c->d_gstate, sizeof(GtauState)));
+  TUPS_HIP(hipMemcpyAsync(&c->d_gstate->mu, &mu, sizeof(double), hipMemcpyHostToDevice, c->stream));
   TUPS_HIP(hipStreamSynchronize(c->stream));
   g->sigma = c->d_sigma;
   g->sigh = c->d_sigh;
@@ -119,6 +122,7 @@ int upload_sigma(tups_ctx* c, const double* sigma, const double* sigma_h, GreenP
   g->ngrain = ngrain;
   g->mu = mu;
   g->beta = c->beta;
+  g->mu_dev = nullptr;
   c->green_mu = mu;
   c->green_ngrain = ngrain;
   c->sigh_default = sigma_h == nullptr;

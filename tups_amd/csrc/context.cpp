@@ -291,6 +291,8 @@ int tups_destroy(tups_ctx* c) {
   f(c->chi[1]);
   f(c->split_ws);
   f(c->sde_ws);
+  f(c->gt_ws);
+  f(c->d_gstate);
   f(c->closed_ws);
   f(c->d_sigma);
   f(c->d_sigh);

@@ -15,7 +15,15 @@ struct GreenParams {
   const double2* sigh;   // Sigma_H(Nx, Ny), column-major
   int nx, nf, ngrain;
   double mu, beta;
+  const double* mu_dev;  // non-null: mu lives on the device (the full iteration's search)
 };
+
+// the GreenParams a kernel works with: mu read once from the device when it lives there
+__device__ __forceinline__ GreenParams green_resolve(const GreenParams& a) {
+  GreenParams g = a;
+  if (g.mu_dev) g.mu = *g.mu_dev;
+  return g;
+}
 
 __device__ __forceinline__ double2 green_at(const GreenParams& a, double ek, int kx, int ky, int nu) {
   const double pi = 3.14159265358979323846;

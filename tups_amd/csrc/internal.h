@@ -62,6 +62,15 @@ struct PhirTable {
 };
 void build_phir_table(const Tables& t, PhirTable& out);
 
+// device state of the chemical-potential search of calc_Gkw_Grt: the current mu,
+// the bracket, the last particle number and the candidates of the current pass
+struct GtauState {
+  double mu, hi, lo, t1, n_target;
+  int steps, done, status, single;  // status 0 ok, 1 no convergence, 2 non-finite n
+  double cand[63];
+  double t1c[63];
+};
+
 }  // namespace tups
 
 struct tups_ctx {
@@ -111,6 +120,13 @@ struct tups_ctx {
   bool sigh_default = true;      // Sigma_H follows Sigma(k, Nf) when Sigma is replaced on the device
   double green_mu = 0.0;
   int green_ngrain = 1;
+  tups::GtauState* d_gstate = nullptr;  // device mu and its search state
+  void* gt_ws = nullptr;         // G(tau): energies, partial sums, G(k, tau), transforms, Grt
+  size_t gt_ws_bytes = 0;
+  int gt_ngrain = 0;             // Ngrain the G(tau) workspace is laid out for
+  bool grt_valid = false;        // Grt of the last calc_Gkw_Grt is in the workspace
+  double full_n_particle = -1.0; // full-iteration graphs: nParticle and Ngrain they were captured with
+  int full_ngrain = 0;
   hipGraphExec_t graph_full[4] = {};  // full iteration, per (ladder mode, buffer parity)
   int full_f_range = -1, full_use_u2 = 0;  // settings the full-iteration graphs were captured with
   double full_u = 0.0;
@@ -149,6 +165,11 @@ int sde_reserve(tups_ctx* c, int f_range);
 int launch_sde_device(tups_ctx* c, const GreenParams& g, int f_range, double u, int use_u2);
 int launch_sigma_commit(tups_ctx* c, int f_range);
 int green_params(const tups_ctx* c, GreenParams* g);
+int gtau_reserve(tups_ctx* c, int ngrain);
+int launch_gtau(tups_ctx* c, int mode, int mu_from_host, double mu_host, double u, double n_target, int set_hartree);
+int launch_order_2nd(tups_ctx* c, double u, double2* sigma_out);
+const double2* gtau_grt(const tups_ctx* c);
+double2* sde_output(tups_ctx* c, int f_range);
 int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms);
 const char* bse_variant_name(int v);
 int bse_variant_count();

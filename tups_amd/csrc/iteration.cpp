@@ -11,6 +11,8 @@
 #include "green.h"
 #include "internal.h"
 
+#include <cmath>
+
 using namespace tups;
 
 namespace tups {
@@ -40,6 +42,7 @@ int green_params(const tups_ctx* c, GreenParams* g) {
   g->ngrain = c->green_ngrain;
   g->mu = c->green_mu;
   g->beta = c->beta;
+  g->mu_dev = &c->d_gstate->mu;
   return TUPS_OK;
 }
 
@@ -53,13 +56,21 @@ static int iteration_sequence(tups_ctx* c, int mode, bool* swapped) {
   return launch_gamma_update(c);
 }
 
-// calc_chi -> BSE -> calc_PhiR -> calculate_PAE -> SDE -> Sigma handed over ->
-// G = F - G, with Sigma and mu on the device (SRC/loop.f90:67-222 without the
-// mu bisection of calc_Gkw_Grt, the convergence check and the output)
-static int full_sequence(tups_ctx* c, int mode, int f_range, double u, int use_u2, bool* swapped) {
+// [calc_Gkw_Grt ->] calc_chi -> BSE -> calc_PhiR -> calculate_PAE -> SDE [+ its
+// second-order term] -> Sigma handed over -> G = F - G, with Sigma and mu on the
+// device (SRC/loop.f90:67-222 without the convergence check and the output).
+// n_particle >= 0 bisects mu as calc_Gkw_Grt does; otherwise mu stays and G(tau)
+// is formed only when use_u2 needs it.
+static int full_sequence(tups_ctx* c, int mode, int f_range, double u, int use_u2, double n_particle,
+                         bool* swapped) {
   GreenParams g{};
   int rc = green_params(c, &g);
   if (rc) return rc;
+  if (n_particle >= 0.0) {
+    if ((rc = launch_gtau(c, 0, 0, 0.0, u, n_particle, 1))) return rc;
+  } else if (use_u2) {
+    if ((rc = launch_gtau(c, 2, 0, 0.0, u, 0.0, 1))) return rc;
+  }
   if ((rc = launch_chi_device(c, g))) return rc;
   if ((rc = mode == TUPS_BSE_LADDER_SPLIT ? launch_bse_split(c) : launch_bse_ladder(c))) return rc;
   swap_vertex_buffers(c);
@@ -67,8 +78,22 @@ static int full_sequence(tups_ctx* c, int mode, int f_range, double u, int use_u
   if ((rc = launch_phir(c))) return rc;
   if ((rc = launch_pae(c))) return rc;
   if ((rc = launch_sde_device(c, g, f_range, u, use_u2))) return rc;
+  if (use_u2 && (rc = launch_order_2nd(c, u, sde_output(c, f_range)))) return rc;
   if ((rc = launch_sigma_commit(c, f_range))) return rc;
   return launch_gamma_update(c);
+}
+
+// the device search state, with its failure turned into a status code
+static int read_gstate(tups_ctx* c, GtauState* st) {
+  TUPS_HIP(hipMemcpyAsync(st, c->d_gstate, sizeof(GtauState), hipMemcpyDeviceToHost, c->stream));
+  TUPS_HIP(hipStreamSynchronize(c->stream));
+  c->green_mu = st->mu;
+  if (st->status == 1) {
+    set_error("mu bisection: |n - nParticle| > 1e-6 after 64 steps (mu between mu -/+ U does not bracket it)");
+    return TUPS_ENOCONV;
+  }
+  TUPS_CHECK(st->status != 2, TUPS_ENONFINITE, "mu bisection: the particle number is not finite");
+  return TUPS_OK;
 }
 
 }  // namespace tups
@@ -88,8 +113,39 @@ extern "C" int tups_get_sigma(tups_ctx* c, double* sigma) {
   return TUPS_OK;
 }
 
+extern "C" int tups_gkw_grt(tups_ctx* c, double u, double n_particle, int32_t cal_chi, double* mu_inout,
+                            int32_t* steps_out, double* grt_out) {
+  TUPS_CHECK(c && mu_inout, TUPS_EINVAL, "null argument");
+  TUPS_CHECK(c->green_set, TUPS_ESTATE, "no Sigma on the device (tups_set_sigma, tups_calc_chi or tups_sde first)");
+  TUPS_CHECK(std::isfinite(u) && std::isfinite(*mu_inout) && std::isfinite(n_particle), TUPS_EINVAL,
+             "u, mu and n_particle must be finite");
+  int rc = gtau_reserve(c, c->green_ngrain);
+  if (rc) return rc;
+  if ((rc = launch_gtau(c, cal_chi ? 1 : 0, 1, *mu_inout, u, n_particle, 1))) return rc;
+  c->sigh_default = true;  // Sigma_H(k) = Sigma(k, Nf) from here on, as in the reference
+  if (grt_out) {
+    const size_t n = (size_t)c->sz.nx * c->gt_ngrain * c->sz.ny * c->gt_ngrain * c->sz.nf;
+    TUPS_HIP(hipMemcpyAsync(grt_out, gtau_grt(c), n * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  }
+  GtauState st{};
+  rc = read_gstate(c, &st);
+  *mu_inout = st.mu;
+  if (steps_out) *steps_out = st.steps;
+  return rc;
+}
+
+extern "C" int tups_get_mu(tups_ctx* c, double* mu, int32_t* steps) {
+  TUPS_CHECK(c && mu, TUPS_EINVAL, "null argument");
+  TUPS_CHECK(c->green_set && c->d_gstate, TUPS_ESTATE, "no mu on the device (tups_set_sigma first)");
+  GtauState st{};
+  const int rc = read_gstate(c, &st);
+  *mu = st.mu;
+  if (steps) *steps = st.steps;
+  return rc;
+}
+
 extern "C" int tups_iteration_full(tups_ctx* c, int32_t mode, int32_t use_graph, int32_t f_range, double u,
-                                   int32_t use_u2) {
+                                   int32_t use_u2, double n_particle) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
   TUPS_CHECK(mode == TUPS_BSE_LADDER_FP64 || mode == TUPS_BSE_LADDER_SPLIT, TUPS_EINVAL,
              "an iteration runs one of the ladder modes");
@@ -100,24 +156,30 @@ extern "C" int tups_iteration_full(tups_ctx* c, int32_t mode, int32_t use_graph,
   if (rc) return rc;
   if (mode == TUPS_BSE_LADDER_SPLIT && (rc = bse_split_reserve(c, nullptr))) return rc;
   if ((rc = sde_reserve(c, f_range))) return rc;
+  if (!(n_particle >= 0.0)) n_particle = -1.0;  // negative or NaN: mu stays
+  const bool gtau = n_particle >= 0.0 || use_u2;
+  if (gtau && (rc = gtau_reserve(c, c->green_ngrain))) return rc;
   bool swapped = false;
   if (!use_graph) {
-    rc = full_sequence(c, mode, f_range, u, use_u2 ? 1 : 0, &swapped);
+    rc = full_sequence(c, mode, f_range, u, use_u2 ? 1 : 0, n_particle, &swapped);
     if (!rc) c->phir_valid = true;
     return rc;
   }
-  if (c->full_f_range != f_range || c->full_u != u || c->full_use_u2 != (use_u2 ? 1 : 0)) {
+  if (c->full_f_range != f_range || c->full_u != u || c->full_use_u2 != (use_u2 ? 1 : 0) ||
+      c->full_n_particle != n_particle || c->full_ngrain != c->green_ngrain) {
     drop_graphs(c);  // the settings are part of the captured launches
     c->full_f_range = f_range;
     c->full_u = u;
     c->full_use_u2 = use_u2 ? 1 : 0;
+    c->full_n_particle = n_particle;
+    c->full_ngrain = c->green_ngrain;
   }
   const int slot = 2 * mode + c->parity;
   if (!c->graph_full[slot]) {
     const bool timing = c->timing;
     c->timing = false;  // no event nodes in the graph
     TUPS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    rc = full_sequence(c, mode, f_range, u, use_u2 ? 1 : 0, &swapped);
+    rc = full_sequence(c, mode, f_range, u, use_u2 ? 1 : 0, n_particle, &swapped);
     hipGraph_t gr = nullptr;
     const hipError_t ec = hipStreamEndCapture(c->stream, &gr);
     c->timing = timing;

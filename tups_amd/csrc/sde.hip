@@ -1,8 +1,8 @@
 // Self-energy from the Schwinger-Dyson equation in form-factor space, FP64.
 //
-// Reference: calc_SDE, SRC/parquet_SDE.f90:27-258, without the use_U2
-// second-order term calc_order_2nd (:263-340, an FFT of G^2 G on the fine
-// grid), with calc_chi_aux (:346-432), calc_LChi (:436-497), calc_PhChi
+// Reference: calc_SDE, SRC/parquet_SDE.f90:27-258 (its use_U2 second-order term
+// calc_order_2nd, :263-340, is added from the device Grt by gtau.hip), with
+// calc_chi_aux (:346-432), calc_LChi (:436-497), calc_PhChi
 // (:501-537), calc_PPChi (:540-575) and symmetrize_array
 // (SRC/parquet_util.f90:1215-1272).  Three kernels:
 //  1. sde_chi_aux: per IBZ q, the auxiliary bubbles
@@ -75,10 +75,11 @@ __device__ __forceinline__ void sym_apply(int s, int ix, int iy, int nx, int& ox
 
 // grid: nq; one thread per (l4, nu) entry of the enlarged box
 __global__ __launch_bounds__(256) void sde_chi_aux(SdeArgs a) {
+  const GreenParams g = green_resolve(a.g);
   const int q = blockIdx.x;
   const int m = a.qm[q];
   const int qx = a.ibz[m], qy = a.ibz[a.nibz + m], w = a.qw[q];
-  const int nx = a.nx, nf = a.nf, ng = a.g.ngrain;
+  const int nx = a.nx, nf = a.nf, ng = g.ngrain;
   const double inv_g2 = 1.0 / ng / ng;
   for (int e = threadIdx.x; e < a.nl * a.ext; e += blockDim.x) {
     const int l4 = e / a.ext, nu = e - l4 * a.ext - a.f_range * nf + 1;  // nu in (-f_range Nf, (f_range + 1) Nf]
@@ -92,11 +93,11 @@ __global__ __launch_bounds__(256) void sde_chi_aux(SdeArgs a) {
         double2 tph = make_double2(0.0, 0.0), tpp = make_double2(0.0, 0.0);
         for (int gx = 1; gx <= ng; ++gx)
           for (int gy = 1; gy <= ng; ++gy) {
-            const double2 gk = green_coarse(a.g, gx, gy, kx, ky, nu);
-            const double2 p = cmul(gk, green_coarse(a.g, gx, gy, kqx, kqy, nu + w - 1));
+            const double2 gk = green_coarse(g, gx, gy, kx, ky, nu);
+            const double2 p = cmul(gk, green_coarse(g, gx, gy, kqx, kqy, nu + w - 1));
             tph.x += p.x * f * inv_g2;
             tph.y += p.y * f * inv_g2;
-            const double2 t = cmul(gk, green_coarse(a.g, ng - gx + 1, ng - gy + 1, qmkx, qmky, -nu + nf + 1 + w - 1));
+            const double2 t = cmul(gk, green_coarse(g, ng - gx + 1, ng - gy + 1, qmkx, qmky, -nu + nf + 1 + w - 1));
             tpp.x += t.x * f * inv_g2;
             tpp.y += t.y * f * inv_g2;
           }
@@ -155,6 +156,7 @@ __global__ __launch_bounds__(256) void sde_vchi(SdeArgs a) {
 
 // grid: nibz * Nf; one workgroup per IBZ (k, nu), threads over the q slots
 __global__ __launch_bounds__(256) void sde_sigma(SdeArgs a) {
+  const GreenParams g = green_resolve(a.g);
   __shared__ double red[2][256];
   const int nf = a.nf, nx = a.nx, nl = a.nl, nc = nx * nx;
   const int km = blockIdx.x / nf, k0 = blockIdx.x - km * nf + 1;
@@ -172,11 +174,11 @@ __global__ __launch_bounds__(256) void sde_sigma(SdeArgs a) {
       if (!((ops >> (s - 1)) & 1u)) continue;
       int sx, sy;
       sym_apply(s, qx, qy, nx, sx, sy);
-      const double2 gkq = green_plain(a.g, (kx - 1 + sx - 1) % nx + 1, (ky - 1 + sy - 1) % nx + 1, k0 + w - 1);
+      const double2 gkq = green_plain(g, (kx - 1 + sx - 1) % nx + 1, (ky - 1 + sy - 1) % nx + 1, k0 + w - 1);
       const int qkx = (mkx - 1 + sx - 1) % nx + 1, qky = (mky - 1 + sy - 1) % nx + 1, qkw = mkw + w - 1;
-      const double2 gqmk = green_plain(a.g, qkx, qky, qkw);
+      const double2 gqmk = green_plain(g, qkx, qky, qkw);
       const double2 gkmq =
-          green_plain(a.g, (-(qkx - 1) + nx) % nx + 1, (-(qky - 1) + nx) % nx + 1, -qkw + nf + 1);
+          green_plain(g, (-(qkx - 1) + nx) % nx + 1, (-(qky - 1) + nx) % nx + 1, -qkw + nf + 1);
       for (int l3 = 0; l3 < nl; ++l3) {
         const double fi = a.ffi[((size_t)(s - 1) * nl + l3) * nc + kidx];
         const int i3 = l3 * nf + k0 - 1;
@@ -230,6 +232,7 @@ int sde_reserve(tups_ctx* c, int f_range) {
   const size_t naux = (size_t)s.nq * s.nl * ext, nv = (size_t)s.nq * s.nz, nt = (size_t)nc * s.nf;
   const size_t need = (2 * naux + 2 * nv + nt) * sizeof(double2);
   if (c->sde_ws_bytes < need) {
+    drop_graphs(c);  // captured graphs hold the old workspace
     if (c->sde_ws) TUPS_HIP(hipFree(c->sde_ws));
     c->sde_ws = nullptr;
     c->sde_ws_bytes = 0;
@@ -327,6 +330,11 @@ int launch_sde(tups_ctx* c, const double* sigma, const double* sigma_h, double m
   GreenParams g{};
   if (int rc = upload_sigma(c, sigma, sigma_h, &g, mu, ngrain)) return rc;
   if (int rc = launch_sde_device(c, g, f_range, u, use_u2)) return rc;
+  if (use_u2) {
+    TUPS_CHECK(c->grt_valid && c->gt_ngrain == ngrain, TUPS_ESTATE,
+               "use_u2 needs Grt on the device for this ngrain (tups_gkw_grt first)");
+    if (int rc = launch_order_2nd(c, u, sde_output(c, f_range))) return rc;
+  }
   const size_t nt = (size_t)s.nx * s.ny * s.nf;
   TUPS_HIP(hipMemcpyAsync(sigma_out, sde_output(c, f_range), nt * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
   TUPS_HIP(hipStreamSynchronize(c->stream));

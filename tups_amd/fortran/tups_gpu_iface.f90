@@ -11,7 +11,8 @@ module tups_gpu_iface
   implicit none
 
   integer(c_int), parameter :: TUPS_OK = 0, TUPS_EINVAL = -1, TUPS_EHIP = -2, &
-                               TUPS_ENOMEM = -3, TUPS_ESTATE = -4, TUPS_ENONFINITE = -5
+                               TUPS_ENOMEM = -3, TUPS_ESTATE = -4, TUPS_ENONFINITE = -5, &
+                               TUPS_ENOCONV = -6
   integer(c_int32_t), parameter :: TUPS_CH_D = 0, TUPS_CH_M = 1, TUPS_CH_S = 2, TUPS_CH_T = 3
   integer(c_int32_t), parameter :: TUPS_ARR_GAMMA = 0, TUPS_ARR_F = 1, TUPS_ARR_CHI = 2, TUPS_ARR_PHIR = 3, &
                                    TUPS_ARR_M = 4, TUPS_ARR_LAMBDA = 5
@@ -20,7 +21,8 @@ module tups_gpu_iface
   integer(c_int32_t), parameter :: TUPS_K_BSE_LADDER = 0, TUPS_K_PHIR = 1, TUPS_K_BUILD_M = 2, &
                                    TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4, TUPS_K_PAE_GATHER = 5, &
                                    TUPS_K_SPLIT_PREP = 6, TUPS_K_SPLIT_GEMM = 7, TUPS_K_BSE_CLOSED = 8, &
-                                   TUPS_K_CALC_CHI = 9, TUPS_K_BSE_BATCH = 10, TUPS_K_SDE = 11
+                                   TUPS_K_CALC_CHI = 9, TUPS_K_BSE_BATCH = 10, TUPS_K_SDE = 11, &
+                                   TUPS_K_GTAU = 12
 
   type, bind(C) :: tups_dims
     integer(c_int32_t) :: nx, nf, nl, nr, nq, reserved
@@ -230,14 +232,34 @@ module tups_gpu_iface
       integer(c_int) :: rc
     end function tups_get_sigma
 
-    function tups_iteration_full(ctx, mode, use_graph, f_range, u, use_u2) &
+    function tups_iteration_full(ctx, mode, use_graph, f_range, u, use_u2, n_particle) &
         bind(C, name="tups_iteration_full") result(rc)
       import :: c_ptr, c_int32_t, c_int, c_double
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: mode, use_graph, f_range, use_u2
-      real(c_double), value :: u
+      real(c_double), value :: u, n_particle
       integer(c_int) :: rc
     end function tups_iteration_full
+
+    ! calc_Gkw_Grt (SRC/parquet_chi.f90:18): mu in/out, Grt(Nx*Ngrain, Ny*Ngrain, Nf) out
+    function tups_gkw_grt(ctx, u, n_particle, cal_chi, mu, steps, grt) bind(C, name="tups_gkw_grt") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), value :: u, n_particle
+      integer(c_int32_t), value :: cal_chi
+      real(c_double), intent(inout) :: mu
+      integer(c_int32_t), intent(out) :: steps
+      complex(c_double_complex), intent(out) :: grt(*)
+      integer(c_int) :: rc
+    end function tups_gkw_grt
+
+    function tups_get_mu(ctx, mu, steps) bind(C, name="tups_get_mu") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: mu
+      integer(c_int32_t), intent(out) :: steps
+      integer(c_int) :: rc
+    end function tups_get_mu
 
     function tups_iteration(ctx, mode, use_graph) bind(C, name="tups_iteration") result(rc)
       import :: c_ptr, c_int32_t, c_int

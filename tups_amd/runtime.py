@@ -21,14 +21,14 @@ LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
 
 GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
 (K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER, K_SPLIT_PREP, K_SPLIT_GEMM,
- K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH, K_SDE) = range(12)
+ K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH, K_SDE, K_GTAU) = range(13)
 CH_D, CH_M, CH_S, CH_T = range(4)
 BSE_LADDER_FP64 = 0
 BSE_LADDER_SPLIT = 1
 BSE_CLOSED_LU = 2
 M_NAMES = ("phbar_to_ph", "pp_to_ph", "ph_to_pp", "phbar_to_pp")
 
-ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE"}
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE", -6: "ENOCONV"}
 
 
 class TupsError(RuntimeError):
@@ -72,7 +72,9 @@ SIGNATURES = {
     "tups_sde": ([_V, _D, _D, C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_int32, _D], C.c_int),
     "tups_set_sigma": ([_V, _D, _D, C.c_double, C.c_int32], C.c_int),
     "tups_get_sigma": ([_V, _D], C.c_int),
-    "tups_iteration_full": ([_V, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32], C.c_int),
+    "tups_iteration_full": ([_V, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double], C.c_int),
+    "tups_gkw_grt": ([_V, C.c_double, C.c_double, C.c_int32, _D, C.POINTER(C.c_int32), _D], C.c_int),
+    "tups_get_mu": ([_V, _D, C.POINTER(C.c_int32)], C.c_int),
     "tups_iteration": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_batch_create": ([C.c_int32, C.c_int32, _I, _I, C.c_double, C.c_double, _P(_V)], C.c_int),
     "tups_batch_destroy": ([_V], C.c_int),
@@ -171,6 +173,7 @@ class TupsContext:
         _check(self.lib.tups_query(self._h, C.byref(s)), "tups_query")
         self.sizes = s
         self.nx, self.nf, self.nl, self.nz = s.nx, s.nf, s.nl, s.nz
+        self._ngrain = 1
         self.nr, self.nw, self.nibz, self.nq, self.nc = s.nr, s.nw, s.nibz, s.nq, s.nc
         self.alpha = s.alpha
         self.beta = beta
@@ -253,12 +256,13 @@ class TupsContext:
             sh = np.asfortranarray(np.asarray(sigma_h, dtype=np.complex128).reshape(self.nx, self.nx))
         _check(self.lib.tups_calc_chi(self._h, _dptr(s), _dptr(sh) if sh is not None else None, float(mu),
                                       int(ngrain)), "tups_calc_chi")
+        self._ngrain = int(ngrain)
 
     def sde(self, sigma, mu: float, U: float, ngrain: int = 1, f_range: int = 0, use_u2: bool = False,
             sigma_h=None) -> np.ndarray:
-        """calc_SDE without its use_U2 second-order term (SRC/parquet_SDE.f90:27-258):
-        Sigma (Nx*Ny*Nf,) over the whole zone from the resident G_r_LL and Lambda.
-        sigma / sigma_h / mu / ngrain build G as in calc_chi."""
+        """calc_SDE (SRC/parquet_SDE.f90:27-258): Sigma (Nx*Ny*Nf,) over the whole zone
+        from the resident G_r_LL and Lambda.  sigma / sigma_h / mu / ngrain build G as in
+        calc_chi; use_u2 adds calc_order_2nd from the Grt of the last gkw_grt."""
         s = np.ascontiguousarray(np.asarray(sigma, dtype=np.complex128).ravel())
         assert s.size == self.nx * self.nx * self.nf
         sh = None
@@ -267,6 +271,7 @@ class TupsContext:
         out = np.zeros(self.nx * self.nx * self.nf, dtype=np.complex128)
         _check(self.lib.tups_sde(self._h, _dptr(s), _dptr(sh) if sh is not None else None, float(mu), int(ngrain),
                                  int(f_range), float(U), 1 if use_u2 else 0, _dptr(out)), "tups_sde")
+        self._ngrain = int(ngrain)
         return out
 
     def set_sigma(self, sigma, mu: float, ngrain: int = 1, sigma_h=None):
@@ -278,6 +283,7 @@ class TupsContext:
             sh = np.asfortranarray(np.asarray(sigma_h, dtype=np.complex128).reshape(self.nx, self.nx))
         _check(self.lib.tups_set_sigma(self._h, _dptr(s), _dptr(sh) if sh is not None else None, float(mu),
                                        int(ngrain)), "tups_set_sigma")
+        self._ngrain = int(ngrain)
 
     def get_sigma(self) -> np.ndarray:
         out = np.zeros(self.nx * self.nx * self.nf, dtype=np.complex128)
@@ -285,11 +291,32 @@ class TupsContext:
         return out
 
     def iteration_full(self, U: float, mode: int = BSE_LADDER_FP64, graph: bool = True, f_range: int = 0,
-                       use_u2: bool = False):
-        """calc_chi -> BSE -> calc_PhiR -> calculate_PAE -> SDE -> Sigma update -> G = F - G
-        on the device (SRC/loop.f90:67-222 without the mu bisection)."""
+                       use_u2: bool = False, n_particle: float = -1.0):
+        """[calc_Gkw_Grt mu search when n_particle >= 0 ->] calc_chi -> BSE -> calc_PhiR ->
+        calculate_PAE -> SDE [+ second order] -> Sigma update -> G = F - G on the device
+        (SRC/loop.f90:67-222)."""
         _check(self.lib.tups_iteration_full(self._h, mode, 1 if graph else 0, int(f_range), float(U),
-                                            1 if use_u2 else 0), "tups_iteration_full")
+                                            1 if use_u2 else 0, float(n_particle)), "tups_iteration_full")
+
+    def gkw_grt(self, U: float, n_particle: float, mu: float, cal_chi: bool = False, want_grt: bool = True):
+        """calc_Gkw_Grt (SRC/parquet_chi.f90:18-210) from the device Sigma: returns
+        (mu, steps, Grt) with Grt of shape (Nx Ng, Ny Ng, Nf) (None unless want_grt)."""
+        m = C.c_double(float(mu))
+        steps = C.c_int32(0)
+        grt = None
+        if want_grt:
+            n1 = self.nx * self._ngrain
+            grt = np.zeros((n1, n1, self.nf), dtype=np.complex128, order="F")
+        _check(self.lib.tups_gkw_grt(self._h, float(U), float(n_particle), 1 if cal_chi else 0, C.byref(m),
+                                     C.byref(steps), _dptr(grt) if grt is not None else None), "tups_gkw_grt")
+        return m.value, steps.value, grt
+
+    def get_mu(self):
+        """(mu, steps) of the device search state."""
+        m = C.c_double(0.0)
+        steps = C.c_int32(0)
+        _check(self.lib.tups_get_mu(self._h, C.byref(m), C.byref(steps)), "tups_get_mu")
+        return m.value, steps.value
 
     def iteration(self, mode: int = BSE_LADDER_FP64, graph: bool = True):
         """BSE -> calc_PhiR -> calculate_PAE -> G = F - G (SRC/loop.f90:78-139,219-222);
