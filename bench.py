@@ -108,36 +108,43 @@ def config1_inputs(lat_or_ctx_nq, nz, nf, nw, seed=7):
     return gam, fv, lam, np.zeros(4 * 4 * nf, dtype=np.complex128)
 
 
+CONFIG1_WORKLOAD = ("one full iteration, Nx=4 Nf=16 Nl=1 Nq=48, beta=2, U=2, nParticle=1, use_U2: mu bisection "
+                    "and G(r, tau), calc_chi, BSE, calc_PhiR, calculate_PAE, SDE + second-order term, Sigma "
+                    "update, G = F - G")
+
+
 def config1_cpu():
-    """One full iteration of the 4x4 / Nf=16 / Nl=1 / beta=2 / U=2 case on the oracle:
-    calc_chi (every q) -> BSE -> calc_PhiR -> calculate_PAE -> SDE (no use_U2 term) -> G = F - G
-    (SRC/loop.f90:67-222 without the mu bisection; M tensors built beforehand, not timed)."""
+    """One full iteration of the 4x4 / Nf=16 / Nl=1 / beta=2 / U=2 / nParticle=1 / use_U2 case
+    on the oracle: calc_Gkw_Grt (mu bisection, G(r, tau)) -> calc_chi (every q) -> BSE ->
+    calc_PhiR -> calculate_PAE -> SDE + calc_order_2nd -> G = F - G (SRC/loop.f90:67-222;
+    M tensors built beforehand, not timed)."""
     import numpy as np
 
     from oracle import bse as B
+    from oracle import gtau as T
     from oracle import parquet as P
     from oracle import sde as S
     from oracle.lattice import Lattice
 
     lat = Lattice(4, 16, 1)
     gam, fv, lam, sigma = config1_inputs(lat.nq, lat.nz, lat.nf, lat.nw)
-    sh = np.zeros((4, 4), dtype=np.complex128)
     mt = P.calc_m_tensors(lat)
     t0 = time.perf_counter()
+    mu, grt, _ = T.bisect_mu(lat, sigma, 0.0, 2.0, 2.0, 1.0)
+    sh = sigma.reshape(4, 4, lat.nf)[:, :, lat.nf - 1].copy()
     ph = np.zeros((lat.nz, lat.nz, lat.nq), dtype=np.complex128, order="F")
     pp = np.zeros_like(ph)
     for q in range(lat.nq):
-        ph[:, :, q], pp[:, :, q] = B.calc_chi(lat, int(lat.qx[q]), int(lat.qy[q]), int(lat.qw[q]), sigma, 0.0, 2.0,
+        ph[:, :, q], pp[:, :, q] = B.calc_chi(lat, int(lat.qx[q]), int(lat.qy[q]), int(lat.qw[q]), sigma, mu, 2.0,
                                               sigma_h=sh)
     phi = B.bse_ladder(lat, gam, fv, (ph, pp), 2.0, 0.5)
     phir = [P.calc_phir(lat, p) for p in phi]
     F = P.calculate_pae(lat, phi, lam, phir, mt)
-    S.calc_sde(lat, phi, (lam[0], lam[1]), sigma, sh, 0.0, 2.0, 2.0)
+    S.calc_sde(lat, phi, (lam[0], lam[1]), sigma, sh, mu, 2.0, 2.0, use_u2=True) + T.order_2nd(lat, grt, 2.0, 2.0)
     [B.gamma_update(f, p) for f, p in zip(F, phi)]
     el = time.perf_counter() - t0
     return {"ms_per_iteration": round(el * 1e3, 1), "cores": 1, "kind": "port",
-            "workload": "one full iteration, Nx=4 Nf=16 Nl=1 Nq=48, beta=2, U=2: calc_chi, BSE, calc_PhiR, "
-                        "calculate_PAE, SDE (no use_U2 term), G = F - G; oracle restatement in numpy"}
+            "workload": CONFIG1_WORKLOAD + "; oracle restatement in numpy"}
 
 
 def large_share(moduli):
@@ -401,6 +408,24 @@ def main():
     print(f"[bench] sde: {sde_ms / sde_n:.2f} ms per call", file=sys.stderr, flush=True)
     sc.close()
 
+    # calc_Gkw_Grt: mu bisection + G(k, tau) + 2D transform on a coarse-grained grid
+    from tups_amd.runtime import K_GTAU
+    gc = TupsContext(nx=8, nf=64, nl=1, beta=2.0, f_damping=0.5)
+    rng = np.random.default_rng(3)
+    sig_g = 0.2 * (rng.standard_normal(8 * 8 * 64) + 1j * rng.standard_normal(8 * 8 * 64))
+    gc.set_sigma(sig_g, 0.3, ngrain=4)
+    gc.gkw_grt(U, 0.9, 0.3, want_grt=False)
+    gc.set_timing(True)
+    for _ in range(5):
+        mu_g, steps_g, _ = gc.gkw_grt(U, 0.9, 0.3, want_grt=False)
+    gt_ms, gt_n = gc.kernel_time(K_GTAU)
+    gc.set_timing(False)
+    gtau = {"ms_per_call": round(gt_ms / gt_n, 3), "mu_evaluations": steps_g,
+            "kernel": "gt_setup + 11 x (gt_density over 63 candidate mu + gt_walk) + gt_fine + gt_dft1 + gt_dft2",
+            "workload": "calc_Gkw_Grt, Nx=8 Ngrain=4 (32x32 fine grid), Nf=64, nParticle=0.9, random Sigma"}
+    print(f"[bench] gtau: {gt_ms / gt_n:.3f} ms per call, {steps_g} evaluations", file=sys.stderr, flush=True)
+    gc.close()
+
     # the reference's CPU case (4x4, Nf=16, Nl=1) as one full iteration on the device, graph replayed
     from tups_amd import F as FV
     c1 = TupsContext(nx=4, nf=16, nl=1, beta=2.0, f_damping=0.5)
@@ -410,18 +435,18 @@ def main():
         c1.upload(FV, r, fv1[r])
         c1.upload(LAMBDA, r, lam1[r])
     c1.calc_M()
-    c1.set_sigma(sig1, 0.0, sigma_h=np.zeros((4, 4), dtype=np.complex128))
-    c1.iteration_full(U, graph=True)
+    c1.set_sigma(sig1, 0.0)
+    c1.iteration_full(U, graph=True, use_u2=True, n_particle=1.0)
     c1.sync()
     c1.record(0)
     for _ in range(20):
-        c1.iteration_full(U, graph=True)
+        c1.iteration_full(U, graph=True, use_u2=True, n_particle=1.0)
     c1.record(1)
     c1.sync()
     c1_ms = c1.elapsed_ms(0, 1) / 20
-    config1 = {"gpu_ms_per_iteration": round(c1_ms, 4), "iterations": 20,
-               "workload": "one full iteration, Nx=4 Nf=16 Nl=1 Nq=48, beta=2, U=2, replayed as one HIP graph: "
-                           "calc_chi, BSE, calc_PhiR, calculate_PAE, SDE (no use_U2 term), Sigma update, G = F - G"}
+    mu1, steps1 = c1.get_mu()
+    config1 = {"gpu_ms_per_iteration": round(c1_ms, 4), "iterations": 20, "mu": mu1, "mu_evaluations": steps1,
+               "workload": CONFIG1_WORKLOAD + ", replayed as one HIP graph"}
     print(f"[bench] config-1 iteration: {c1_ms:.3f} ms", file=sys.stderr, flush=True)
     c1.close()
 
@@ -460,6 +485,7 @@ def main():
         "batching": batching,
         "large_share": large,
         "sde": sde,
+        "gtau": gtau,
         "config1_iteration": config1,
         "cpu_baseline": cpu,
     }

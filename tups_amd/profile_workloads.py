@@ -109,7 +109,7 @@ def batch_flops(b: TupsBatch) -> float:
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["bse", "split", "crossing", "phir", "calib", "batch"])
+    ap.add_argument("workload", choices=["bse", "split", "crossing", "phir", "calib", "batch", "gtau", "config1"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--nx", type=int, default=16)
     ap.add_argument("--nf", type=int, default=64)
@@ -117,6 +117,36 @@ def main(argv=None):
     ap.add_argument("--moduli", type=int, default=12)
     ap.add_argument("--variant", default="", help="batch: tile variant by name (default: autotune)")
     args = ap.parse_args(argv)
+    if args.workload == "gtau":
+        # calc_Gkw_Grt: Nx=8, Ngrain=4 (32 x 32 fine grid), Nf=64, random Sigma, nParticle 0.9
+        c = TupsContext(nx=8, nf=64, nl=1, beta=2.0, f_damping=0.5)
+        rng = np.random.default_rng(3)
+        sig = 0.2 * (rng.standard_normal(8 * 8 * 64) + 1j * rng.standard_normal(8 * 8 * 64))
+        c.set_sigma(sig, 0.3, ngrain=4)
+        for _ in range(args.steps):
+            mu, steps, _ = c.gkw_grt(2.0, 0.9, 0.3, want_grt=False)
+        print(f"gtau: {args.steps} calls, mu {mu:.6f} after {steps} evaluations")
+        c.close()
+        sys.stdout.flush()
+        return
+    if args.workload == "config1":
+        # one full iteration of Nx=4 Nf=16 Nl=1 (beta=2, U=2, nParticle=1, use_U2), graph replayed
+        c = TupsContext(nx=4, nf=16, nl=1, beta=2.0, f_damping=0.5)
+        rng = np.random.default_rng(7)
+        for r in range(4):
+            g = 0.05 * (rng.standard_normal((c.nz, c.nz, c.nq)) + 1j * rng.standard_normal((c.nz, c.nz, c.nq)))
+            c.upload(GAMMA, r, g + 0.5 * np.eye(c.nz)[:, :, None])
+            c.upload(F, r, g + 0.5 * np.eye(c.nz)[:, :, None])
+            c.upload(LAMBDA, r, np.full((c.nf, c.nf, c.nw), (2.0, -2.0, 4.0, 0.0)[r], dtype=np.complex128))
+        c.calc_M()
+        c.set_sigma(np.zeros(4 * 4 * 16, dtype=np.complex128), 0.0)
+        for _ in range(args.steps):
+            c.iteration_full(2.0, graph=True, use_u2=True, n_particle=1.0)
+        c.sync()
+        print(f"config1: {args.steps} graph-replayed iterations, mu {c.get_mu()}")
+        c.close()
+        sys.stdout.flush()
+        return
     if args.workload == "batch":
         b = batch_context()
         if args.variant:
