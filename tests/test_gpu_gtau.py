@@ -3,8 +3,9 @@ the SDE's second-order term, against the oracle (oracle/gtau.py).
 
 The device search evaluates six levels of the bisection tree per pass and walks
 them as the sequential loop would, so mu and the number of evaluations must be
-the oracle's exactly; Grt (direct 2D DFTs on the device, numpy FFTs in the
-oracle) agrees to 1e-12 relative Frobenius error.
+the oracle's exactly (the one-workgroup search of small grids and the
+multi-pass search of larger ones, 8x8 at Ngrain 4); Grt (direct 2D DFTs on the
+device, numpy FFTs in the oracle) agrees to 1e-12 relative Frobenius error.
 """
 import sys
 
@@ -24,8 +25,8 @@ def _say(*a):
     print(*a, flush=True, file=sys.stderr)
 
 
-@pytest.mark.parametrize("nx,nf,ngrain,n_target", [(4, 16, 1, 0.9), (4, 16, 3, 1.1), (8, 32, 2, 0.75),
-                                                    (6, 8, 1, 1.0)])
+@pytest.mark.parametrize("nx,nf,ngrain,n_target",
+                         [(4, 16, 1, 0.9), (4, 16, 3, 1.1), (8, 32, 2, 0.75), (8, 32, 4, 0.8), (6, 8, 1, 1.0)])
 def test_gkw_grt_bisection_parity(nx, nf, ngrain, n_target):
     from tups_amd import TupsContext
 

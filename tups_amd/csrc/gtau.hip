@@ -18,7 +18,8 @@
 // One pass evaluates the next six levels of the bisection tree at once (63
 // candidate mu, one row of workgroups each); one workgroup then walks the tree
 // as the sequential loop would, stops at the first mu within the tolerance, and
-// lays out the next subtree.  The mu found is the one the reference's loop stops
+// lays out the next subtree.  Small grids run every pass inside one workgroup
+// of 1024 lanes (16 per candidate) and stop as soon as the walk ends.  The mu found is the one the reference's loop stops
 // at; the number of launches is fixed (passes after convergence exit at once),
 // so the whole search sits in a HIP graph.
 //
@@ -38,6 +39,7 @@ constexpr int kCand = (1 << kDepth) - 1;  // candidates per pass
 constexpr int kMaxSteps = 64;             // evaluations before the search gives up
 constexpr int kPasses = (kMaxSteps + kDepth - 1) / kDepth;
 constexpr double kTol = 1e-6;             // |n - nParticle|, SRC/parquet_chi.f90:117
+constexpr double kSmallWork = 1 << 19;    // 63 Nfine Nf up to this: the one-workgroup search
 
 struct GtauArgs {
   const double2* sigma;  // (Nx Ny Nf)
@@ -145,6 +147,16 @@ __global__ __launch_bounds__(256) void gt_setup(GtauArgs a, int mu_from_host, do
   fill_tree(st->cand, bounds[0], bounds[1], tid);
 }
 
+// G(k, beta) of fine point j at mu: the summand of the particle number
+__device__ __forceinline__ double g_beta(const GtauArgs& a, double mu, int j) {
+  const double e = a.etab[j];
+  const int kc = coarse_of(a, j % a.n1, j / a.n1);
+  const double2* s = a.sigma + (size_t)kc * a.nf;
+  double acc = 0.0;
+  for (int nu = 1; nu <= a.nf; ++nu) acc += dress(wn(nu, a.nf, a.beta), mu, e, s[nu - 1]).x;
+  return -acc / a.beta - fermi(a.beta * (e - mu));
+}
+
 // grid (nbk, kCand): partial sums of G(k, beta) over the fine grid per candidate
 __global__ __launch_bounds__(256) void gt_density(GtauArgs a) {
   __shared__ double red[256];
@@ -152,22 +164,60 @@ __global__ __launch_bounds__(256) void gt_density(GtauArgs a) {
   const int c = blockIdx.y;
   const double mu = a.st->cand[c];
   const int j = blockIdx.x * 256 + threadIdx.x;
-  double v = 0.0;
-  if (j < a.nfine) {
-    const double e = a.etab[j];
-    const int kc = coarse_of(a, j % a.n1, j / a.n1);
-    const double2* s = a.sigma + (size_t)kc * a.nf;
-    double acc = 0.0;
-    for (int nu = 1; nu <= a.nf; ++nu) acc += dress(wn(nu, a.nf, a.beta), mu, e, s[nu - 1]).x;
-    v = -acc / a.beta - fermi(a.beta * (e - mu));
-  }
-  red[threadIdx.x] = v;
+  red[threadIdx.x] = j < a.nfine ? g_beta(a, mu, j) : 0.0;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
     if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
     __syncthreads();
   }
   if (threadIdx.x == 0) a.partial[(size_t)c * a.nbk + blockIdx.x] = red[0];
+}
+
+// one lane: walks the evaluated subtree (candidates cand, particle numbers t1c) as the
+// sequential loop of SRC/parquet_chi.f90:117-198 would; returns whether the search ended
+// and leaves the new bracket in bounds
+__device__ int walk_tree(GtauState* st, const double* cand, const double* t1c, double* bounds) {
+  double h = st->hi, l = st->lo;
+  const double target = st->n_target;
+  int node = 1, steps = st->steps, done = 0, status = 0;
+  double mu = st->mu, t1 = st->t1;
+  for (int level = 0; level < kDepth; ++level) {
+    mu = cand[node - 1];
+    t1 = t1c[node - 1];
+    steps += 1;
+    if (st->single) {
+      done = 1;
+      break;
+    }
+    // the loop condition abs(t1 - nParticle) > 1e-6 is false for a NaN t1 as well
+    if (!(fabs(t1 - target) > kTol)) {
+      done = 1;
+      if (!isfinite(t1)) status = 2;
+      break;
+    }
+    if (t1 > target) {
+      h = mu;
+      node = 2 * node;
+    } else {
+      l = mu;
+      node = 2 * node + 1;
+    }
+    if (steps >= kMaxSteps) {
+      done = 1;
+      status = 1;
+      break;
+    }
+  }
+  st->mu = mu;
+  st->t1 = t1;
+  st->steps = steps;
+  st->hi = h;
+  st->lo = l;
+  st->status = status;
+  st->done = done;
+  bounds[0] = h;
+  bounds[1] = l;
+  return done;
 }
 
 // one workgroup of 64: particle numbers of the candidates, the walk, the next subtree
@@ -185,51 +235,41 @@ __global__ __launch_bounds__(64) void gt_walk(GtauArgs a) {
     st->t1c[tid] = t1c[tid];
   }
   __syncthreads();
-  if (tid == 0) {
-    double h = st->hi, l = st->lo;
-    const double target = st->n_target;
-    int node = 1, steps = st->steps, done = 0, status = 0;
-    double mu = st->mu, t1 = st->t1;
-    for (int level = 0; level < kDepth; ++level) {
-      mu = st->cand[node - 1];
-      t1 = t1c[node - 1];
-      steps += 1;
-      if (st->single) {
-        done = 1;
-        break;
-      }
-      // the loop condition abs(t1 - nParticle) > 1e-6 is false for a NaN t1 as well
-      if (!(fabs(t1 - target) > kTol)) {
-        done = 1;
-        if (!isfinite(t1)) status = 2;
-        break;
-      }
-      if (t1 > target) {
-        h = mu;
-        node = 2 * node;
-      } else {
-        l = mu;
-        node = 2 * node + 1;
-      }
-      if (steps >= kMaxSteps) {
-        done = 1;
-        status = 1;
-        break;
-      }
-    }
-    st->mu = mu;
-    st->t1 = t1;
-    st->steps = steps;
-    st->hi = h;
-    st->lo = l;
-    st->status = status;
-    st->done = done;
-    bounds[0] = h;
-    bounds[1] = l;
-    finished = done;
-  }
+  if (tid == 0) finished = walk_tree(st, st->cand, t1c, bounds);
   __syncthreads();
   if (!finished) fill_tree(st->cand, bounds[0], bounds[1], tid);
+}
+
+// the whole search in one workgroup of 1024 for small grids (63 Nfine Nf within
+// kSmallWork): 16 lanes per candidate, passes until the walk ends, one launch
+__global__ __launch_bounds__(1024) void gt_search_small(GtauArgs a) {
+  __shared__ double cand[kCand + 1];
+  __shared__ double t1c[kCand + 1];
+  __shared__ double part[1024];
+  __shared__ double bounds[2];
+  __shared__ int finished;
+  GtauState* st = a.st;
+  if (st->done) return;
+  const int tid = threadIdx.x, grp = tid >> 4, lane = tid & 15;
+  if (tid < kCand) cand[tid] = st->cand[tid];
+  for (int pass = 0; pass < kPasses; ++pass) {
+    __syncthreads();
+    double v = 0.0;
+    if (grp < kCand)
+      for (int j = lane; j < a.nfine; j += 16) v += g_beta(a, cand[grp], j);
+    part[tid] = v;
+    __syncthreads();
+    if (lane == 0 && grp < kCand) {
+      double s = 0.0;
+      for (int i = 0; i < 16; ++i) s += part[tid + i];
+      t1c[grp] = -2.0 * s / (double)a.nfine;
+    }
+    __syncthreads();
+    if (tid == 0) finished = walk_tree(st, cand, t1c, bounds);
+    __syncthreads();
+    if (finished) break;
+    fill_tree(cand, bounds[0], bounds[1], tid);
+  }
 }
 
 // grid: nfine; G(k_fine, tau) for every tau at the final mu (SRC/parquet_chi.f90:124-170)
@@ -351,7 +391,7 @@ __global__ __launch_bounds__(256) void u2_ggg(GtauArgs a) {
 
 // grid Nc: FDfit + nfourier of one coarse momentum, added to Sigma at +/- nu
 // (SRC/parquet_SDE.f90:317-333)
-__global__ __launch_bounds__(64) void u2_spline(GtauArgs a) {
+__global__ __launch_bounds__(256) void u2_spline(GtauArgs a) {
   extern __shared__ double sp[];
   const int nf = a.nf, L = nf - 1, kidx = blockIdx.x;
   double* r = sp;             // [L + 1]
@@ -391,34 +431,53 @@ __global__ __launch_bounds__(64) void u2_spline(GtauArgs a) {
     for (int k = L - 1; k >= 0; --k) xm[k] = qv[k] * xm[k + 1] + uv[k];
   }
   __syncthreads();
+  // the Nf/2 frequencies, each summed over the L intervals by `ks` lanes (j = 1 + part,
+  // 1 + part + ks, ...) whose sums are then added in order
+  __shared__ double2 red[256];
   const double pi = 3.14159265358979323846;
   double2* out = a.sigma_out + (size_t)kidx * nf;
-  for (int i = threadIdx.x; i < nf / 2; i += blockDim.x) {
-    const double om = (2.0 * (i + 1) - 1.0) * pi / beta;
-    const double o2 = om * om, o3 = o2 * om, o4 = o2 * o2;
+  const int nw = nf / 2;
+  int ks = 1;
+  while (ks * 2 * nw <= 256 && ks * 2 <= L) ks *= 2;
+  const int per = 256 / ks, part = threadIdx.x % ks, slot = threadIdx.x / ks;
+  for (int i0 = 0; i0 < nw; i0 += per) {
+    const int i = i0 + slot;
     double re = 0.0, im = 0.0;
-    for (int j = 1; j <= L; ++j) {
-      const double aj = r[j - 1], cj = xm[j - 1] / 2.0;
-      const double bj = (r[j] - r[j - 1]) / delta - (2.0 * xm[j - 1] + xm[j]) * delta / 6.0;
-      const double dj = (xm[j] - xm[j - 1]) / (6.0 * delta);
-      double s1, c1, s0, c0;
-      sincos(om * delta * j, &s1, &c1);
-      sincos(om * delta * (j - 1), &s0, &c0);
-      // e^{i w delta j} (P1r + i P1i) + e^{i w delta (j-1)} (P0r + i P0i)
-      const double p1r = -6.0 * dj / o4 + (bj + 2.0 * delta * cj + 3.0 * delta * delta * dj) / o2;
-      const double p1i = (2.0 * cj + 6.0 * delta * dj) / o3 +
-                         (-aj - delta * bj - delta * delta * cj - delta * delta * delta * dj) / om;
-      const double p0r = 6.0 * dj / o4 - bj / o2;
-      const double p0i = -2.0 * cj / o3 + aj / om;
-      re += c1 * p1r - s1 * p1i + c0 * p0r - s0 * p0i;
-      im += c1 * p1i + s1 * p1r + c0 * p0i + s0 * p0r;
+    if (i < nw) {
+      const double om = (2.0 * (i + 1) - 1.0) * pi / beta;
+      const double o2 = om * om, o3 = o2 * om, o4 = o2 * o2;
+      for (int j = 1 + part; j <= L; j += ks) {
+        const double aj = r[j - 1], cj = xm[j - 1] / 2.0;
+        const double bj = (r[j] - r[j - 1]) / delta - (2.0 * xm[j - 1] + xm[j]) * delta / 6.0;
+        const double dj = (xm[j] - xm[j - 1]) / (6.0 * delta);
+        double s1, c1, s0, c0;
+        sincos(om * delta * j, &s1, &c1);
+        sincos(om * delta * (j - 1), &s0, &c0);
+        // e^{i w delta j} (P1r + i P1i) + e^{i w delta (j-1)} (P0r + i P0i)
+        const double p1r = -6.0 * dj / o4 + (bj + 2.0 * delta * cj + 3.0 * delta * delta * dj) / o2;
+        const double p1i = (2.0 * cj + 6.0 * delta * dj) / o3 +
+                           (-aj - delta * bj - delta * delta * cj - delta * delta * delta * dj) / om;
+        const double p0r = 6.0 * dj / o4 - bj / o2;
+        const double p0i = -2.0 * cj / o3 + aj / om;
+        re += c1 * p1r - s1 * p1i + c0 * p0r - s0 * p0i;
+        im += c1 * p1i + s1 * p1r + c0 * p0i + s0 * p0r;
+      }
     }
-    // ret(i) lands at nu = Nf/2 - i (conjugated) and at Nf + 1 - nu
-    const int nu = nf / 2 - i;
-    double2 v = out[nu - 1];
-    out[nu - 1] = make_double2(v.x + re, v.y - im);
-    v = out[nf - nu];
-    out[nf - nu] = make_double2(v.x + re, v.y + im);
+    red[threadIdx.x] = make_double2(re, im);
+    __syncthreads();
+    if (part == 0 && i < nw) {
+      for (int t = 1; t < ks; ++t) {
+        re += red[threadIdx.x + t].x;
+        im += red[threadIdx.x + t].y;
+      }
+      // ret(i) lands at nu = Nf/2 - i (conjugated) and at Nf + 1 - nu
+      const int nu = nw - i;
+      double2 v = out[nu - 1];
+      out[nu - 1] = make_double2(v.x + re, v.y - im);
+      v = out[nf - nu];
+      out[nf - nu] = make_double2(v.x + re, v.y + im);
+    }
+    __syncthreads();
   }
 }
 
@@ -496,7 +555,12 @@ int launch_gtau(tups_ctx* c, int mode, int mu_from_host, double mu_host, double 
                      set_hartree);
   TUPS_HIP(hipGetLastError());
   const int passes = mode == 2 ? 0 : (mode == 1 ? 1 : kPasses);
-  for (int p = 0; p < passes; ++p) {
+  const bool small = (double)kCand * a.nfine * nf <= kSmallWork;
+  if (passes > 0 && small) {
+    hipLaunchKernelGGL(gt_search_small, dim3(1), dim3(1024), 0, c->stream, a);
+    TUPS_HIP(hipGetLastError());
+  }
+  for (int p = 0; p < (small ? 0 : passes); ++p) {
     hipLaunchKernelGGL(gt_density, dim3((unsigned)a.nbk, kCand), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(gt_walk, dim3(1), dim3(64), 0, c->stream, a);
@@ -524,7 +588,7 @@ int launch_order_2nd(tups_ctx* c, double u, double2* sigma_out) {
   const int nf = c->sz.nf, nc = c->sz.nx * c->sz.ny;
   hipLaunchKernelGGL(u2_ggg, dim3((unsigned)nc, (unsigned)nf), dim3(256), 0, c->stream, a);
   TUPS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(u2_spline, dim3((unsigned)nc), dim3(64), 4 * (size_t)nf * sizeof(double), c->stream, a);
+  hipLaunchKernelGGL(u2_spline, dim3((unsigned)nc), dim3(256), 4 * (size_t)nf * sizeof(double), c->stream, a);
   TUPS_HIP(hipGetLastError());
   return TUPS_OK;
 }

@@ -73,20 +73,30 @@ __device__ __forceinline__ void sym_apply(int s, int ix, int iy, int nx, int& ox
   oy = ((b % nx) + nx) % nx + 1;
 }
 
-// grid: nq; one thread per (l4, nu) entry of the enlarged box
+// grid: nq; the (l4, nu) entries of the enlarged box, each summed over k by `ks`
+// lanes (k = part, part + ks, ...; ks = 256 / entries rounded down to a power of
+// two, 1 once there are 256 entries or more) and the lanes' sums added in order
 __global__ __launch_bounds__(256) void sde_chi_aux(SdeArgs a) {
+  __shared__ double2 red[2][256];
   const GreenParams g = green_resolve(a.g);
   const int q = blockIdx.x;
   const int m = a.qm[q];
   const int qx = a.ibz[m], qy = a.ibz[a.nibz + m], w = a.qw[q];
-  const int nx = a.nx, nf = a.nf, ng = g.ngrain;
+  const int nx = a.nx, nc = nx * nx, nf = a.nf, ng = g.ngrain;
+  const int nent = a.nl * a.ext;
+  int ks = 1;
+  while (ks * 2 * nent <= 256 && ks * 2 <= nc) ks *= 2;
+  const int per = 256 / ks;  // entries per round
+  const int part = threadIdx.x % ks, slot = threadIdx.x / ks;
   const double inv_g2 = 1.0 / ng / ng;
-  for (int e = threadIdx.x; e < a.nl * a.ext; e += blockDim.x) {
-    const int l4 = e / a.ext, nu = e - l4 * a.ext - a.f_range * nf + 1;  // nu in (-f_range Nf, (f_range + 1) Nf]
+  for (int e0 = 0; e0 < nent; e0 += per) {
+    const int e = e0 + slot;
     double2 sph = make_double2(0.0, 0.0), spp = make_double2(0.0, 0.0);
-    for (int ky = 1; ky <= nx; ++ky)
-      for (int kx = 1; kx <= nx; ++kx) {
-        const double f = a.ff[(size_t)l4 * nx * nx + (ky - 1) * nx + (kx - 1)];
+    if (e < nent) {
+      const int l4 = e / a.ext, nu = e - l4 * a.ext - a.f_range * nf + 1;  // nu in (-f_range Nf, (f_range + 1) Nf]
+      for (int k = part; k < nc; k += ks) {
+        const int kx = k % nx + 1, ky = k / nx + 1;
+        const double f = a.ff[(size_t)l4 * nc + k];
         const int kqx = (kx - 1 + qx - 1) % nx + 1, kqy = (ky - 1 + qy - 1) % nx + 1;
         const int mkx = (-(kx - 1) + nx) % nx + 1, mky = (-(ky - 1) + nx) % nx + 1;
         const int qmkx = (mkx - 1 + qx - 1) % nx + 1, qmky = (mky - 1 + qy - 1) % nx + 1;
@@ -106,8 +116,29 @@ __global__ __launch_bounds__(256) void sde_chi_aux(SdeArgs a) {
         spp.x += tpp.x;
         spp.y += tpp.y;
       }
-    a.caux_ph[(size_t)q * a.nl * a.ext + e] = sph;
-    a.caux_pp[(size_t)q * a.nl * a.ext + e] = spp;
+    }
+    if (ks == 1) {
+      if (e < nent) {
+        a.caux_ph[(size_t)q * nent + e] = sph;
+        a.caux_pp[(size_t)q * nent + e] = spp;
+      }
+      continue;
+    }
+    red[0][threadIdx.x] = sph;
+    red[1][threadIdx.x] = spp;
+    __syncthreads();
+    if (part == 0 && e < nent) {
+      double2 tph = red[0][threadIdx.x], tpp = red[1][threadIdx.x];
+      for (int i = 1; i < ks; ++i) {
+        tph.x += red[0][threadIdx.x + i].x;
+        tph.y += red[0][threadIdx.x + i].y;
+        tpp.x += red[1][threadIdx.x + i].x;
+        tpp.y += red[1][threadIdx.x + i].y;
+      }
+      a.caux_ph[(size_t)q * nent + e] = tph;
+      a.caux_pp[(size_t)q * nent + e] = tpp;
+    }
+    __syncthreads();
   }
 }
 
@@ -154,7 +185,7 @@ __global__ __launch_bounds__(256) void sde_vchi(SdeArgs a) {
   a.pp[(size_t)q * nz + i3] = pp;
 }
 
-// grid: nibz * Nf; one workgroup per IBZ (k, nu), threads over the q slots
+// grid: nibz * Nf; one workgroup per IBZ (k, nu), threads over (q slot, star operation)
 __global__ __launch_bounds__(256) void sde_sigma(SdeArgs a) {
   const GreenParams g = green_resolve(a.g);
   __shared__ double red[2][256];
@@ -164,42 +195,41 @@ __global__ __launch_bounds__(256) void sde_sigma(SdeArgs a) {
   const int mkx = (-(kx - 1) + nx) % nx + 1, mky = (-(ky - 1) + nx) % nx + 1, mkw = -k0 + nf + 1;
   const int kidx = (ky - 1) * nx + (kx - 1);
   double ar = 0.0, ai = 0.0;
-  for (int q = threadIdx.x; q < a.nq; q += blockDim.x) {
+  for (int item = threadIdx.x; item < a.nq * 8; item += blockDim.x) {  // (q slot, operation of q's star)
+    const int q = item >> 3, s = (item & 7) + 1;
     const int m = a.qm[q];
     const int qx = a.ibz[m], qy = a.ibz[a.nibz + m], w = a.qw[q];
     const unsigned ops = star_ops(qx, qy, a.nxi);
     const double2* pl = a.pl + (size_t)q * a.nz;
     const double2* pp = a.pp + (size_t)q * a.nz;
-    for (int s = 1; s <= 8; ++s) {
-      if (!((ops >> (s - 1)) & 1u)) continue;
-      int sx, sy;
-      sym_apply(s, qx, qy, nx, sx, sy);
-      const double2 gkq = green_plain(g, (kx - 1 + sx - 1) % nx + 1, (ky - 1 + sy - 1) % nx + 1, k0 + w - 1);
-      const int qkx = (mkx - 1 + sx - 1) % nx + 1, qky = (mky - 1 + sy - 1) % nx + 1, qkw = mkw + w - 1;
-      const double2 gqmk = green_plain(g, qkx, qky, qkw);
-      const double2 gkmq =
-          green_plain(g, (-(qkx - 1) + nx) % nx + 1, (-(qky - 1) + nx) % nx + 1, -qkw + nf + 1);
-      for (int l3 = 0; l3 < nl; ++l3) {
-        const double fi = a.ffi[((size_t)(s - 1) * nl + l3) * nc + kidx];
-        const int i3 = l3 * nf + k0 - 1;
-        const double2 x = pl[i3], y = pp[i3];
-        double2 t = cmul(gkq, x);
-        ar += fi * t.x;
-        ai += fi * t.y;
-        t = cmul(gqmk, y);
-        ar += fi * t.x;
-        ai += fi * t.y;
-        if (w == 1) continue;
-        const int i3m = l3 * nf + (-k0 + nf + 1) - 1;
-        const double sg = (double)a.signs[l3];
-        const double2 xm = pl[i3m], ym = pp[i3m];
-        t = cmul(gkmq, make_double2(xm.x, -xm.y));
-        ar += fi * sg * t.x;
-        ai += fi * sg * t.y;
-        t = cmul(make_double2(gkq.x, -gkq.y), make_double2(ym.x, -ym.y));
-        ar += fi * sg * t.x;
-        ai += fi * sg * t.y;
-      }
+    if (!((ops >> (s - 1)) & 1u)) continue;
+    int sx, sy;
+    sym_apply(s, qx, qy, nx, sx, sy);
+    const double2 gkq = green_plain(g, (kx - 1 + sx - 1) % nx + 1, (ky - 1 + sy - 1) % nx + 1, k0 + w - 1);
+    const int qkx = (mkx - 1 + sx - 1) % nx + 1, qky = (mky - 1 + sy - 1) % nx + 1, qkw = mkw + w - 1;
+    const double2 gqmk = green_plain(g, qkx, qky, qkw);
+    const double2 gkmq =
+        green_plain(g, (-(qkx - 1) + nx) % nx + 1, (-(qky - 1) + nx) % nx + 1, -qkw + nf + 1);
+    for (int l3 = 0; l3 < nl; ++l3) {
+      const double fi = a.ffi[((size_t)(s - 1) * nl + l3) * nc + kidx];
+      const int i3 = l3 * nf + k0 - 1;
+      const double2 x = pl[i3], y = pp[i3];
+      double2 t = cmul(gkq, x);
+      ar += fi * t.x;
+      ai += fi * t.y;
+      t = cmul(gqmk, y);
+      ar += fi * t.x;
+      ai += fi * t.y;
+      if (w == 1) continue;
+      const int i3m = l3 * nf + (-k0 + nf + 1) - 1;
+      const double sg = (double)a.signs[l3];
+      const double2 xm = pl[i3m], ym = pp[i3m];
+      t = cmul(gkmq, make_double2(xm.x, -xm.y));
+      ar += fi * sg * t.x;
+      ai += fi * sg * t.y;
+      t = cmul(make_double2(gkq.x, -gkq.y), make_double2(ym.x, -ym.y));
+      ar += fi * sg * t.x;
+      ai += fi * sg * t.y;
     }
   }
   red[0][threadIdx.x] = ar;
