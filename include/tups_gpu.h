@@ -196,6 +196,23 @@ int tups_gkw_grt(tups_ctx* ctx, double u, double n_particle, int32_t cal_chi, do
  * and the number of evaluations of the last search; returns the search's
  * failure, if any, as tups_gkw_grt would.  Synchronous. */
 int tups_get_mu(tups_ctx* ctx, double* mu, int32_t* steps);
+/* conv() (SRC/parquet_check.f90:39-150) on the device.  e = |SigmaOld - Sigma| /
+ * |Sigma| elementwise over Nx Ny Nf: sigma_avg = |sum(e)| / Nt, sigma_max =
+ * max(e) at 1-based (kx, ky, nu) (the first maximum, as maxloc); per channel
+ * f_max = max |F_r| over the resident F and f_rel_err = |1 - f_max / f_max of
+ * the previous call on this context| (infinite on the first call, where the
+ * reference's previous value is unset); converged = sigma_max < f_sigma and
+ * every f_rel_err < f_vert (the reference uses 1e-8 for both,
+ * SRC/parquet_ini.f90:31-32).  sigma_old and sigma: host arrays (Nx Ny Nf
+ * complex), or both NULL for the pair the last tups_iteration_full left on the
+ * device.  Synchronous. */
+typedef struct tups_conv_info {
+  double sigma_avg, sigma_max;
+  int32_t kx, ky, nu, converged;
+  double f_max[4], f_rel_err[4]; /* channels d, m, s, t */
+} tups_conv_info;
+int tups_conv(tups_ctx* ctx, const double* sigma_old, const double* sigma, double f_sigma, double f_vert,
+              tups_conv_info* info);
 /* The loop body of SRC/loop.f90:67-222 on the device (no counterpart as one
  * call): calc_Gkw_Grt's mu search when n_particle >= 0 (else mu stays as set)
  * -> calc_chi from the device Sigma -> BSE(mode) -> calc_PhiR -> calculate_PAE
@@ -273,7 +290,8 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 #define TUPS_K_BSE_BATCH 10   /* batching form: variant names only (tups_variant_name/count) */
 #define TUPS_K_SDE 11         /* self-energy: auxiliary bubbles, vertex contraction, star sums */
 #define TUPS_K_GTAU 12        /* G(tau): mu bisection, fine-grid G(k, tau), 2D transform */
-#define TUPS_K_COUNT 13
+#define TUPS_K_CONV 13        /* convergence check: Sigma change, vertex maxima */
+#define TUPS_K_COUNT 14
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);
 /* scans the resident array for NaN/Inf; returns TUPS_ENONFINITE if any */

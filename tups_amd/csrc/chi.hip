@@ -98,6 +98,8 @@ int upload_sigma(tups_ctx* c, const double* sigma, const double* sigma_h, GreenP
   TUPS_CHECK(ngrain >= 1 && ngrain <= 64, TUPS_EINVAL, "ngrain must be in 1..64");
   if (!c->d_sigma) TUPS_HIP(hipMalloc((void**)&c->d_sigma, nt * sizeof(double2)));
   if (!c->d_sigh) TUPS_HIP(hipMalloc((void**)&c->d_sigh, (size_t)nc * sizeof(double2)));
+  if (!c->d_sigma_old) TUPS_HIP(hipMalloc((void**)&c->d_sigma_old, nt * sizeof(double2)));
+  c->sigma_old_valid = false;
   std::vector<double> sh(2 * (size_t)nc);
   if (sigma_h) {
     std::copy(sigma_h, sigma_h + 2 * (size_t)nc, sh.begin());

@@ -292,6 +292,8 @@ int tups_destroy(tups_ctx* c) {
   f(c->split_ws);
   f(c->sde_ws);
   f(c->gt_ws);
+  f(c->conv_ws);
+  f(c->d_sigma_old);
   f(c->d_gstate);
   f(c->closed_ws);
   f(c->d_sigma);

@@ -125,6 +125,10 @@ struct tups_ctx {
   size_t gt_ws_bytes = 0;
   int gt_ngrain = 0;             // Ngrain the G(tau) workspace is laid out for
   bool grt_valid = false;        // Grt of the last calc_Gkw_Grt is in the workspace
+  double2* d_sigma_old = nullptr;  // Sigma before the last device SDE update (SigmaOld of conv)
+  bool sigma_old_valid = false;
+  void* conv_ws = nullptr;       // convergence check: host Sigma pair, partials, result
+  double conv_fmax_prev[4] = {};  // max |F_r| of the previous convergence check
   double full_n_particle = -1.0; // full-iteration graphs: nParticle and Ngrain they were captured with
   int full_ngrain = 0;
   hipGraphExec_t graph_full[4] = {};  // full iteration, per (ladder mode, buffer parity)

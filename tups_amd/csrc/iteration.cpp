@@ -162,7 +162,7 @@ extern "C" int tups_iteration_full(tups_ctx* c, int32_t mode, int32_t use_graph,
   bool swapped = false;
   if (!use_graph) {
     rc = full_sequence(c, mode, f_range, u, use_u2 ? 1 : 0, n_particle, &swapped);
-    if (!rc) c->phir_valid = true;
+    if (!rc) c->phir_valid = c->sigma_old_valid = true;
     return rc;
   }
   if (c->full_f_range != f_range || c->full_u != u || c->full_use_u2 != (use_u2 ? 1 : 0) ||
@@ -203,7 +203,7 @@ extern "C" int tups_iteration_full(tups_ctx* c, int32_t mode, int32_t use_graph,
   }
   TUPS_HIP(hipGraphLaunch(c->graph_full[slot], c->stream));
   swap_vertex_buffers(c);  // the graph's BSE wrote Phi into the partner buffers
-  c->phir_valid = true;
+  c->phir_valid = c->sigma_old_valid = true;
   return TUPS_OK;
 }
 

@@ -327,14 +327,17 @@ int launch_sde_device(tups_ctx* c, const GreenParams& g, int f_range, double u, 
   return TUPS_OK;
 }
 
-// Sigma <- the SDE output, and with the default rule Sigma_H(k) = Sigma(k, Nf)
-// (SRC/parquet_chi.f90:103-110): the device state for the next iteration
+// SigmaOld <- Sigma (SRC/loop.f90:152), Sigma <- the SDE output, and with the default
+// rule Sigma_H(k) = Sigma(k, Nf) (SRC/parquet_chi.f90:103-110): the device state for the
+// convergence check and the next iteration
 __global__ __launch_bounds__(256) void sigma_commit(const double2* __restrict__ src, double2* __restrict__ sigma,
-                                                    double2* __restrict__ sigh, int nx, int nf, int sigh_default) {
+                                                    double2* __restrict__ sigma_old, double2* __restrict__ sigh,
+                                                    int nx, int nf, int sigh_default) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t nt = (size_t)nx * nx * nf;
   if (i >= nt) return;
   const double2 v = src[i];
+  sigma_old[i] = sigma[i];
   sigma[i] = v;
   if (sigh_default && (int)(i % nf) == nf - 1) {
     const size_t k = i / nf;  // (kx - 1) Ny + ky - 1
@@ -347,7 +350,8 @@ int launch_sigma_commit(tups_ctx* c, int f_range) {
   const tups_sizes& s = c->sz;
   const size_t nt = (size_t)s.nx * s.ny * s.nf;
   hipLaunchKernelGGL(sigma_commit, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream,
-                     sde_output(c, f_range), c->d_sigma, c->d_sigh, s.nx, s.nf, c->sigh_default ? 1 : 0);
+                     sde_output(c, f_range), c->d_sigma, c->d_sigma_old, c->d_sigh, s.nx, s.nf,
+                     c->sigh_default ? 1 : 0);
   TUPS_HIP(hipGetLastError());
   return TUPS_OK;
 }

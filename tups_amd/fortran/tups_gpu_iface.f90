@@ -22,12 +22,18 @@ module tups_gpu_iface
                                    TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4, TUPS_K_PAE_GATHER = 5, &
                                    TUPS_K_SPLIT_PREP = 6, TUPS_K_SPLIT_GEMM = 7, TUPS_K_BSE_CLOSED = 8, &
                                    TUPS_K_CALC_CHI = 9, TUPS_K_BSE_BATCH = 10, TUPS_K_SDE = 11, &
-                                   TUPS_K_GTAU = 12
+                                   TUPS_K_GTAU = 12, TUPS_K_CONV = 13
 
   type, bind(C) :: tups_dims
     integer(c_int32_t) :: nx, nf, nl, nr, nq, reserved
     real(c_double) :: beta, f_damping
   end type tups_dims
+
+  type, bind(C) :: tups_conv_info
+    real(c_double) :: sigma_avg, sigma_max
+    integer(c_int32_t) :: kx, ky, nu, converged
+    real(c_double) :: f_max(4), f_rel_err(4)
+  end type tups_conv_info
 
   type, bind(C) :: tups_sizes
     integer(c_int32_t) :: nx, ny, nf, nl, nz, nr, nw, nibz, nq, nc, full_ibz, reserved
@@ -252,6 +258,15 @@ module tups_gpu_iface
       complex(c_double_complex), intent(out) :: grt(*)
       integer(c_int) :: rc
     end function tups_gkw_grt
+
+    ! conv() (SRC/parquet_check.f90:39): pass c_null_ptr twice for the device Sigma pair
+    function tups_conv(ctx, sigma_old, sigma, f_sigma, f_vert, info) bind(C, name="tups_conv") result(rc)
+      import :: c_ptr, c_int, c_double, tups_conv_info
+      type(c_ptr), value :: ctx, sigma_old, sigma
+      real(c_double), value :: f_sigma, f_vert
+      type(tups_conv_info), intent(out) :: info
+      integer(c_int) :: rc
+    end function tups_conv
 
     function tups_get_mu(ctx, mu, steps) bind(C, name="tups_get_mu") result(rc)
       import :: c_ptr, c_int32_t, c_int, c_double

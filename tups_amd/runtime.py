@@ -21,7 +21,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
 
 GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
 (K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER, K_SPLIT_PREP, K_SPLIT_GEMM,
- K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH, K_SDE, K_GTAU) = range(13)
+ K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH, K_SDE, K_GTAU, K_CONV) = range(14)
 CH_D, CH_M, CH_S, CH_T = range(4)
 BSE_LADDER_FP64 = 0
 BSE_LADDER_SPLIT = 1
@@ -46,6 +46,12 @@ class Dims(C.Structure):
 class Sizes(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("nx", "ny", "nf", "nl", "nz", "nr", "nw", "nibz", "nq",
                                          "nc", "full_ibz", "reserved")] + [("alpha", C.c_double)]
+
+
+class ConvInfo(C.Structure):
+    _fields_ = [("sigma_avg", C.c_double), ("sigma_max", C.c_double), ("kx", C.c_int32), ("ky", C.c_int32),
+                ("nu", C.c_int32), ("converged", C.c_int32), ("f_max", C.c_double * 4),
+                ("f_rel_err", C.c_double * 4)]
 
 
 _P = C.POINTER
@@ -75,6 +81,7 @@ SIGNATURES = {
     "tups_iteration_full": ([_V, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double], C.c_int),
     "tups_gkw_grt": ([_V, C.c_double, C.c_double, C.c_int32, _D, C.POINTER(C.c_int32), _D], C.c_int),
     "tups_get_mu": ([_V, _D, C.POINTER(C.c_int32)], C.c_int),
+    "tups_conv": ([_V, _D, _D, C.c_double, C.c_double, _P(ConvInfo)], C.c_int),
     "tups_iteration": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_batch_create": ([C.c_int32, C.c_int32, _I, _I, C.c_double, C.c_double, _P(_V)], C.c_int),
     "tups_batch_destroy": ([_V], C.c_int),
@@ -310,6 +317,22 @@ class TupsContext:
         _check(self.lib.tups_gkw_grt(self._h, float(U), float(n_particle), 1 if cal_chi else 0, C.byref(m),
                                      C.byref(steps), _dptr(grt) if grt is not None else None), "tups_gkw_grt")
         return m.value, steps.value, grt
+
+    def conv(self, f_sigma: float = 1e-8, f_vert: float = 1e-8, sigma_old=None, sigma=None) -> dict:
+        """conv() (SRC/parquet_check.f90:39-150) on the device: the Sigma pair from the host, or
+        (both None) the pair the last iteration_full left on the device; F from the resident
+        vertices; the previous F maxima are the context's."""
+        so = sn = None
+        if sigma is not None:
+            so = np.ascontiguousarray(np.asarray(sigma_old, dtype=np.complex128).ravel())
+            sn = np.ascontiguousarray(np.asarray(sigma, dtype=np.complex128).ravel())
+        info = ConvInfo()
+        _check(self.lib.tups_conv(self._h, _dptr(so) if so is not None else None,
+                                  _dptr(sn) if sn is not None else None, float(f_sigma), float(f_vert),
+                                  C.byref(info)), "tups_conv")
+        return {"sigma_avg": info.sigma_avg, "sigma_max": info.sigma_max, "kx": info.kx, "ky": info.ky,
+                "nu": info.nu, "converged": bool(info.converged), "f_max": list(info.f_max),
+                "f_rel_err": list(info.f_rel_err)}
 
     def get_mu(self):
         """(mu, steps) of the device search state."""
