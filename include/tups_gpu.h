@@ -89,6 +89,26 @@ typedef struct tups_dims {
 
 typedef struct tups_ctx tups_ctx;
 
+/* The run-parameter file (set_parameters, SRC/parquet_ini.f90:289-329): 21
+ * records, one list-directed value each, in this order.  Logicals are 0/1;
+ * character values are cut to the reference's 100 characters. */
+typedef struct tups_prms {
+  double beta, u, n_particle;             /* beta, xU, nParticle */
+  int32_t nx, nf, neighbours, readinpick; /* Nx (= Ny), Nf, neighbours, readinpick */
+  int32_t ngrain, f_range, ite_max, dga;  /* Ngrain, f_range, ite_max, DGA */
+  int32_t use_sig_dmft, use_u2, save_f, oneshot, old, sde_old;
+  double f_damping;
+  char pick_file[104], fname_l[104], fname_sig_dmft[104];
+} tups_prms;
+/* Reads the file at `path` with the reference's list-directed rules (first
+ * value of each record, D or E exponents, .true./T/F logicals, quoted or bare
+ * strings); TUPS_EINVAL names the line and field of the first bad value. */
+int tups_read_prms(const char* path, tups_prms* prms);
+/* The context sizes the parameters imply: Nx, Nf, Nl from neighbours (1, 2, 3
+ * -> 1, 5, 9 form factors; SRC/parquet_formfactors.f90:495-515), beta,
+ * f_damping; NR and Nq left to the library's defaults. */
+int tups_prms_dims(const tups_prms* prms, tups_dims* dims);
+
 /* derived sizes returned by tups_query: */
 typedef struct tups_sizes {
   int32_t nx, ny, nf, nl, nz, nr, nw, nibz, nq, nc, full_ibz;

@@ -25,9 +25,9 @@ LIB = PKG / "libtups_gpu.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["context.cpp", "tables.cpp", "iteration.cpp", "bse.hip", "phir.hip", "eqmatrix.hip", "crossing.hip",
-           "vertex_ops.hip", "bse_split.hip",
-           "bse_closed.hip", "chi.hip", "sde.hip", "gtau.hip", "conv.hip"]
+SOURCES = ["context.cpp", "tables.cpp", "iteration.cpp", "prms.cpp", "bse.hip", "phir.hip", "eqmatrix.hip",
+           "crossing.hip", "vertex_ops.hip", "bse_split.hip", "bse_closed.hip", "chi.hip", "sde.hip", "gtau.hip",
+           "conv.hip"]
 
 # Packed FP32 instructions are disabled for every device compile.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

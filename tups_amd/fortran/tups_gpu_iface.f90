@@ -35,6 +35,14 @@ module tups_gpu_iface
     real(c_double) :: f_max(4), f_rel_err(4)
   end type tups_conv_info
 
+  type, bind(C) :: tups_prms
+    real(c_double) :: beta, u, n_particle
+    integer(c_int32_t) :: nx, nf, neighbours, readinpick, ngrain, f_range, ite_max, dga
+    integer(c_int32_t) :: use_sig_dmft, use_u2, save_f, oneshot, old, sde_old
+    real(c_double) :: f_damping
+    character(kind=c_char) :: pick_file(104), fname_l(104), fname_sig_dmft(104)
+  end type tups_prms
+
   type, bind(C) :: tups_sizes
     integer(c_int32_t) :: nx, ny, nf, nl, nz, nr, nw, nibz, nq, nc, full_ibz, reserved
     real(c_double) :: alpha
@@ -267,6 +275,21 @@ module tups_gpu_iface
       type(tups_conv_info), intent(out) :: info
       integer(c_int) :: rc
     end function tups_conv
+
+    ! set_parameters (SRC/parquet_ini.f90:289): path is a NUL-terminated file name
+    function tups_read_prms(path, prms) bind(C, name="tups_read_prms") result(rc)
+      import :: c_char, c_int, tups_prms
+      character(kind=c_char), intent(in) :: path(*)
+      type(tups_prms), intent(out) :: prms
+      integer(c_int) :: rc
+    end function tups_read_prms
+
+    function tups_prms_dims(prms, dims) bind(C, name="tups_prms_dims") result(rc)
+      import :: c_int, tups_prms, tups_dims
+      type(tups_prms), intent(in) :: prms
+      type(tups_dims), intent(out) :: dims
+      integer(c_int) :: rc
+    end function tups_prms_dims
 
     function tups_get_mu(ctx, mu, steps) bind(C, name="tups_get_mu") result(rc)
       import :: c_ptr, c_int32_t, c_int, c_double

@@ -43,6 +43,14 @@ class Dims(C.Structure):
                 ("f_damping", C.c_double)]
 
 
+class Prms(C.Structure):
+    _fields_ = ([("beta", C.c_double), ("u", C.c_double), ("n_particle", C.c_double)]
+                + [(n, C.c_int32) for n in ("nx", "nf", "neighbours", "readinpick", "ngrain", "f_range", "ite_max",
+                                            "dga", "use_sig_dmft", "use_u2", "save_f", "oneshot", "old", "sde_old")]
+                + [("f_damping", C.c_double), ("pick_file", C.c_char * 104), ("fname_l", C.c_char * 104),
+                   ("fname_sig_dmft", C.c_char * 104)])
+
+
 class Sizes(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("nx", "ny", "nf", "nl", "nz", "nr", "nw", "nibz", "nq",
                                          "nc", "full_ibz", "reserved")] + [("alpha", C.c_double)]
@@ -82,6 +90,8 @@ SIGNATURES = {
     "tups_gkw_grt": ([_V, C.c_double, C.c_double, C.c_int32, _D, C.POINTER(C.c_int32), _D], C.c_int),
     "tups_get_mu": ([_V, _D, C.POINTER(C.c_int32)], C.c_int),
     "tups_conv": ([_V, _D, _D, C.c_double, C.c_double, _P(ConvInfo)], C.c_int),
+    "tups_read_prms": ([C.c_char_p, _P(Prms)], C.c_int),
+    "tups_prms_dims": ([_P(Prms), _P(Dims)], C.c_int),
     "tups_iteration": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_batch_create": ([C.c_int32, C.c_int32, _I, _I, C.c_double, C.c_double, _P(_V)], C.c_int),
     "tups_batch_destroy": ([_V], C.c_int),
@@ -130,6 +140,34 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
     if path is None:
         _lib = lib
     return lib
+
+
+_PRMS_BOOL = ("readinpick", "dga", "use_sig_dmft", "use_u2", "save_f", "oneshot", "old", "sde_old")
+
+
+def read_prms(path) -> dict:
+    """The reference's run-parameter file (set_parameters, SRC/parquet_ini.f90:289-329) as a
+    dict of its 21 values, plus the context sizes it implies under "dims" (nx, nf, nl, beta,
+    f_damping; an unsupported neighbours value leaves "dims" as None with the reason)."""
+    lib = load_library()
+    p = Prms()
+    _check(lib.tups_read_prms(str(path).encode(), C.byref(p)), "tups_read_prms")
+    out = {}
+    for name, _ in Prms._fields_:
+        v = getattr(p, name)
+        if isinstance(v, bytes):
+            v = v.decode()
+        elif name in _PRMS_BOOL:
+            v = bool(v)
+        out[name] = v
+    d = Dims()
+    rc = lib.tups_prms_dims(C.byref(p), C.byref(d))
+    if rc == 0:
+        out["dims"] = {"nx": d.nx, "nf": d.nf, "nl": d.nl, "beta": d.beta, "f_damping": d.f_damping}
+    else:
+        out["dims"] = None
+        out["dims_error"] = lib.tups_last_error().decode()
+    return out
 
 
 def _check(rc: int, where: str):
