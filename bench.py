@@ -141,7 +141,7 @@ def config1_cpu():
     phir = [P.calc_phir(lat, p) for p in phi]
     F = P.calculate_pae(lat, phi, lam, phir, mt)
     S.calc_sde(lat, phi, (lam[0], lam[1]), sigma, sh, mu, 2.0, 2.0, use_u2=True) + T.order_2nd(lat, grt, 2.0, 2.0)
-    [B.gamma_update(f, p) for f, p in zip(F, phi)]
+    B.gamma_update(F, phi)
     el = time.perf_counter() - t0
     return {"ms_per_iteration": round(el * 1e3, 1), "cores": 1, "kind": "port",
             "workload": CONFIG1_WORKLOAD + "; oracle restatement in numpy"}
@@ -203,6 +203,8 @@ def large_share(moduli):
 
 
 def main():
+    import numpy as np
+
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU q-sharding "
@@ -291,6 +293,23 @@ def main():
               "unit_roofline": "TFLOP/s", "frac": round(cl_flops / (cl_step_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
               "flops_per_call": cl_flops, "kernel": "closed_build + closed_solve"}
     print(f"[bench] closed form: {cl_step_ms:.1f} ms per call", file=sys.stderr, flush=True)
+    # --- convergence check: max |F_r| over the four resident vertices (HBM read) -----------
+    from tups_amd.runtime import K_CONV
+    nt = nx * nx * nf
+    s_new = np.ones(nt, dtype=np.complex128)
+    ctx.conv(sigma_old=s_new, sigma=s_new)
+    ctx.set_timing(True)
+    for _ in range(5):
+        ctx.conv(sigma_old=s_new, sigma=s_new)
+    cv_ms, cv_n = ctx.kernel_time(K_CONV)
+    ctx.set_timing(False)
+    cv_bytes = 4.0 * D * D * nq * 16
+    conv = {"ms_per_call": round(cv_ms / cv_n, 4), "bytes": cv_bytes,
+            "achieved": round(cv_bytes / (cv_ms / cv_n / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(cv_bytes / (cv_ms / cv_n / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernel": "conv_sigma + conv_fmax + conv_final",
+            "workload": f"conv() over the four resident F (D={D}, Nq={nq}) and a {nt}-point Sigma pair"}
+    print(f"[bench] conv: {cv_ms / cv_n:.3f} ms per call", file=sys.stderr, flush=True)
     ctx.close()
 
     # split error per channel vs the FP64 form, on 64 q of the same workload
@@ -301,7 +320,6 @@ def main():
     batched_bse_inputs(ec, U=U, seed=5)
     ec.set_split(args.moduli, 0)
     ec.BSE(0, mode=BSE_LADDER_SPLIT)
-    import numpy as np
     errs = []
     for r in range(4):
         got = ec.download(GAMMA, r)
@@ -486,6 +504,7 @@ def main():
         "large_share": large,
         "sde": sde,
         "gtau": gtau,
+        "conv": conv,
         "config1_iteration": config1,
         "cpu_baseline": cpu,
     }

@@ -60,7 +60,8 @@ def build(verbose: bool = False) -> Path:
 
 
 def build_fortran(verbose: bool = False) -> Path:
-    """Compile the Fortran ISO_C_BINDING interface + drop-in modules and the driver."""
+    """Compile the Fortran ISO_C_BINDING interface + drop-in modules, the one-iteration driver
+    and the whole-run program tups_run."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     fdir = PKG / "fortran"
     out = BUILD / "fortran"
@@ -81,6 +82,19 @@ def build_fortran(verbose: bool = False) -> Path:
         raise RuntimeError(f"Fortran link failed:\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {exe}")
+    # the whole run from a parameter file
+    obj = out / "tups_run.o"
+    r = subprocess.run([fc, "-O2", "-J", str(out), "-c", str(fdir / "tups_run.f90"), "-o", str(obj)],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed for tups_run.f90:\n{r.stdout}\n{r.stderr}")
+    run = PKG / "tups_run"
+    r = subprocess.run([fc, str(out / "tups_gpu_iface.o"), str(obj), f"-L{PKG}", "-ltups_gpu", "-Wl,-rpath,$ORIGIN",
+                        "-o", str(run)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran link failed for tups_run:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {run}")
     return exe
 
 
