@@ -233,6 +233,18 @@ typedef struct tups_conv_info {
 } tups_conv_info;
 int tups_conv(tups_ctx* ctx, const double* sigma_old, const double* sigma, double f_sigma, double f_vert,
               tups_conv_info* info);
+/* calc_sus (SRC/parquet_sus.f90:18-362) from the resident vertices and the
+ * Sigma and mu on the device: sus_out receives sus_X_IBZ(q, ff) for X = d, m,
+ * s, t, as [X][ff][q] complex (4 x 4 x Nq; ff the external form factors 1,
+ * cos kx - cos ky, sin kx, sin kx sin ky of fill_FF_sdwave,
+ * SRC/parquet_formfactors.f90:792-815).  u the Hubbard U of the bare vertex
+ * outside the box; f_range the enlarged box; after_update = 0 when the GAMMA
+ * arrays hold Phi (the reference calls calc_sus before G = F - G,
+ * SRC/loop.f90:193-222), 1 when they hold Gamma = F - Phi (after
+ * tups_iteration_full).  The reference precedes it with calc_Gkw_Grt(cal_chi):
+ * tups_gkw_grt with cal_chi != 0 sets that mu and Sigma_H.  Needs a full-IBZ
+ * context, Nx <= 16.  Synchronous. */
+int tups_calc_sus(tups_ctx* ctx, double u, int32_t f_range, int32_t after_update, double* sus_out);
 /* The loop body of SRC/loop.f90:67-222 on the device (no counterpart as one
  * call): calc_Gkw_Grt's mu search when n_particle >= 0 (else mu stays as set)
  * -> calc_chi from the device Sigma -> BSE(mode) -> calc_PhiR -> calculate_PAE
@@ -311,7 +323,8 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 #define TUPS_K_SDE 11         /* self-energy: auxiliary bubbles, vertex contraction, star sums */
 #define TUPS_K_GTAU 12        /* G(tau): mu bisection, fine-grid G(k, tau), 2D transform */
 #define TUPS_K_CONV 13        /* convergence check: Sigma change, vertex maxima */
-#define TUPS_K_COUNT 14
+#define TUPS_K_SUS 14         /* susceptibilities: Green's-function pairs and their contraction */
+#define TUPS_K_COUNT 15
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);
 /* scans the resident array for NaN/Inf; returns TUPS_ENONFINITE if any */

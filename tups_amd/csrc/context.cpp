@@ -293,6 +293,7 @@ int tups_destroy(tups_ctx* c) {
   f(c->sde_ws);
   f(c->gt_ws);
   f(c->conv_ws);
+  f(c->sus_ws);
   f(c->d_sigma_old);
   f(c->d_gstate);
   f(c->closed_ws);

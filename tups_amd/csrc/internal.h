@@ -127,6 +127,8 @@ struct tups_ctx {
   bool grt_valid = false;        // Grt of the last calc_Gkw_Grt is in the workspace
   double2* d_sigma_old = nullptr;  // Sigma before the last device SDE update (SigmaOld of conv)
   bool sigma_old_valid = false;
+  void* sus_ws = nullptr;        // susceptibilities: Green's-function pairs, result
+  size_t sus_ws_bytes = 0;
   void* conv_ws = nullptr;       // convergence check: host Sigma pair, partials, result
   double conv_fmax_prev[4] = {};  // max |F_r| of the previous convergence check
   double full_n_particle = -1.0; // full-iteration graphs: nParticle and Ngrain they were captured with

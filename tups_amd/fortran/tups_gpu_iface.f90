@@ -22,7 +22,7 @@ module tups_gpu_iface
                                    TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4, TUPS_K_PAE_GATHER = 5, &
                                    TUPS_K_SPLIT_PREP = 6, TUPS_K_SPLIT_GEMM = 7, TUPS_K_BSE_CLOSED = 8, &
                                    TUPS_K_CALC_CHI = 9, TUPS_K_BSE_BATCH = 10, TUPS_K_SDE = 11, &
-                                   TUPS_K_GTAU = 12, TUPS_K_CONV = 13
+                                   TUPS_K_GTAU = 12, TUPS_K_CONV = 13, TUPS_K_SUS = 14
 
   type, bind(C) :: tups_dims
     integer(c_int32_t) :: nx, nf, nl, nr, nq, reserved
@@ -275,6 +275,16 @@ module tups_gpu_iface
       type(tups_conv_info), intent(out) :: info
       integer(c_int) :: rc
     end function tups_conv
+
+    ! calc_sus (SRC/parquet_sus.f90:18): sus(Nq, 4, 4) = sus_X_IBZ(q, ff) for X = d, m, s, t
+    function tups_calc_sus(ctx, u, f_range, after_update, sus) bind(C, name="tups_calc_sus") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), value :: u
+      integer(c_int32_t), value :: f_range, after_update
+      complex(c_double_complex), intent(out) :: sus(*)
+      integer(c_int) :: rc
+    end function tups_calc_sus
 
     ! set_parameters (SRC/parquet_ini.f90:289): path is a NUL-terminated file name
     function tups_read_prms(path, prms) bind(C, name="tups_read_prms") result(rc)

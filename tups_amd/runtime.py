@@ -21,7 +21,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
 
 GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
 (K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER, K_SPLIT_PREP, K_SPLIT_GEMM,
- K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH, K_SDE, K_GTAU, K_CONV) = range(14)
+ K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH, K_SDE, K_GTAU, K_CONV, K_SUS) = range(15)
 CH_D, CH_M, CH_S, CH_T = range(4)
 BSE_LADDER_FP64 = 0
 BSE_LADDER_SPLIT = 1
@@ -91,6 +91,7 @@ SIGNATURES = {
     "tups_get_mu": ([_V, _D, C.POINTER(C.c_int32)], C.c_int),
     "tups_conv": ([_V, _D, _D, C.c_double, C.c_double, _P(ConvInfo)], C.c_int),
     "tups_read_prms": ([C.c_char_p, _P(Prms)], C.c_int),
+    "tups_calc_sus": ([_V, C.c_double, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_prms_dims": ([_P(Prms), _P(Dims)], C.c_int),
     "tups_iteration": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_batch_create": ([C.c_int32, C.c_int32, _I, _I, C.c_double, C.c_double, _P(_V)], C.c_int),
@@ -371,6 +372,13 @@ class TupsContext:
         return {"sigma_avg": info.sigma_avg, "sigma_max": info.sigma_max, "kx": info.kx, "ky": info.ky,
                 "nu": info.nu, "converged": bool(info.converged), "f_max": list(info.f_max),
                 "f_rel_err": list(info.f_rel_err)}
+
+    def calc_sus(self, U: float, f_range: int = 0, after_update: bool = False) -> np.ndarray:
+        """calc_sus (SRC/parquet_sus.f90:18-362): (4 channels d, m, s, t; 4 form factors; Nq)."""
+        out = np.zeros((4, 4, self.nq), dtype=np.complex128)
+        _check(self.lib.tups_calc_sus(self._h, float(U), int(f_range), 1 if after_update else 0, _dptr(out)),
+               "tups_calc_sus")
+        return out
 
     def get_mu(self):
         """(mu, steps) of the device search state."""
