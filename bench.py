@@ -424,6 +424,21 @@ def main():
            "kernel": "sde_chi_aux + sde_vchi + sde_sigma (FP64)",
            "workload": f"calc_SDE without use_U2, Nx=8 Nf=64 Nl=5, Nq={sc.nq} (all IBZ q), f_range=0"}
     print(f"[bench] sde: {sde_ms / sde_n:.2f} ms per call", file=sys.stderr, flush=True)
+    # calc_sus on the same context (F and Phi resident, Sigma = 0 on the device)
+    from tups_amd.runtime import K_SUS
+    sc.set_sigma(sig, 0.0)
+    sc.calc_sus(U)
+    sc.set_timing(True)
+    for _ in range(3):
+        sc.calc_sus(U)
+    sus_ms, sus_n = sc.kernel_time(K_SUS)
+    sc.set_timing(False)
+    nent = sc.nz
+    sus_flops = 4.0 * sc.nq * nent * nent * 4 * 8  # 4 channels x Nq x D^2 entries x 4 form factors, complex MAC
+    sus = {"ms_per_call": round(sus_ms / sus_n, 3), "kernel": "sus_gg + sus_contract (FP64)",
+           "fp64_tflops": round(sus_flops / (sus_ms / sus_n / 1e3) / 1e12, 2),
+           "workload": f"calc_sus, Nx=8 Nf=64 Nl=5, Nq={sc.nq}, f_range=0, 4 external form factors"}
+    print(f"[bench] sus: {sus_ms / sus_n:.2f} ms per call", file=sys.stderr, flush=True)
     sc.close()
 
     # calc_Gkw_Grt: mu bisection + G(k, tau) + 2D transform on a coarse-grained grid
@@ -503,6 +518,7 @@ def main():
         "batching": batching,
         "large_share": large,
         "sde": sde,
+        "sus": sus,
         "gtau": gtau,
         "conv": conv,
         "config1_iteration": config1,
