@@ -100,12 +100,14 @@ __global__ __launch_bounds__(256) void conv_fmax(ConvArgs a) {
   }
 }
 
-// one workgroup: the partials in block order
-__global__ __launch_bounds__(64) void conv_final(ConvArgs a) {
-  const int t = threadIdx.x;
-  if (t == 0) {
+// one workgroup of five waves: wave 0 the Sigma partials, wave 1 + r channel r's;
+// lanes take the partials strided by 64, then a butterfly over the wave (a sum's
+// partners add the same two values, so every lane holds the same total)
+__global__ __launch_bounds__(320) void conv_final(ConvArgs a) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wave == 0) {
     double sum = 0.0, mx = -1.0, ix = 0.0;
-    for (int b = 0; b < kConvBlocksS; ++b) {
+    for (int b = lane; b < kConvBlocksS; b += 64) {
       sum += a.s_part[3 * b];
       const double m = a.s_part[3 * b + 1], i = a.s_part[3 * b + 2];
       if (m > mx || (m == mx && i < ix)) {
@@ -113,19 +115,37 @@ __global__ __launch_bounds__(64) void conv_final(ConvArgs a) {
         ix = i;
       }
     }
-    a.out[0] = sum;
-    a.out[1] = mx;
-    a.out[2] = ix;
-  } else if (t <= 4) {
-    const double* p = a.f_part + 3 * (size_t)(t - 1) * kConvBlocksF;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double so = __shfl_xor(sum, off), mo = __shfl_xor(mx, off), io = __shfl_xor(ix, off);
+      sum += so;
+      if (mo > mx || (mo == mx && io < ix)) {
+        mx = mo;
+        ix = io;
+      }
+    }
+    if (lane == 0) {
+      a.out[0] = sum;
+      a.out[1] = mx;
+      a.out[2] = ix;
+    }
+  } else {
+    const double* p = a.f_part + 3 * (size_t)(wave - 1) * kConvBlocksF;
     double best = -1.0, re = 0.0, im = 0.0;
-    for (int b = 0; b < kConvBlocksF; ++b)
+    for (int b = lane; b < kConvBlocksF; b += 64)
       if (p[3 * b] > best) {
         best = p[3 * b];
         re = p[3 * b + 1];
         im = p[3 * b + 2];
       }
-    a.out[2 + t] = hypot(re, im);
+    for (int off = 32; off > 0; off >>= 1) {
+      const double bo = __shfl_xor(best, off), ro = __shfl_xor(re, off), io = __shfl_xor(im, off);
+      if (bo > best) {
+        best = bo;
+        re = ro;
+        im = io;
+      }
+    }
+    if (lane == 0) a.out[2 + wave] = hypot(re, im);
   }
 }
 
@@ -169,7 +189,7 @@ extern "C" int tups_conv(tups_ctx* c, const double* sigma_old, const double* sig
   TUPS_HIP(hipGetLastError());
   hipLaunchKernelGGL(conv_fmax, dim3(kConvBlocksF, 4), dim3(256), 0, c->stream, a);
   TUPS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(conv_final, dim3(1), dim3(64), 0, c->stream, a);
+  hipLaunchKernelGGL(conv_final, dim3(1), dim3(320), 0, c->stream, a);
   TUPS_HIP(hipGetLastError());
   if (int rc = timer_end(c, TUPS_K_CONV)) return rc;
   double out[7];

@@ -109,7 +109,8 @@ def batch_flops(b: TupsBatch) -> float:
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["bse", "split", "crossing", "phir", "calib", "batch", "gtau", "config1"])
+    ap.add_argument("workload", choices=["bse", "split", "crossing", "phir", "calib", "batch", "gtau", "config1",
+                                         "conv", "sus"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--nx", type=int, default=16)
     ap.add_argument("--nf", type=int, default=64)
@@ -117,6 +118,32 @@ def main(argv=None):
     ap.add_argument("--moduli", type=int, default=12)
     ap.add_argument("--variant", default="", help="batch: tile variant by name (default: autotune)")
     args = ap.parse_args(argv)
+    if args.workload == "sus":
+        # calc_sus on Nx=8, Nf=64, Nl=5 (all 480 IBZ q): random F and Phi, Sigma = 0
+        c = TupsContext(nx=8, nf=64, nl=5, beta=2.0, f_damping=0.5)
+        rng = np.random.default_rng(5)
+        for r in range(4):
+            for which in (F, GAMMA):
+                c.upload(which, r, 0.1 * (rng.standard_normal((c.nz, c.nz, c.nq))
+                                          + 1j * rng.standard_normal((c.nz, c.nz, c.nq))))
+        c.set_sigma(np.zeros(8 * 8 * 64, dtype=np.complex128), 0.0)
+        for _ in range(args.steps):
+            c.calc_sus(2.0)
+        print(f"sus: {args.steps} calls, Nq={c.nq}, D={c.nz}")
+        c.close()
+        sys.stdout.flush()
+        return
+    if args.workload == "conv":
+        # conv over the four resident F of the D=320, Nq=1024 batched workload
+        c = TupsContext(nx=8, nf=64, nl=5, nq=1024, beta=2.0, f_damping=0.5)
+        batched_bse_inputs(c)
+        s = np.ones(8 * 8 * 64, dtype=np.complex128)
+        for _ in range(args.steps):
+            c.conv(sigma_old=s, sigma=s)
+        print(f"conv: {args.steps} calls, F bytes per call {4.0 * c.nz * c.nz * c.nq * 16:.6e}")
+        c.close()
+        sys.stdout.flush()
+        return
     if args.workload == "gtau":
         # calc_Gkw_Grt: Nx=8, Ngrain=4 (32 x 32 fine grid), Nf=64, random Sigma, nParticle 0.9
         c = TupsContext(nx=8, nf=64, nl=1, beta=2.0, f_damping=0.5)
