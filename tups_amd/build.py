@@ -59,6 +59,28 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+HDF5_ROOT = Path("/opt/conda")
+H5_LIB = PKG / "libtups_h5.so"
+
+
+def build_h5(verbose: bool = False):
+    """libtups_h5.so: the HDF5 vertex dump (host C++ over the HDF5 C library and
+    libtups_gpu.so).  Skipped, with a message, where the HDF5 headers are absent."""
+    if not (HDF5_ROOT / "include" / "hdf5.h").exists() or not (HDF5_ROOT / "lib" / "libhdf5.so").exists():
+        if verbose:
+            print(f"HDF5 not found under {HDF5_ROOT}: libtups_h5.so not built")
+        return None
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}",
+           f"-I{HDF5_ROOT / 'include'}", str(CSRC / "vertex_h5.cpp"), f"-L{PKG}", "-ltups_gpu",
+           f"-L{HDF5_ROOT / 'lib'}", "-lhdf5", f"-Wl,-rpath,$ORIGIN:{HDF5_ROOT / 'lib'}", "-o", str(H5_LIB)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"libtups_h5.so build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {H5_LIB}")
+    return H5_LIB
+
+
 def build_fortran(verbose: bool = False) -> Path:
     """Compile the Fortran ISO_C_BINDING interface + drop-in modules, the one-iteration driver
     and the whole-run program tups_run."""

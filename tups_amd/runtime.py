@@ -143,6 +143,41 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
     return lib
 
 
+H5_LIB_PATH = Path(__file__).resolve().parent / "libtups_h5.so"
+_h5 = None
+
+
+def load_h5_library() -> C.CDLL:
+    """libtups_h5.so (the HDF5 vertex dump), loaded after libtups_gpu.so."""
+    global _h5
+    if _h5 is None:
+        load_library()
+        if not H5_LIB_PATH.exists():
+            raise RuntimeError(f"{H5_LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(needs the HDF5 C library)")
+        lib = C.CDLL(str(H5_LIB_PATH))
+        lib.tups_h5_write_vertex.argtypes = [C.c_char_p, C.c_char_p, _D, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        lib.tups_h5_write_vertex.restype = C.c_int
+        lib.tups_h5_write_f.argtypes = [_V, C.c_char_p]
+        lib.tups_h5_write_f.restype = C.c_int
+        lib.tups_h5_last_error.argtypes = []
+        lib.tups_h5_last_error.restype = C.c_char_p
+        _h5 = lib
+    return _h5
+
+
+def h5_write_vertex(path, dset: str, data, create: bool = True) -> None:
+    """hdf5_write_data of one (Nz, Nz, Nb) vertex as reshape(., (Nf, Nl, Nf, Nl, Nb)); data is
+    passed with its (nf, nl, nb) as a (nf, nl, nf, nl, nb) Fortran-order array."""
+    a = np.asfortranarray(np.asarray(data, dtype=np.complex128))
+    assert a.ndim == 5 and a.shape[0] == a.shape[2] and a.shape[1] == a.shape[3]
+    lib = load_h5_library()
+    rc = lib.tups_h5_write_vertex(str(path).encode(), dset.encode(), a.ctypes.data_as(_D), a.shape[0], a.shape[1],
+                                  a.shape[4], 1 if create else 0)
+    if rc != 0:
+        raise TupsError(rc, "tups_h5_write_vertex", lib.tups_h5_last_error().decode())
+
+
 _PRMS_BOOL = ("readinpick", "dga", "use_sig_dmft", "use_u2", "save_f", "oneshot", "old", "sde_old")
 
 
@@ -379,6 +414,14 @@ class TupsContext:
         _check(self.lib.tups_calc_sus(self._h, float(U), int(f_range), 1 if after_update else 0, _dptr(out)),
                "tups_calc_sus")
         return out
+
+    def write_f(self, path) -> None:
+        """write_F (SRC/parquet_plot.f90:237-276): the four resident F_r into F/F_d .. F/F_t of a
+        new HDF5 file (libtups_h5.so)."""
+        h5 = load_h5_library()
+        rc = h5.tups_h5_write_f(self._h, str(path).encode())
+        if rc != 0:
+            raise TupsError(rc, "tups_h5_write_f", h5.tups_h5_last_error().decode())
 
     def get_mu(self):
         """(mu, steps) of the device search state."""
