@@ -342,13 +342,14 @@ def main():
     gat_ms, gat_n = cx.kernel_time(K_PAE_GATHER)
     cx.set_timing(False)
     cbytes = crossing_bytes(cx)
-    # one calculate_PAE = PhiR re-index pass + streaming crossing; both counted
+    # one calculate_PAE = PhiR re-index pass (unless folded into the variant) + streaming crossing
     per_call_ms = (pae_ms + gat_ms) / pae_n
     gbs = cbytes / (per_call_ms / 1e3) / 1e9
     crossing = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4),
                 "traffic": profile_traffic("crossing_pae_nl1_nx16_nf64.md"),
-                "kernel": f"pae_stream_nl1 variant {cvar} (+ pae_pregather_nl1)",
+                "kernel": f"pae_stream_nl1 variant {cvar}" + (" (PhiR re-index folded in)" if cvar.startswith("nl1_f")
+                                                               else " (+ pae_pregather_nl1)"),
                 "avg_launch_ms": round(pae_ms / pae_n, 4), "pregather_ms": round(gat_ms / max(1, gat_n), 4),
                 "bytes_per_call": cbytes,
                 "workload": f"calculate_PAE Nx=16 Nf=64 Nl=1 NR=1, Nq={cx.nq} (all IBZ q), 4 channels"}

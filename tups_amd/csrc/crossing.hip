@@ -87,21 +87,11 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
 // then reads Y contiguously: F = Phi + Lambda Nc + (1/Nx) sum_R M(Q,R) (Y_a + Y_b).
 
 // grid: ((per + 255) / 256, 8): blockIdx.y = k, one output entry per thread
-__global__ __launch_bounds__(256) void pae_pregather_nl1(const double2* __restrict__ pd,
-                                                         const double2* __restrict__ pm,
-                                                         const double2* __restrict__ ps,
-                                                         const double2* __restrict__ pt,
-                                                         double2* __restrict__ Y, int nf, int nw,
-                                                         int nr) {
+// entry (nu1, nu2, qw, R) of Y slice k (1-based frequencies; pd..pt the PhiR of d, m, s, t)
+__device__ __forceinline__ double2 y_entry(int k, int nu1, int nu2, int qw, int R, int nf, int nw,
+                                           const double2* __restrict__ pd, const double2* __restrict__ pm,
+                                           const double2* __restrict__ ps, const double2* __restrict__ pt) {
   const size_t nf2 = (size_t)nf * nf;
-  const size_t per = nf2 * nw * nr;  // one k slice
-  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= per) return;
-  const int k = blockIdx.y;  // 2 * routine + target of the pair
-  const int nu1 = (int)(gid % nf) + 1;
-  const int nu2 = (int)((gid / nf) % nf) + 1;
-  const int qw = (int)((gid / nf2) % nw) + 1;
-  const int R = (int)(gid / (nf2 * nw));
   const size_t rb = nf2 * (size_t)nw * R;  // PhiR(nu_a, nu_b, w2, R) = rb + (w2-1)*nf2 + (nub-1)*nf + nua-1
   const int dn = nu2 - nu1;
   const bool pb_plain = dn >= 0 && dn + 1 <= nw;
@@ -129,10 +119,29 @@ __global__ __launch_bounds__(256) void pae_pregather_nl1(const double2* __restri
   const size_t i = rb + (size_t)(t.w2 - 1) * nf2 + (size_t)(t.nub - 1) * nf + (t.nua - 1);
   const double2 x0 = p0[i], x1 = p1[i];
   const double im = t.tr ? -1.0 : 1.0;
-  Y[(size_t)k * per + gid] = make_double2(t.wt * (c0 * x0.x + c1 * x1.x), t.wt * im * (c0 * x0.y + c1 * x1.y));
+  return make_double2(t.wt * (c0 * x0.x + c1 * x1.x), t.wt * im * (c0 * x0.y + c1 * x1.y));
+}
+
+__global__ __launch_bounds__(256) void pae_pregather_nl1(const double2* __restrict__ pd,
+                                                         const double2* __restrict__ pm,
+                                                         const double2* __restrict__ ps,
+                                                         const double2* __restrict__ pt,
+                                                         double2* __restrict__ Y, int nf, int nw,
+                                                         int nr) {
+  const size_t nf2 = (size_t)nf * nf;
+  const size_t per = nf2 * nw * nr;  // one k slice
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= per) return;
+  const int k = blockIdx.y;  // 2 * routine + target of the pair
+  const int nu1 = (int)(gid % nf) + 1;
+  const int nu2 = (int)((gid / nf) % nf) + 1;
+  const int qw = (int)((gid / nf2) % nw) + 1;
+  const int R = (int)(gid / (nf2 * nw));
+  Y[(size_t)k * per + gid] = y_entry(k, nu1, nu2, qw, R, nf, nw, pd, pm, ps, pt);
 }
 
 struct PaeNl1Args {
+  const double2* phir[4];  // PhiR of d, m, s, t (the fused variants re-index them in place)
   const double2* phi[4];
   double2* F[4];
   const double2* lam[4];
@@ -274,6 +283,61 @@ __global__ __launch_bounds__(256) void pae_stream_nl1_mom(PaeNl1Args a) {
   }
 }
 
+// pae_stream_nl1_mom with the re-index pass folded in: each thread forms its four
+// Y entries from PhiR itself (y_entry) instead of reading the pre-pass output;
+// the entries of one (nu1, nu2, qw) are recomputed by every momentum group,
+// which reads PhiR through the cache in place of the pre-pass's write and read
+template <int KM>
+__global__ __launch_bounds__(256) void pae_stream_nl1_fused(PaeNl1Args a) {
+  const int nf = a.nf, nw = a.nw;
+  const size_t nf2 = (size_t)nf * nf;
+  const int tx = (nf + 63) >> 6, ty = (nf + 3) >> 2;
+  const int ngm = (a.nibz + KM - 1) / KM;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = logical % (tx * ty);
+  int rest = logical / (tx * ty);
+  const int mg = rest % ngm;
+  rest /= ngm;
+  const int pair = rest & 1;
+  const int w0 = rest >> 1;
+  const int nu1 = (tile % tx) * 64 + (threadIdx.x & 63);
+  const int nu2 = (tile / tx) * 4 + (threadIdx.x >> 6);
+  if (nu1 >= nf || nu2 >= nf) return;
+  const int ta = pair == 0 ? 0 : 2, tb = ta + 1;
+  const int ka = pair == 0 ? 0 : 4;
+  const double2 *pd = a.phir[0], *pm = a.phir[1], *ps = a.phir[2], *pt = a.phir[3];
+  const double2 ya0 = y_entry(ka + 0, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+  const double2 yb0 = y_entry(ka + 1, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+  const double2 ya1 = y_entry(ka + 2, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+  const double2 yb1 = y_entry(ka + 3, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+  const size_t li = (size_t)nu1 + (size_t)nf * ((size_t)nu2 + (size_t)nf * w0);
+  const double2 la = a.lam[ta][li], lb = a.lam[tb][li];
+  const double2* Ma = a.M[pair == 0 ? 0 : 2];
+  const double2* Mb = a.M[pair == 0 ? 1 : 3];
+  double2 fa[KM], fb[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const int m = mg * KM + k < a.nibz ? mg * KM + k : a.nibz - 1;
+    const size_t idx = (size_t)nu1 + (size_t)nf * nu2 + nf2 * ((size_t)m * nw + w0);
+    fa[k] = ld_stream(a.phi[ta] + idx);
+    fb[k] = ld_stream(a.phi[tb] + idx);
+  }
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const int m = mg * KM + k;
+    if (m >= a.nibz) break;
+    const double2 m0 = Ma[m], m1 = Mb[m];
+    double2 oa = fa[k], ob = fb[k];
+    oa.x += la.x * a.nc_aux + (m0.x * ya0.x - m0.y * ya0.y + m1.x * ya1.x - m1.y * ya1.y) * a.inv_nx;
+    oa.y += la.y * a.nc_aux + (m0.x * ya0.y + m0.y * ya0.x + m1.x * ya1.y + m1.y * ya1.x) * a.inv_nx;
+    ob.x += lb.x * a.nc_aux + (m0.x * yb0.x - m0.y * yb0.y + m1.x * yb1.x - m1.y * yb1.y) * a.inv_nx;
+    ob.y += lb.y * a.nc_aux + (m0.x * yb0.y + m0.y * yb0.x + m1.x * yb1.y + m1.y * yb1.x) * a.inv_nx;
+    const size_t idx = (size_t)nu1 + (size_t)nf * nu2 + nf2 * ((size_t)m * nw + w0);
+    st_stream(a.F[ta] + idx, oa);
+    st_stream(a.F[tb] + idx, ob);
+  }
+}
+
 typedef void (*PaeKernel)(PaeNl1Args);
 struct PaeVariant {
   int cols;   // columns per thread (column kernels)
@@ -281,6 +345,7 @@ struct PaeVariant {
   bool nr1;
   PaeKernel fn;
   const char* name;
+  bool fused = false;  // forms the re-indexed PhiR itself (no pre-pass)
 };
 static const PaeVariant kPaeVariants[] = {
     {2, 0, true, pae_stream_nl1<2, true>, "nl1_c2"},
@@ -291,6 +356,9 @@ static const PaeVariant kPaeVariants[] = {
     {1, 2, true, pae_stream_nl1_mom<2>, "nl1_m2"},
     {1, 4, true, pae_stream_nl1_mom<4>, "nl1_m4"},
     {1, 8, true, pae_stream_nl1_mom<8>, "nl1_m8"},
+    {1, 16, true, pae_stream_nl1_mom<16>, "nl1_m16"},
+    {1, 8, true, pae_stream_nl1_fused<8>, "nl1_f8", true},
+    {1, 16, true, pae_stream_nl1_fused<16>, "nl1_f16", true},
 };
 constexpr int kNumPaeVariants = sizeof(kPaeVariants) / sizeof(kPaeVariants[0]);
 
@@ -316,6 +384,7 @@ static PaeNl1Args nl1_args(tups_ctx* c) {
   const tups_sizes& s = c->sz;
   PaeNl1Args b{};
   for (int r = 0; r < 4; ++r) {
+    b.phir[r] = c->phir[r];
     b.phi[r] = c->gamma[r];
     b.F[r] = c->fvert[r];
     b.lam[r] = c->lam[r];
@@ -343,7 +412,8 @@ static int pae_pregather(tups_ctx* c) {
 }
 
 // Times the usable streaming variants on the resident inputs (the output F is
-// rewritten identically each time) and keeps the fastest.
+// rewritten identically each time), each with the re-index pre-pass it needs,
+// and keeps the fastest.
 int autotune_pae(tups_ctx* c, int reps, int* chosen, float* best_ms) {
   TUPS_CHECK(c->sz.nl == 1 && c->pae_y, TUPS_EINVAL, "crossing variants exist for Nl = 1 only");
   if (int rc = pae_pregather(c)) return rc;
@@ -352,10 +422,14 @@ int autotune_pae(tups_ctx* c, int reps, int* chosen, float* best_ms) {
   int arg = -1;
   for (int v = 0; v < kNumPaeVariants; ++v) {
     if (!pae_variant_ok(c, v)) continue;
+    const bool pre = !kPaeVariants[v].fused;  // the pre-pass counts for the variants that read it
     if (int rc = pae_nl1_launch(c, v, b)) return rc;
     TUPS_HIP(hipEventRecord(c->tev[0], c->stream));
-    for (int i = 0; i < reps; ++i)
+    for (int i = 0; i < reps; ++i) {
+      if (pre)
+        if (int rc = pae_pregather(c)) return rc;
       if (int rc = pae_nl1_launch(c, v, b)) return rc;
+    }
     TUPS_HIP(hipEventRecord(c->tev[1], c->stream));
     TUPS_HIP(hipEventSynchronize(c->tev[1]));
     float ms = 0.f;
@@ -601,9 +675,11 @@ int launch_pae(tups_ctx* c) {
   TUPS_CHECK(c->pae_y, TUPS_ESTATE, "crossing operand buffer missing (not a full-IBZ context)");
   int v = c->pae_variant;
   if (!pae_variant_ok(c, v)) v = s.nr == 1 ? 1 : 3;
-  if (int rc = timer_begin(c)) return rc;
-  if (int rc = pae_pregather(c)) return rc;
-  if (int rc = timer_end(c, TUPS_K_PAE_GATHER)) return rc;
+  if (!kPaeVariants[v].fused) {
+    if (int rc = timer_begin(c)) return rc;
+    if (int rc = pae_pregather(c)) return rc;
+    if (int rc = timer_end(c, TUPS_K_PAE_GATHER)) return rc;
+  }
   const PaeNl1Args b = nl1_args(c);
   if (int rc = timer_begin(c)) return rc;
   if (int rc = pae_nl1_launch(c, v, b)) return rc;
