@@ -1,6 +1,6 @@
 """GPU parity: every hot-path routine through the C ABI against the oracle.
 
-Bar (BASELINE north star): relative Frobenius error <= 1e-12 per channel for the
+Bar: relative Frobenius error <= 1e-12 per channel for the
 FP64 path; integer index maps enter through the crossing and are exact.
 """
 import sys
