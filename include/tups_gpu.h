@@ -212,6 +212,12 @@ int tups_get_sigma(tups_ctx* ctx, double* sigma);
  * Synchronous. */
 int tups_gkw_grt(tups_ctx* ctx, double u, double n_particle, int32_t cal_chi, double* mu_inout, int32_t* steps_out,
                  double* grt_out);
+/* The bare bubbles of calc_Gkw_Grt with cal_chi (SRC/parquet_chi.f90:216-430)
+ * from the Grt (and G(k, tau)) of the last tups_gkw_grt: Chi0_ph, Chi0_pp
+ * (Nred = NIBZ Nf/2 complex each, the reference's IBZ order) and Chi0_ph_FF,
+ * Chi0_pp_FF (Nred, 4) complex, column-major, for the external form factors
+ * 1, cos kx - cos ky, sin kx, sin kx sin ky.  Synchronous. */
+int tups_chi0(tups_ctx* ctx, double* chi0_ph, double* chi0_pp, double* chi0_ph_ff, double* chi0_pp_ff);
 /* The device mu (no counterpart: the reference keeps mu in a module variable)
  * and the number of evaluations of the last search; returns the search's
  * failure, if any, as tups_gkw_grt would.  Synchronous. */

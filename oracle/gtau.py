@@ -9,6 +9,10 @@ Restates
   back in closed form, forward 2D FFT, particle number -2 G(r=0, tau=beta),
   bisection of mu between mu -/+ U to |n - nParticle| <= 1e-6)
 * calc_order_2nd                     SRC/parquet_SDE.f90:263-340
+* the bare bubbles of calc_Gkw_Grt with cal_chi, SRC/parquet_chi.f90:216-430
+  (Chi0_ph = FT of -Grt(r, tau) Grt(-r, beta - tau), Chi0_pp of -1/2 Grt^2, and the
+  form-factor versions with FFG = fftf2d of G(k, tau) FF_sdwave(k, ff)^2 in place of
+  the second Grt; bosonic nfourier at the IBZ momenta)
 * FDfit, nfourier                    SRC/MATH_mod.f90:95-212
 * fftf2d / fftb2d                    SRC/MATH_mod.f90:215-259 (FFTPACK zfftf /
   zfftb: forward e^{-i}, divided by n1 n2; backward e^{+i}, unscaled) -- here
@@ -26,7 +30,10 @@ Pinned by (tests/test_oracle_gtau.py):
   dispersion is particle-hole symmetric), the bisection returning mu = 0;
 * the second-order term at Sigma = 0 against the box-free brute-force
   Matsubara sum of the second-order diagram, to the spline's accuracy (the
-  tau grid has Nf points; tolerance in the test).
+  tau grid has Nf points; tolerance in the test);
+* the bare bubbles at Sigma = 0 against the Lindhard functions (ph and pp, with
+  and without the form-factor weight), and form factor 1 against the plain
+  bubbles exactly.
 """
 from __future__ import annotations
 
@@ -50,9 +57,10 @@ def fine_index(lat: Lattice, ngrain: int, kx: int, ky: int, gx: int, gy: int):
     return i1, i2
 
 
-def g_tau_k(lat: Lattice, sigma, mu, beta, ngrain=1):
+def g_tau_k(lat: Lattice, sigma, mu, beta, ngrain=1, weight=None):
     """G(k_fine, tau) before the FFT: (Nx Ng, Ny Ng, Nf) real, SRC/parquet_chi.f90:124-170
-    (the nu and tau loops as numpy vectors)."""
+    (the nu and tau loops as numpy vectors); weight (Nx, Ny) multiplies every grain of a
+    coarse momentum (the FF_sdwave^2 of FFG, SRC/parquet_chi.f90:341-343)."""
     nf = lat.nf
     ekg = ek_grain(lat, ngrain)
     out = np.zeros((lat.nx * ngrain, lat.ny * ngrain, nf))
@@ -72,6 +80,8 @@ def g_tau_k(lat: Lattice, sigma, mu, beta, ngrain=1):
                     d = 1.0 / (z + mu - e - s) - 1.0 / (z + mu - e)
                     acc = (phase * d[None, :]).real.sum(axis=1) / beta
                     out[i1 - 1, i2 - 1, :] = acc - np.exp((beta - tau) * (e - mu)) / (1.0 + math.exp(beta * (e - mu)))
+                    if weight is not None:
+                        out[i1 - 1, i2 - 1, :] *= weight[kx - 1, ky - 1]
     return out
 
 
@@ -199,3 +209,43 @@ def order_2nd(lat: Lattice, grt_rt, U, beta, ngrain=1):
                 out[ik] += np.conj(ret[nf // 2 - nu])
                 out[imk] += ret[nf // 2 - nu]
     return out
+
+
+def ffg(lat: Lattice, sigma, mu, beta, ngrain, ff):
+    """FFG for external form factor ff (0-based): fftf2d of G(k, tau) FF_sdwave(k_coarse, ff)^2
+    (SRC/parquet_chi.f90:302-351)."""
+    from .sus import ff_sdwave
+    w = ff_sdwave(lat.nx)[:, :, ff] ** 2
+    return fftf2d(g_tau_k(lat, sigma, mu, beta, ngrain, weight=w).astype(np.complex128))
+
+
+def _bubble(lat: Lattice, prod, beta, ngrain):
+    """fftb2d of prod(r, tau), read at the IBZ momenta (i, j), j <= i, then FDfit + nfourier
+    (bosonic) per momentum: (NIBZ Nf/2,) in the order ((i (i-1))/2 + j - 1) Nf/2 + k."""
+    nf = lat.nf
+    t = fftb2d(prod)
+    nxi = (lat.nx + 2) // 2
+    out = np.zeros(nxi * (nxi + 1) // 2 * (nf // 2), dtype=np.complex128)
+    for i in range(1, nxi + 1):
+        for j in range(1, i + 1):
+            data = t[(i - 1) * ngrain, (j - 1) * ngrain, :].real
+            fd1, fd2 = fdfit(data, beta / (nf - 1))
+            ret = nfourier(False, nf - 1, nf // 2, fd1, fd2, data, beta)
+            base = ((i * (i - 1)) // 2 + j - 1) * (nf // 2)
+            out[base:base + nf // 2] = ret
+    return out
+
+
+def _mirror(a):
+    """a(-r, beta - tau): index 1 -> 1, else n - idx + 2 on both axes, Nf - tau + 1 in time."""
+    return np.roll(a[::-1, ::-1, ::-1], shift=(1, 1), axis=(0, 1))
+
+
+def chi0(lat: Lattice, grt_rt, ffgs, beta, ngrain=1):
+    """The bare bubbles of calc_Gkw_Grt with cal_chi (SRC/parquet_chi.f90:216-430):
+    Chi0_ph, Chi0_pp (Nred,) and Chi0_ph_FF, Chi0_pp_FF (Nred, 4), ffgs the four FFG."""
+    ph = _bubble(lat, -grt_rt * _mirror(grt_rt), beta, ngrain)
+    pp = _bubble(lat, -0.5 * grt_rt ** 2, beta, ngrain)
+    ph_ff = np.stack([_bubble(lat, -grt_rt * _mirror(f), beta, ngrain) for f in ffgs], axis=1)
+    pp_ff = np.stack([_bubble(lat, -0.5 * grt_rt * f, beta, ngrain) for f in ffgs], axis=1)
+    return ph, pp, ph_ff, pp_ff

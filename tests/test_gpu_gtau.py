@@ -127,3 +127,36 @@ def test_sde_use_u2_needs_grt():
     with TupsContext(nx=nx, nf=nf, nl=1, beta=BETA, f_damping=0.5) as c:
         with pytest.raises(TupsError):
             c.sde(np.zeros(nx * nx * nf, dtype=np.complex128), 0.0, U, use_u2=True)
+
+
+@pytest.mark.parametrize("nx,nf,ngrain", [(4, 16, 1), (6, 8, 2), (8, 16, 1)])
+def test_chi0_parity(nx, nf, ngrain):
+    """The bare bubbles of calc_Gkw_Grt with cal_chi (plain and form-factor weighted,
+    ph and pp) from the device Grt against the oracle at the same mu, 1e-12."""
+    from tups_amd import TupsContext
+
+    _say(f"chi0 nx={nx} nf={nf} ngrain={ngrain}")
+    rng = np.random.default_rng(nx * 7 + nf + ngrain)
+    lat = Lattice(nx, nf, 1)
+    sigma = crandn(rng, (nx * nx * nf,), 0.2).ravel()
+    mu0 = 0.2
+    with TupsContext(nx=nx, nf=nf, nl=1, beta=BETA, f_damping=0.5) as c:
+        c.set_sigma(sigma, mu0, ngrain=ngrain)
+        mu, _, _ = c.gkw_grt(U, 1.0, mu0, cal_chi=True, want_grt=False)
+        got = c.chi0()
+    g = T.grt(lat, sigma, mu, BETA, ngrain)
+    ffs = [T.ffg(lat, sigma, mu, BETA, ngrain, f) for f in range(4)]
+    exp = T.chi0(lat, g, ffs, BETA, ngrain)
+    for name, a, b in zip(("ph", "pp", "ph_ff", "pp_ff"), got, exp):
+        err = rel_fro(a, b)
+        _say(f"  {name}: rel err {err:.2e}")
+        assert err < TOL, name
+
+
+def test_chi0_needs_grt():
+    from tups_amd import TupsContext, TupsError
+
+    with TupsContext(nx=4, nf=8, nl=1, beta=BETA, f_damping=0.5) as c:
+        c.set_sigma(np.zeros(4 * 4 * 8, dtype=np.complex128), 0.0)
+        with pytest.raises(TupsError):
+            c.chi0()

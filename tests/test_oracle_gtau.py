@@ -157,3 +157,54 @@ def test_bisection_hits_the_bare_density():
     assert abs(T.particle_number(g) - target) <= 1e-6
     assert abs(dens(mu) - target) <= 1e-6
     assert abs(mu - mu_exact) < 1e-5 and steps > 5
+
+
+def _lindhard(lat, mu, beta, q, w, channel, weight=None):
+    """Bare bubbles at Sigma = 0 in closed form (the tau integrals done exactly):
+    ph: 1/N sum_k FF(k)^2 (f_k - f_{k+q}) / (i w + e_k - e_{k+q})   (-beta f (1 - f) when degenerate)
+    pp: 1/(2N) sum_k FF(k)^2 (1 - f_k - f_{q-k}) / (i w - e_k - e_{q-k})."""
+    nx = lat.nx
+    k = 2 * math.pi * np.arange(nx) / nx
+    eps = (-2 * np.cos(k)[:, None] - 2 * np.cos(k)[None, :]) - mu
+    f = 1.0 / (np.exp(beta * eps) + 1.0)
+    om = 2j * math.pi * w / beta
+    s = 0j
+    for a in range(nx):
+        for b in range(nx):
+            wt = 1.0 if weight is None else weight[a, b]
+            if channel == "ph":
+                a2, b2 = (a + q[0]) % nx, (b + q[1]) % nx
+                de = eps[a, b] - eps[a2, b2]
+                x = -beta * f[a, b] * (1 - f[a, b]) if (abs(de) < 1e-12 and w == 0) else (f[a, b] - f[a2, b2]) / (om + de)
+            else:
+                a2, b2 = (q[0] - a) % nx, (q[1] - b) % nx
+                x = 0.5 * (1 - f[a, b] - f[a2, b2]) / (om - eps[a, b] - eps[a2, b2])
+            s += wt * x
+    return s / (nx * nx)
+
+
+def test_bare_bubbles_are_the_lindhard_functions():
+    """calc_Gkw_Grt's bubbles at Sigma = 0 against the closed forms, for the plain and the
+    form-factor weighted versions (the weight sits on the momentum k of the first Green's
+    function); the form factor 1 reproduces the plain bubbles exactly."""
+    from oracle import sus as SU
+    lat = Lattice(4, 64, 1)
+    sigma = np.zeros(lat.nx * lat.ny * lat.nf, dtype=np.complex128)
+    mu = 0.3
+    g = T.grt(lat, sigma, mu, BETA)
+    ffs = [T.ffg(lat, sigma, mu, BETA, 1, f) for f in range(4)]
+    ph, pp, phf, ppf = T.chi0(lat, g, ffs, BETA)
+    assert np.array_equal(phf[:, 0], ph) and np.array_equal(ppf[:, 0], pp)
+    fs = SU.ff_sdwave(lat.nx)
+    nw = lat.nf // 2
+    for (i, j) in [(1, 1), (2, 1), (3, 2), (3, 3)]:
+        for w in (0, 1, 5):
+            idx = ((i * (i - 1)) // 2 + j - 1) * nw + w
+            q = (i - 1, j - 1)
+            for got, ch in ((ph[idx], "ph"), (pp[idx], "pp")):
+                exp = _lindhard(lat, mu, BETA, q, w, ch)
+                assert abs(got - exp) < 2e-6 * max(abs(exp), 0.05), (i, j, w, ch, got, exp)
+            for f in (1, 2, 3):
+                for got, ch in ((phf[idx, f], "ph"), (ppf[idx, f], "pp")):
+                    exp = _lindhard(lat, mu, BETA, q, w, ch, weight=fs[:, :, f] ** 2)
+                    assert abs(got - exp) < 2e-6 * max(abs(exp), 0.05), (i, j, w, ch, f, got, exp)

@@ -294,6 +294,7 @@ int tups_destroy(tups_ctx* c) {
   f(c->gt_ws);
   f(c->conv_ws);
   f(c->sus_ws);
+  f(c->c0_ws);
   f(c->d_sigma_old);
   f(c->d_gstate);
   f(c->closed_ws);

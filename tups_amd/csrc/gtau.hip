@@ -52,6 +52,7 @@ struct GtauArgs {
   double2* grt;          // Grt(j1, j2, it), same layout
   double* u2data;        // [kidx Nf + it]: Re fftb2d(GGG) at the coarse momenta
   double2* sigma_out;    // Sigma receiving the second-order term
+  const double* wfine;   // weight of each fine point in the forward transform (null: 1)
   int nx, nf, ng, n1, n2, nfine, nbk;
   double beta, u;
 };
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(64) void gt_dft1(GtauArgs a) {
   const int n1 = a.n1;
   const size_t base = ((size_t)blockIdx.y * a.n2 + blockIdx.x) * n1;
   for (int i = threadIdx.x; i < n1; i += blockDim.x) {
-    row[i] = a.greal[base + i];
+    row[i] = a.wfine ? a.greal[base + i] * a.wfine[(size_t)blockIdx.x * n1 + i] : a.greal[base + i];
     double sn, cs;
     sincospi(2.0 * i / n1, &sn, &cs);
     tw[i] = make_double2(cs, -sn);
@@ -389,80 +390,95 @@ __global__ __launch_bounds__(256) void u2_ggg(GtauArgs a) {
   if (threadIdx.x == 0) a.u2data[(size_t)kidx * nf + it] = red[0];
 }
 
+// FDfit with dt = delta (SRC/MATH_mod.f90:95-107) and the cubic spline's tridiagonal
+// system with those end slopes (nfourier, SRC/MATH_mod.f90:131-160): xm receives the
+// second derivatives at the L + 1 points r (one lane; qv, uv are its scratch)
+__device__ void spline_fit(const double* r, double* xm, double* qv, double* uv, int L, double delta) {
+  const double fd1 = (-25.0 * r[0] + 48.0 * r[1] - 36.0 * r[2] + 16.0 * r[3] - 3.0 * r[4]) / 12.0 / delta;
+  const double fd2 = (25.0 * r[L] - 48.0 * r[L - 1] + 36.0 * r[L - 2] - 16.0 * r[L - 3] + 3.0 * r[L - 4]) / 12.0 / delta;
+  if (fd1 > 0.99e30) {
+    qv[0] = 0.0;
+    uv[0] = 0.0;
+  } else {
+    qv[0] = -0.5;
+    uv[0] = (3.0 / delta) * ((r[1] - r[0]) / delta - fd1);
+  }
+  for (int k = 1; k < L; ++k) {
+    const double p = qv[k - 1] / 2.0 + 2.0;
+    qv[k] = -1.0 / 2.0 / p;
+    const double t = 3.0 / (delta * delta) * (r[k + 1] + r[k - 1] - 2.0 * r[k]);
+    uv[k] = (t - uv[k - 1] / 2.0) / p;
+  }
+  double qn, un;
+  if (fd2 > 0.99e30) {
+    qn = 0.0;
+    un = 0.0;
+  } else {
+    qn = 0.5;
+    un = (3.0 / delta) * (fd2 - (r[L] - r[L - 1]) / delta);
+  }
+  xm[L] = (un - qn * uv[L - 1]) / (qn * qv[L - 1] + 1.0);
+  for (int k = L - 1; k >= 0; --k) xm[k] = qv[k] * xm[k + 1] + uv[k];
+}
+
+// the spline's exact integral against e^{i om tau} over the intervals j = j0, j0 + js, ...
+// <= L (SRC/MATH_mod.f90:170-207), added to (re, im)
+__device__ void spline_integral(const double* r, const double* xm, int L, double delta, double om, int j0, int js,
+                                double& re, double& im) {
+  const double o2 = om * om, o3 = o2 * om, o4 = o2 * o2;
+  for (int j = j0; j <= L; j += js) {
+    const double aj = r[j - 1], cj = xm[j - 1] / 2.0;
+    const double bj = (r[j] - r[j - 1]) / delta - (2.0 * xm[j - 1] + xm[j]) * delta / 6.0;
+    const double dj = (xm[j] - xm[j - 1]) / (6.0 * delta);
+    if (fabs(om) < 1e-6) {
+      re += aj * delta + 0.5 * bj * delta * delta + cj * delta * delta * delta / 3.0 +
+            dj * delta * delta * delta * delta / 4.0;
+      continue;
+    }
+    double s1, c1, s0, c0;
+    sincos(om * delta * j, &s1, &c1);
+    sincos(om * delta * (j - 1), &s0, &c0);
+    // e^{i w delta j} (P1r + i P1i) + e^{i w delta (j-1)} (P0r + i P0i)
+    const double p1r = -6.0 * dj / o4 + (bj + 2.0 * delta * cj + 3.0 * delta * delta * dj) / o2;
+    const double p1i = (2.0 * cj + 6.0 * delta * dj) / o3 +
+                       (-aj - delta * bj - delta * delta * cj - delta * delta * delta * dj) / om;
+    const double p0r = 6.0 * dj / o4 - bj / o2;
+    const double p0i = -2.0 * cj / o3 + aj / om;
+    re += c1 * p1r - s1 * p1i + c0 * p0r - s0 * p0i;
+    im += c1 * p1i + s1 * p1r + c0 * p0i + s0 * p0r;
+  }
+}
+
+// lanes per frequency for nw frequencies of L intervals in a workgroup of 256
+__device__ __forceinline__ int lanes_per_freq(int nw, int L) {
+  int ks = 1;
+  while (ks * 2 * nw <= 256 && ks * 2 <= L) ks *= 2;
+  return ks;
+}
+
 // grid Nc: FDfit + nfourier of one coarse momentum, added to Sigma at +/- nu
 // (SRC/parquet_SDE.f90:317-333)
 __global__ __launch_bounds__(256) void u2_spline(GtauArgs a) {
   extern __shared__ double sp[];
+  __shared__ double2 red[256];
   const int nf = a.nf, L = nf - 1, kidx = blockIdx.x;
   double* r = sp;             // [L + 1]
   double* xm = sp + nf;       // [L + 1]
-  double* qv = sp + 2 * nf;   // [L + 1]
-  double* uv = sp + 3 * nf;   // [L + 1]
   const double beta = a.beta, delta = beta / L;
   for (int i = threadIdx.x; i < nf; i += blockDim.x) r[i] = a.u2data[(size_t)kidx * nf + i];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    // FDfit with dt = beta / (Nf - 1) (SRC/MATH_mod.f90:95-107)
-    const double fd1 = (-25.0 * r[0] + 48.0 * r[1] - 36.0 * r[2] + 16.0 * r[3] - 3.0 * r[4]) / 12.0 / delta;
-    const double fd2 = (25.0 * r[L] - 48.0 * r[L - 1] + 36.0 * r[L - 2] - 16.0 * r[L - 3] + 3.0 * r[L - 4]) / 12.0 / delta;
-    // the spline's tridiagonal system (SRC/MATH_mod.f90:131-160)
-    if (fd1 > 0.99e30) {
-      qv[0] = 0.0;
-      uv[0] = 0.0;
-    } else {
-      qv[0] = -0.5;
-      uv[0] = (3.0 / delta) * ((r[1] - r[0]) / delta - fd1);
-    }
-    for (int k = 1; k < L; ++k) {
-      const double p = qv[k - 1] / 2.0 + 2.0;
-      qv[k] = -1.0 / 2.0 / p;
-      const double t = 3.0 / (delta * delta) * (r[k + 1] + r[k - 1] - 2.0 * r[k]);
-      uv[k] = (t - uv[k - 1] / 2.0) / p;
-    }
-    double qn, un;
-    if (fd2 > 0.99e30) {
-      qn = 0.0;
-      un = 0.0;
-    } else {
-      qn = 0.5;
-      un = (3.0 / delta) * (fd2 - (r[L] - r[L - 1]) / delta);
-    }
-    xm[L] = (un - qn * uv[L - 1]) / (qn * qv[L - 1] + 1.0);
-    for (int k = L - 1; k >= 0; --k) xm[k] = qv[k] * xm[k + 1] + uv[k];
-  }
+  if (threadIdx.x == 0) spline_fit(r, xm, sp + 2 * nf, sp + 3 * nf, L, delta);
   __syncthreads();
-  // the Nf/2 frequencies, each summed over the L intervals by `ks` lanes (j = 1 + part,
-  // 1 + part + ks, ...) whose sums are then added in order
-  __shared__ double2 red[256];
+  // the Nf/2 fermionic frequencies, each summed over the L intervals by `ks` lanes
+  // whose sums are then added in order
   const double pi = 3.14159265358979323846;
   double2* out = a.sigma_out + (size_t)kidx * nf;
-  const int nw = nf / 2;
-  int ks = 1;
-  while (ks * 2 * nw <= 256 && ks * 2 <= L) ks *= 2;
+  const int nw = nf / 2, ks = lanes_per_freq(nw, L);
   const int per = 256 / ks, part = threadIdx.x % ks, slot = threadIdx.x / ks;
   for (int i0 = 0; i0 < nw; i0 += per) {
     const int i = i0 + slot;
     double re = 0.0, im = 0.0;
-    if (i < nw) {
-      const double om = (2.0 * (i + 1) - 1.0) * pi / beta;
-      const double o2 = om * om, o3 = o2 * om, o4 = o2 * o2;
-      for (int j = 1 + part; j <= L; j += ks) {
-        const double aj = r[j - 1], cj = xm[j - 1] / 2.0;
-        const double bj = (r[j] - r[j - 1]) / delta - (2.0 * xm[j - 1] + xm[j]) * delta / 6.0;
-        const double dj = (xm[j] - xm[j - 1]) / (6.0 * delta);
-        double s1, c1, s0, c0;
-        sincos(om * delta * j, &s1, &c1);
-        sincos(om * delta * (j - 1), &s0, &c0);
-        // e^{i w delta j} (P1r + i P1i) + e^{i w delta (j-1)} (P0r + i P0i)
-        const double p1r = -6.0 * dj / o4 + (bj + 2.0 * delta * cj + 3.0 * delta * delta * dj) / o2;
-        const double p1i = (2.0 * cj + 6.0 * delta * dj) / o3 +
-                           (-aj - delta * bj - delta * delta * cj - delta * delta * delta * dj) / om;
-        const double p0r = 6.0 * dj / o4 - bj / o2;
-        const double p0i = -2.0 * cj / o3 + aj / om;
-        re += c1 * p1r - s1 * p1i + c0 * p0r - s0 * p0i;
-        im += c1 * p1i + s1 * p1r + c0 * p0i + s0 * p0r;
-      }
-    }
+    if (i < nw) spline_integral(r, xm, L, delta, (2.0 * (i + 1) - 1.0) * pi / beta, 1 + part, ks, re, im);
     red[threadIdx.x] = make_double2(re, im);
     __syncthreads();
     if (part == 0 && i < nw) {
@@ -476,6 +492,107 @@ __global__ __launch_bounds__(256) void u2_spline(GtauArgs a) {
       out[nu - 1] = make_double2(v.x + re, v.y - im);
       v = out[nf - nu];
       out[nf - nu] = make_double2(v.x + re, v.y + im);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- the bare bubbles of calc_Gkw_Grt with cal_chi (SRC/parquet_chi.f90:216-430) ----
+
+constexpr int kC0 = 10;  // ph, pp, ph_ff[4], pp_ff[4]
+
+struct C0Args {
+  const double2* grt;     // Grt(j1, j2, it)
+  const double2* ffg;     // [ff][nfs]: FFG of the four external form factors
+  const int* ibz;         // IBZ momenta x[0..nibz), y[0..nibz) (1-based)
+  double* data;           // [m][kC0][Nf]: Re fftb2d of the products at IBZ momentum m
+  double2* out;           // [kC0][nibz Nw]
+  int nx, nf, n1, n2, nfine, nibz;
+  double beta;
+};
+
+// FF_sdwave(kx, ky, ff)^2 of each fine point's coarse momentum (SRC/parquet_formfactors.f90:792-815)
+__global__ __launch_bounds__(256) void c0_weights(GtauArgs a, double* w) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= a.nfine) return;
+  const int kc = coarse_of(a, j % a.n1, j / a.n1);
+  const int kx = kc / a.nx, ky = kc % a.nx;  // 0-based
+  const double pi = 3.14159265358979323846;
+  const double px = 2.0 / a.nx * pi * kx, py = 2.0 / a.nx * pi * ky;
+  const double f[4] = {1.0, cos(px) - cos(py), sin(px), sin(px) * sin(py)};
+#pragma unroll
+  for (int ff = 0; ff < 4; ++ff) w[(size_t)ff * a.nfine + j] = f[ff] * f[ff];
+}
+
+// grid (nibz, Nf): the ten products at slice it, transformed back at IBZ momentum m
+__global__ __launch_bounds__(256) void c0_pair(C0Args a) {
+  __shared__ double red[kC0][256];
+  const int m = blockIdx.x, it = blockIdx.y;
+  const int nx = a.nx, n1 = a.n1, n2 = a.n2, nf = a.nf;
+  const int qx = a.ibz[m] - 1, qy = a.ibz[a.nibz + m] - 1;
+  const size_t nfs = (size_t)a.nfine * nf;
+  const size_t slab = (size_t)it * n1 * n2, mslab = (size_t)(nf - 1 - it) * n1 * n2;
+  double acc[kC0];
+#pragma unroll
+  for (int c = 0; c < kC0; ++c) acc[c] = 0.0;
+  for (int j = threadIdx.x; j < a.nfine; j += blockDim.x) {
+    const int j1 = j % n1, j2 = j / n1;
+    const int m1 = j1 == 0 ? 0 : n1 - j1, m2 = j2 == 0 ? 0 : n2 - j2;
+    const size_t jm = (size_t)m2 * n1 + m1;
+    const double2 g = a.grt[slab + j], gm = a.grt[mslab + jm];
+    // e^{+2 pi i (qx Ng j1 / n1 + qy Ng j2 / n2)} = e^{2 pi i (qx j1 + qy j2) / Nx}
+    double sn, cs;
+    sincospi(2.0 * ((qx * j1 + qy * j2) % nx) / nx, &sn, &cs);
+    auto re_of = [&](double2 p, double c) __attribute__((always_inline)) { return c * (p.x * cs - p.y * sn); };
+    acc[0] += re_of(cmul(g, gm), -1.0);
+    acc[1] += re_of(cmul(g, g), -0.5);
+#pragma unroll
+    for (int ff = 0; ff < 4; ++ff) {
+      const double2* F = a.ffg + (size_t)ff * nfs;
+      acc[2 + ff] += re_of(cmul(g, F[mslab + jm]), -1.0);
+      acc[6 + ff] += re_of(cmul(g, F[slab + j]), -0.5);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kC0; ++c) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h)
+#pragma unroll
+      for (int c = 0; c < kC0; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < kC0) a.data[((size_t)m * kC0 + threadIdx.x) * nf + it] = red[threadIdx.x][0];
+}
+
+// grid (nibz, kC0): FDfit + bosonic nfourier of one momentum and quantity
+__global__ __launch_bounds__(256) void c0_spline(C0Args a) {
+  extern __shared__ double sp[];
+  __shared__ double2 red[256];
+  const int nf = a.nf, L = nf - 1, m = blockIdx.x, c = blockIdx.y;
+  double* r = sp;
+  double* xm = sp + nf;
+  const double beta = a.beta, delta = beta / L;
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) r[i] = a.data[((size_t)m * kC0 + c) * nf + i];
+  __syncthreads();
+  if (threadIdx.x == 0) spline_fit(r, xm, sp + 2 * nf, sp + 3 * nf, L, delta);
+  __syncthreads();
+  const double pi = 3.14159265358979323846;
+  const int nw = nf / 2, ks = lanes_per_freq(nw, L);
+  const int per = 256 / ks, part = threadIdx.x % ks, slot = threadIdx.x / ks;
+  double2* out = a.out + (size_t)c * a.nibz * nw + (size_t)m * nw;
+  for (int i0 = 0; i0 < nw; i0 += per) {
+    const int i = i0 + slot;
+    double re = 0.0, im = 0.0;
+    if (i < nw) spline_integral(r, xm, L, delta, 2.0 * i * pi / beta, 1 + part, ks, re, im);
+    red[threadIdx.x] = make_double2(re, im);
+    __syncthreads();
+    if (part == 0 && i < nw) {
+      for (int t = 1; t < ks; ++t) {
+        re += red[threadIdx.x + t].x;
+        im += red[threadIdx.x + t].y;
+      }
+      out[i] = make_double2(re, im);
     }
     __syncthreads();
   }
@@ -592,6 +709,75 @@ int launch_order_2nd(tups_ctx* c, double u, double2* sigma_out) {
   TUPS_HIP(hipGetLastError());
   return TUPS_OK;
 }
+
+}  // namespace tups
+
+using namespace tups;
+
+extern "C" int tups_chi0(tups_ctx* c, double* chi0_ph, double* chi0_pp, double* chi0_ph_ff, double* chi0_pp_ff) {
+  TUPS_CHECK(c && chi0_ph && chi0_pp && chi0_ph_ff && chi0_pp_ff, TUPS_EINVAL, "null argument");
+  TUPS_CHECK(c->grt_valid, TUPS_ESTATE, "the bare bubbles need Grt on the device (tups_gkw_grt first)");
+  const tups_sizes& s = c->sz;
+  GtauArgs a{};
+  fill_args(c, &a, 0.0);
+  const size_t nfine = (size_t)a.nfine, nfs = nfine * s.nf, nw = s.nf / 2, nred = (size_t)s.nibz * nw;
+  const size_t need = (4 * nfine + (size_t)s.nibz * kC0 * s.nf) * sizeof(double) +
+                      (4 * nfs + kC0 * nred) * sizeof(double2) + 4 * 256;
+  if (c->c0_ws_bytes < need) {
+    if (c->c0_ws) TUPS_HIP(hipFree(c->c0_ws));
+    c->c0_ws = nullptr;
+    c->c0_ws_bytes = 0;
+    TUPS_HIP(hipMalloc(&c->c0_ws, need));
+    c->c0_ws_bytes = need;
+  }
+  char* w = static_cast<char*>(c->c0_ws);
+  auto take = [&w](size_t bytes) {
+    char* p = w;
+    w += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  double* wts = reinterpret_cast<double*>(take(4 * nfine * sizeof(double)));
+  C0Args b{};
+  b.ffg = reinterpret_cast<double2*>(take(4 * nfs * sizeof(double2)));
+  b.data = reinterpret_cast<double*>(take((size_t)s.nibz * kC0 * s.nf * sizeof(double)));
+  b.out = reinterpret_cast<double2*>(take(kC0 * nred * sizeof(double2)));
+  b.grt = a.grt;
+  b.ibz = c->d_ibz;
+  b.nx = s.nx;
+  b.nf = s.nf;
+  b.n1 = a.n1;
+  b.n2 = a.n2;
+  b.nfine = a.nfine;
+  b.nibz = s.nibz;
+  b.beta = c->beta;
+  hipLaunchKernelGGL(c0_weights, dim3((unsigned)((nfine + 255) / 256)), dim3(256), 0, c->stream, a, wts);
+  TUPS_HIP(hipGetLastError());
+  for (int ff = 0; ff < 4; ++ff) {  // FFG: fftf2d of G(k, tau) FF_sdwave(k, ff)^2
+    GtauArgs f = a;
+    f.wfine = wts + ff * nfine;
+    f.grt = const_cast<double2*>(b.ffg) + ff * nfs;
+    hipLaunchKernelGGL(gt_dft1, dim3((unsigned)f.n2, (unsigned)s.nf), dim3(64),
+                       (size_t)f.n1 * (sizeof(double) + sizeof(double2)), c->stream, f);
+    TUPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gt_dft2, dim3((unsigned)f.n2, (unsigned)s.nf), dim3(64), (size_t)f.n2 * sizeof(double2),
+                       c->stream, f);
+    TUPS_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(c0_pair, dim3((unsigned)s.nibz, (unsigned)s.nf), dim3(256), 0, c->stream, b);
+  TUPS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(c0_spline, dim3((unsigned)s.nibz, kC0), dim3(256), 4 * (size_t)s.nf * sizeof(double), c->stream,
+                     b);
+  TUPS_HIP(hipGetLastError());
+  const size_t z = nred * sizeof(double2);
+  TUPS_HIP(hipMemcpyAsync(chi0_ph, b.out, z, hipMemcpyDeviceToHost, c->stream));
+  TUPS_HIP(hipMemcpyAsync(chi0_pp, b.out + nred, z, hipMemcpyDeviceToHost, c->stream));
+  TUPS_HIP(hipMemcpyAsync(chi0_ph_ff, b.out + 2 * nred, 4 * z, hipMemcpyDeviceToHost, c->stream));
+  TUPS_HIP(hipMemcpyAsync(chi0_pp_ff, b.out + 6 * nred, 4 * z, hipMemcpyDeviceToHost, c->stream));
+  TUPS_HIP(hipStreamSynchronize(c->stream));
+  return TUPS_OK;
+}
+
+namespace tups {
 
 const double2* gtau_grt(const tups_ctx* c) {
   GtauArgs a{};

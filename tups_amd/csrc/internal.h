@@ -125,6 +125,8 @@ struct tups_ctx {
   size_t gt_ws_bytes = 0;
   int gt_ngrain = 0;             // Ngrain the G(tau) workspace is laid out for
   bool grt_valid = false;        // Grt of the last calc_Gkw_Grt is in the workspace
+  void* c0_ws = nullptr;         // bare bubbles: weights, FFG, transformed products, result
+  size_t c0_ws_bytes = 0;
   double2* d_sigma_old = nullptr;  // Sigma before the last device SDE update (SigmaOld of conv)
   bool sigma_old_valid = false;
   void* sus_ws = nullptr;        // susceptibilities: Green's-function pairs, result

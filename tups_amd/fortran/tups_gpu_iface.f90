@@ -301,6 +301,15 @@ module tups_gpu_iface
       integer(c_int) :: rc
     end function tups_prms_dims
 
+    ! calc_Gkw_Grt with cal_chi (SRC/parquet_chi.f90:216-430): Chi0_ph, Chi0_pp (Nred),
+    ! Chi0_ph_FF, Chi0_pp_FF (Nred, 4)
+    function tups_chi0(ctx, chi0_ph, chi0_pp, chi0_ph_ff, chi0_pp_ff) bind(C, name="tups_chi0") result(rc)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: chi0_ph(*), chi0_pp(*), chi0_ph_ff(*), chi0_pp_ff(*)
+      integer(c_int) :: rc
+    end function tups_chi0
+
     function tups_get_mu(ctx, mu, steps) bind(C, name="tups_get_mu") result(rc)
       import :: c_ptr, c_int32_t, c_int, c_double
       type(c_ptr), value :: ctx

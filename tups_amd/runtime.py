@@ -89,6 +89,7 @@ SIGNATURES = {
     "tups_iteration_full": ([_V, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double], C.c_int),
     "tups_gkw_grt": ([_V, C.c_double, C.c_double, C.c_int32, _D, C.POINTER(C.c_int32), _D], C.c_int),
     "tups_get_mu": ([_V, _D, C.POINTER(C.c_int32)], C.c_int),
+    "tups_chi0": ([_V, _D, _D, _D, _D], C.c_int),
     "tups_conv": ([_V, _D, _D, C.c_double, C.c_double, _P(ConvInfo)], C.c_int),
     "tups_read_prms": ([C.c_char_p, _P(Prms)], C.c_int),
     "tups_calc_sus": ([_V, C.c_double, C.c_int32, C.c_int32, _D], C.c_int),
@@ -422,6 +423,17 @@ class TupsContext:
         rc = h5.tups_h5_write_f(self._h, str(path).encode())
         if rc != 0:
             raise TupsError(rc, "tups_h5_write_f", h5.tups_h5_last_error().decode())
+
+    def chi0(self):
+        """The bare bubbles of calc_Gkw_Grt with cal_chi from the device Grt:
+        (Chi0_ph, Chi0_pp) of shape (Nred,) and (Chi0_ph_FF, Chi0_pp_FF) of shape (Nred, 4)."""
+        nred = self.nibz * (self.nf // 2)
+        ph = np.zeros(nred, dtype=np.complex128)
+        pp = np.zeros(nred, dtype=np.complex128)
+        phf = np.zeros((nred, 4), dtype=np.complex128, order="F")
+        ppf = np.zeros((nred, 4), dtype=np.complex128, order="F")
+        _check(self.lib.tups_chi0(self._h, _dptr(ph), _dptr(pp), _dptr(phf), _dptr(ppf)), "tups_chi0")
+        return ph, pp, phf, ppf
 
     def get_mu(self):
         """(mu, steps) of the device search state."""
