@@ -147,6 +147,30 @@ def config1_cpu():
             "workload": CONFIG1_WORKLOAD + "; oracle restatement in numpy"}
 
 
+def reference_prms_run():
+    """tups_amd/tups_run on the parameter file the reference ships (tests/golden/prms_reference):
+    Initialization's start, then up to ite_max = 100 device iterations with the mu search,
+    the SDE with use_U2 and conv(); wall time of the whole program (a child process)."""
+    import subprocess
+
+    root = os.path.dirname(os.path.abspath(__file__))
+    exe = os.path.join(root, "tups_amd", "tups_run")
+    prm = os.path.join(root, "tests", "golden", "prms_reference")
+    out = {"workload": "the reference's prms: Nx=16, Nf=64, Nl=1 (Nq=1440), beta=5, U=2, nParticle=1, use_U2, "
+                       "ite_max=100; whole program incl. set-up"}
+    if not os.path.exists(exe):
+        out["skipped"] = "tups_run not built"
+        return out
+    t0 = time.perf_counter()
+    r = subprocess.run([exe, prm], capture_output=True, text=True, timeout=300)
+    wall = time.perf_counter() - t0
+    its = [ln for ln in r.stdout.splitlines() if ln.strip().startswith("iteration")]
+    out.update({"returncode": r.returncode, "wall_s": round(wall, 3), "iterations": len(its),
+                "converged": "converged" in r.stdout, "last": its[-1].strip() if its else None})
+    print(f"[bench] reference prms run: {len(its)} iterations in {wall:.2f} s", file=sys.stderr, flush=True)
+    return out
+
+
 def large_share(moduli):
     """Nx=16, Nl=9, Nf=64 (D=576), Nq=4096 = 256 momenta x 16 bosonic frequencies:
     12 vertex arrays of 21.7 GB resident.  One FP64 and one split step (after a
@@ -484,6 +508,9 @@ def main():
     print(f"[bench] config-1 iteration: {c1_ms:.3f} ms", file=sys.stderr, flush=True)
     c1.close()
 
+    # the reference's own parameter file (Nx=16, Nf=64, Nl=1, beta=5, U=2, 100 iterations) run whole
+    prms_run = reference_prms_run()
+
     large = None if args.no_large else large_share(args.moduli)
 
     cpu = None
@@ -523,6 +550,7 @@ def main():
         "gtau": gtau,
         "conv": conv,
         "config1_iteration": config1,
+        "reference_prms_run": prms_run,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
