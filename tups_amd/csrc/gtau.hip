@@ -275,25 +275,33 @@ __global__ __launch_bounds__(1024) void gt_search_small(GtauArgs a) {
 
 // grid: nfine; G(k_fine, tau) for every tau at the final mu (SRC/parquet_chi.f90:124-170)
 __global__ __launch_bounds__(64) void gt_fine(GtauArgs a) {
-  extern __shared__ double2 dnu[];  // [Nf]
+  extern __shared__ double2 dnu[];  // [Nf] dressed-minus-bare G, then [2 (Nf - 1)] phases
   const int j = blockIdx.x;
   const int nf = a.nf;
+  const int per = 2 * (nf - 1);
+  double2* ph = dnu + nf;  // e^{-i pi m / (Nf - 1)} as (cos, sin) of pi m / (Nf - 1)
   const double mu = dev_mu(a.st), beta = a.beta, e = a.etab[j];
   const int kc = coarse_of(a, j % a.n1, j / a.n1);
   const double2* s = a.sigma + (size_t)kc * nf;
   for (int nu = threadIdx.x + 1; nu <= nf; nu += blockDim.x) dnu[nu - 1] = dress(wn(nu, nf, beta), mu, e, s[nu - 1]);
+  for (int m = threadIdx.x; m < per; m += blockDim.x) {
+    double sn, cs;
+    sincospi((double)m / (double)(nf - 1), &sn, &cs);
+    ph[m] = make_double2(cs, sn);
+  }
   __syncthreads();
-  const int per = 2 * (nf - 1);
   for (int it = threadIdx.x; it < nf; it += blockDim.x) {
     double acc = 0.0;
+    // e^{-i nu_n tau_it}: phase pi (2 nu - Nf - 1) it / (Nf - 1), reduced exactly; the
+    // index steps by 2 it (mod 2 (Nf - 1)) from nu to nu + 1
+    int m = ((1 - nf) * it) % per;
+    if (m < 0) m += per;
+    const int step = (2 * it) % per;
     for (int nu = 1; nu <= nf; ++nu) {
-      // e^{-i nu_n tau_it}: phase pi (2 nu - Nf - 1) it / (Nf - 1), reduced exactly
-      int m = ((2 * nu - nf - 1) * it) % per;
-      if (m < 0) m += per;
-      double sn, cs;
-      sincospi((double)m / (double)(nf - 1), &sn, &cs);
-      const double2 d = dnu[nu - 1];
-      acc += (cs * d.x + sn * d.y) / beta;
+      const double2 t = ph[m], d = dnu[nu - 1];
+      acc += (t.x * d.x + t.y * d.y) / beta;
+      m += step;
+      if (m >= per) m -= per;
     }
     const double tau = beta / (nf - 1) * it;
     const double x = e - mu;
@@ -683,7 +691,7 @@ int launch_gtau(tups_ctx* c, int mode, int mu_from_host, double mu_host, double 
     hipLaunchKernelGGL(gt_walk, dim3(1), dim3(64), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(gt_fine, dim3((unsigned)a.nfine), dim3(64), (size_t)nf * sizeof(double2), c->stream, a);
+  hipLaunchKernelGGL(gt_fine, dim3((unsigned)a.nfine), dim3(64), (size_t)(3 * nf) * sizeof(double2), c->stream, a);
   TUPS_HIP(hipGetLastError());
   hipLaunchKernelGGL(gt_dft1, dim3((unsigned)a.n2, (unsigned)nf), dim3(64),
                      (size_t)a.n1 * (sizeof(double) + sizeof(double2)), c->stream, a);
