@@ -99,8 +99,9 @@ def config1_inputs(lat_or_ctx_nq, nz, nf, nw, seed=7):
     import numpy as np
 
     rng = np.random.default_rng(seed)
-    mk = lambda d: np.asfortranarray(0.05 * (rng.standard_normal((nz, nz, lat_or_ctx_nq))
-                                             + 1j * rng.standard_normal((nz, nz, lat_or_ctx_nq))) + d * np.eye(nz)[:, :, None])
+    shape = (nz, nz, lat_or_ctx_nq)
+    mk = lambda d: np.asfortranarray(0.05 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+                                     + d * np.eye(nz)[:, :, None])
     gam = [mk(0.5) for _ in range(4)]
     fv = [mk(0.5) for _ in range(4)]
     U = 2.0
@@ -397,7 +398,8 @@ def main():
                  "unit": "TFLOP/s", "frac": round(fl5 / t5 / 1e12 / FP64_PEAK_TFLOPS, 4),
                  "avg_launch_ms": round(p5_ms / p5_n, 3), "flops_per_call": fl5,
                  "traffic": profile_traffic("crossing_pae_nl5_nx8_nf64.md"),
-                 "bytes_per_call": crossing_bytes(c5), "kernel": "pae_ybuild + pae_mm (FP64 MFMA GEMM per bosonic frequency)",
+                 "bytes_per_call": crossing_bytes(c5),
+                 "kernel": "pae_ybuild + pae_mm (FP64 MFMA GEMM per bosonic frequency)",
                  "workload": f"calculate_PAE Nx=8 Nf=64 Nl=5 NR={c5.nr}, Nq={c5.nq} (all IBZ q), 4 channels"}
     print(f"[bench] crossing Nl=5: {p5_ms / p5_n:.2f} ms per call", file=sys.stderr, flush=True)
     c5.close()

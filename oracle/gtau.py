@@ -193,7 +193,8 @@ def order_2nd(lat: Lattice, grt_rt, U, beta, ngrain=1):
             kym = 1 if ky == 1 else n2 - ky + 2
             for kx in range(1, n1 + 1):
                 kxm = 1 if kx == 1 else n1 - kx + 2
-                ggg[kx - 1, ky - 1, it] = grt_rt[kx - 1, ky - 1, it] ** 2 * grt_rt[kxm - 1, kym - 1, nf - it - 1] * U * U
+                ggg[kx - 1, ky - 1, it] = (grt_rt[kx - 1, ky - 1, it] ** 2
+                                           * grt_rt[kxm - 1, kym - 1, nf - it - 1] * U * U)
     ggg = fftb2d(ggg)
     out = np.zeros(lat.nx * lat.ny * nf, dtype=np.complex128)
     for kx in range(1, lat.nx + 1):

@@ -175,7 +175,10 @@ def _lindhard(lat, mu, beta, q, w, channel, weight=None):
             if channel == "ph":
                 a2, b2 = (a + q[0]) % nx, (b + q[1]) % nx
                 de = eps[a, b] - eps[a2, b2]
-                x = -beta * f[a, b] * (1 - f[a, b]) if (abs(de) < 1e-12 and w == 0) else (f[a, b] - f[a2, b2]) / (om + de)
+                if abs(de) < 1e-12 and w == 0:
+                    x = -beta * f[a, b] * (1 - f[a, b])
+                else:
+                    x = (f[a, b] - f[a2, b2]) / (om + de)
             else:
                 a2, b2 = (q[0] - a) % nx, (q[1] - b) % nx
                 x = 0.5 * (1 - f[a, b] - f[a2, b2]) / (om - eps[a, b] - eps[a2, b2])

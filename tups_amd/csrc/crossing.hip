@@ -548,8 +548,8 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
       const bool kok = k < ktot;
       const int rt = k >= kk ? 1 : 0, jj = k - rt * kk;
       const int jl = jj % nl2, jr = jj / nl2;
-      ra[i] = (rok && kok) ? a.M[2 * p + rt][(size_t)rl + (size_t)nl2 * jl + (size_t)nl2 * nl2 * (rm + (size_t)a.nibz * jr)]
-                           : make_double2(0.0, 0.0);
+      const size_t mi = (size_t)rl + (size_t)nl2 * jl + (size_t)nl2 * nl2 * (rm + (size_t)a.nibz * jr);
+      ra[i] = (rok && kok) ? a.M[2 * p + rt][mi] : make_double2(0.0, 0.0);
       const int kt = 4 * p + 2 * rt + t;
       rb[i] = (nok && kok) ? a.Y[((size_t)(kt * a.nw + w0) * kk + jj) * nn + gn] : make_double2(0.0, 0.0);
     }

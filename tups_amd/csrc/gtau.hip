@@ -19,8 +19,8 @@
 // candidate mu, one row of workgroups each); one workgroup then walks the tree
 // as the sequential loop would, stops at the first mu within the tolerance, and
 // lays out the next subtree.  Small grids run every pass inside one workgroup
-// of 1024 lanes (16 per candidate) and stop as soon as the walk ends.  The mu found is the one the reference's loop stops
-// at; the number of launches is fixed (passes after convergence exit at once),
+// of 1024 lanes (16 per candidate) and stop as soon as the walk ends.  The mu
+// found is the one the reference's loop stops at; the number of launches is fixed (passes after convergence exit at once),
 // so the whole search sits in a HIP graph.
 //
 // The 2D transforms are direct DFTs (n1 n2 (n1 + n2) Nf complex MACs, small next
@@ -403,7 +403,8 @@ __global__ __launch_bounds__(256) void u2_ggg(GtauArgs a) {
 // second derivatives at the L + 1 points r (one lane; qv, uv are its scratch)
 __device__ void spline_fit(const double* r, double* xm, double* qv, double* uv, int L, double delta) {
   const double fd1 = (-25.0 * r[0] + 48.0 * r[1] - 36.0 * r[2] + 16.0 * r[3] - 3.0 * r[4]) / 12.0 / delta;
-  const double fd2 = (25.0 * r[L] - 48.0 * r[L - 1] + 36.0 * r[L - 2] - 16.0 * r[L - 3] + 3.0 * r[L - 4]) / 12.0 / delta;
+  const double fd2 =
+      (25.0 * r[L] - 48.0 * r[L - 1] + 36.0 * r[L - 2] - 16.0 * r[L - 3] + 3.0 * r[L - 4]) / 12.0 / delta;
   if (fd1 > 0.99e30) {
     qv[0] = 0.0;
     uv[0] = 0.0;
