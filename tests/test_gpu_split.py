@@ -94,13 +94,13 @@ def test_split_error_falls_with_moduli():
     assert errs[-1] < 1e-14
 
 
-def test_split_rows_and_columns_of_very_different_scale():
+@pytest.mark.parametrize("nx,nf,nl", [(4, 8, 5), (4, 64, 1)])
+def test_split_rows_and_columns_of_very_different_scale(nx, nf, nl):
     """Gamma and F with rows scaled by 1e-9 and columns by 1e6: every element of
     Phi, measured in units of its own row and column scale, stays accurate.
     A single scale per matrix would leave the small rows and columns with
     errors of order 2^-35 * 1e15."""
     rng = np.random.default_rng(9)
-    nx, nf, nl = 4, 8, 5
     lat = Lattice(nx, nf, nl)
     with _ctx(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=1.0) as c:
         D, nq = c.nz, c.nq
