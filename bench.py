@@ -481,7 +481,8 @@ def main():
     gt_ms, gt_n = gc.kernel_time(K_GTAU)
     gc.set_timing(False)
     gtau = {"ms_per_call": round(gt_ms / gt_n, 3), "mu_evaluations": steps_g,
-            "kernel": "gt_setup + 11 x (gt_density over 63 candidate mu + gt_walk) + gt_fine + gt_dft1 + gt_dft2",
+            "kernel": "gt_setup + 11 x gt_density (63 candidate mu, tree walk in the last workgroup) + gt_fine + "
+                      "gt_dft1 + gt_dft2",
             "workload": "calc_Gkw_Grt, Nx=8 Ngrain=4 (32x32 fine grid), Nf=64, nParticle=0.9, random Sigma"}
     print(f"[bench] gtau: {gt_ms / gt_n:.3f} ms per call, {steps_g} evaluations", file=sys.stderr, flush=True)
     gc.close()

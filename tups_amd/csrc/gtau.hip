@@ -143,6 +143,7 @@ __global__ __launch_bounds__(256) void gt_setup(GtauArgs a, int mu_from_host, do
     st->done = mode == 2;
     st->t1 = 0.0;
     st->mu = mu0;
+    st->arrived = 0;
   }
   __syncthreads();
   fill_tree(st->cand, bounds[0], bounds[1], tid);
@@ -158,20 +159,51 @@ __device__ __forceinline__ double g_beta(const GtauArgs& a, double mu, int j) {
   return -acc / a.beta - fermi(a.beta * (e - mu));
 }
 
-// grid (nbk, kCand): partial sums of G(k, beta) over the fine grid per candidate
+__device__ int walk_tree(GtauState* st, const double* cand, const double* t1c, double* bounds);
+
+// grid (nbk, kCand): partial sums of G(k, beta) over the fine grid per candidate.
+// The last workgroup to finish (counted in st->arrived, which it resets) adds
+// the partials in a fixed order, walks the subtree and lays out the next one.
+// Every workgroup reads st->done before any of them can count itself in, so a
+// pass runs whole or not at all.
 __global__ __launch_bounds__(256) void gt_density(GtauArgs a) {
   __shared__ double red[256];
-  if (a.st->done) return;
+  __shared__ double t1c[kCand];
+  __shared__ double bounds[2];
+  __shared__ int last, finished;
+  GtauState* st = a.st;
+  if (st->done) return;
+  const int tid = threadIdx.x;
   const int c = blockIdx.y;
-  const double mu = a.st->cand[c];
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  red[threadIdx.x] = j < a.nfine ? g_beta(a, mu, j) : 0.0;
+  const double mu = st->cand[c];
+  const int j = blockIdx.x * 256 + tid;
+  red[tid] = j < a.nfine ? g_beta(a, mu, j) : 0.0;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    if (tid < h) red[tid] += red[tid + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) a.partial[(size_t)c * a.nbk + blockIdx.x] = red[0];
+  if (tid == 0) {
+    a.partial[(size_t)c * a.nbk + blockIdx.x] = red[0];
+    __threadfence();
+    last = atomicAdd(&st->arrived, 1u) == gridDim.x * gridDim.y - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (tid < kCand) {
+    double s = 0.0;
+    for (int b = 0; b < a.nbk; ++b) s += a.partial[(size_t)tid * a.nbk + b];
+    t1c[tid] = -2.0 * s / (double)a.nfine;
+    st->t1c[tid] = t1c[tid];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    st->arrived = 0;
+    finished = walk_tree(st, st->cand, t1c, bounds);
+  }
+  __syncthreads();
+  if (!finished) fill_tree(st->cand, bounds[0], bounds[1], tid);
 }
 
 // one lane: walks the evaluated subtree (candidates cand, particle numbers t1c) as the
@@ -219,26 +251,6 @@ __device__ int walk_tree(GtauState* st, const double* cand, const double* t1c, d
   bounds[0] = h;
   bounds[1] = l;
   return done;
-}
-
-// one workgroup of 64: particle numbers of the candidates, the walk, the next subtree
-__global__ __launch_bounds__(64) void gt_walk(GtauArgs a) {
-  __shared__ double t1c[kCand];
-  __shared__ double bounds[2];
-  __shared__ int finished;
-  GtauState* st = a.st;
-  if (st->done) return;
-  const int tid = threadIdx.x;
-  if (tid < kCand) {
-    double s = 0.0;
-    for (int b = 0; b < a.nbk; ++b) s += a.partial[(size_t)tid * a.nbk + b];
-    t1c[tid] = -2.0 * s / (double)a.nfine;
-    st->t1c[tid] = t1c[tid];
-  }
-  __syncthreads();
-  if (tid == 0) finished = walk_tree(st, st->cand, t1c, bounds);
-  __syncthreads();
-  if (!finished) fill_tree(st->cand, bounds[0], bounds[1], tid);
 }
 
 // the whole search in one workgroup of 1024 for small grids (63 Nfine Nf within
@@ -688,8 +700,6 @@ int launch_gtau(tups_ctx* c, int mode, int mu_from_host, double mu_host, double 
   }
   for (int p = 0; p < (small ? 0 : passes); ++p) {
     hipLaunchKernelGGL(gt_density, dim3((unsigned)a.nbk, kCand), dim3(256), 0, c->stream, a);
-    TUPS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gt_walk, dim3(1), dim3(64), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(gt_fine, dim3((unsigned)a.nfine), dim3(64), (size_t)(3 * nf) * sizeof(double2), c->stream, a);

@@ -67,6 +67,7 @@ void build_phir_table(const Tables& t, PhirTable& out);
 struct GtauState {
   double mu, hi, lo, t1, n_target;
   int steps, done, status, single;  // status 0 ok, 1 no convergence, 2 non-finite n
+  unsigned arrived;                  // workgroups of the running density pass that are done
   double cand[63];
   double t1c[63];
 };
