@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void conv_fmax(ConvArgs a) {
   const double2* f = a.f[blockIdx.y];
   double best = -1.0, bre = 0.0, bim = 0.0;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < a.nf_elems; i += (size_t)gridDim.x * 256) {
-    const double2 v = f[i];
+    const double2 v = ld_stream(f + i);
     const double s = v.x * v.x + v.y * v.y;
     if (s > best) {
       best = s;

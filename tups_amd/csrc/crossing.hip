@@ -31,22 +31,10 @@
 namespace tups {
 
 
-typedef double dv2 __attribute__((ext_vector_type(2)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int clampnu(int x, int nf) { return x < 1 ? 1 : (x > nf ? nf : x); }
 
-__device__ __forceinline__ double2 ld_stream(const double2* p) {
-  const dv2 v = __builtin_nontemporal_load(reinterpret_cast<const dv2*>(p));
-  return make_double2(v.x, v.y);
-}
-
-__device__ __forceinline__ void st_stream(double2* p, double2 v) {
-  dv2 w;
-  w.x = v.x;
-  w.y = v.y;
-  __builtin_nontemporal_store(w, reinterpret_cast<dv2*>(p));
-}
 
 // One crossing term, branch-free: `wt` is 0 for an entry the term does not
 // reach (its indices are then any in-range value), `tr` selects the

@@ -62,6 +62,20 @@ struct PhirTable {
 };
 void build_phir_table(const Tables& t, PhirTable& out);
 
+// Loads and stores of data touched once per kernel, marked non-temporal: a
+// plain G = F - G stream reaches 81% of HBM peak with them against 73% without.
+typedef double stream2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_stream(const double2* p) {
+  const stream2 v = __builtin_nontemporal_load(reinterpret_cast<const stream2*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(double2* p, double2 v) {
+  stream2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<stream2*>(p));
+}
+
 // device state of the chemical-potential search of calc_Gkw_Grt: the current mu,
 // the bracket, the last particle number and the candidates of the current pass
 struct GtauState {

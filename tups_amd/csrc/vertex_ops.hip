@@ -18,8 +18,8 @@ __global__ __launch_bounds__(256) void gamma_update_kernel(double2* g0, double2*
   const int r = blockIdx.y;
   double2* g = r == 0 ? g0 : r == 1 ? g1 : r == 2 ? g2 : g3;
   const double2* f = r == 0 ? f0 : r == 1 ? f1 : r == 2 ? f2 : f3;
-  const double2 a = f[i], b = g[i];
-  g[i] = make_double2(a.x - b.x, a.y - b.y);
+  const double2 a = ld_stream(f + i), b = ld_stream(g + i);
+  st_stream(g + i, make_double2(a.x - b.x, a.y - b.y));
 }
 
 int launch_gamma_update(tups_ctx* c) {
