@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void crt_rowmax_a(CrtArgs a) {
   if (row < D) {
 #pragma unroll 4
     for (int k = k0; k < k1; ++k) {
-      const double2 v = k < D ? G[(size_t)k * D + row] : F[(size_t)(k - D) * D + row];
+      const double2 v = ld_stream(k < D ? G + (size_t)k * D + row : F + (size_t)(k - D) * D + row);
       mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
     }
   }
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
   for (int u = 0; u < 16; ++u) {
     const int k = k0 + u;
     double2 v = make_double2(0.0, 0.0);
-    if (rok && k < 2 * D) v = k < D ? G[(size_t)k * D + row] : F[(size_t)(k - D) * D + row];
+    if (rok && k < 2 * D) v = ld_stream(k < D ? G + (size_t)k * D + row : F + (size_t)(k - D) * D + row);
     limbs(rint(v.x * sc), r1[u], r0[u]);
     limbs(rint(v.y * sc), i1[u], i0[u]);
   }
