@@ -157,7 +157,8 @@ __global__ __launch_bounds__(256) void sde_vchi(SdeArgs a) {
       const int nu = o - a.f_range * nf + 1;  // l4_0
       const int col = l4 * nf + (nu < 1 ? 1 : (nu > nf ? nf : nu)) - 1;  // pickNu_range
       const size_t idx = base + (size_t)col * nz + i3;
-      const double2 gd = a.gam[0][idx], gm = a.gam[1][idx], gs = a.gam[2][idx], gt = a.gam[3][idx];
+      const double2 gd = ld_stream(a.gam[0] + idx), gm = ld_stream(a.gam[1] + idx);
+      const double2 gs = ld_stream(a.gam[2] + idx), gt = ld_stream(a.gam[3] + idx);
       const double2 xp = cph[l4 * a.ext + o], xq = cpp[l4 * a.ext + o];
       const double2 vp = make_double2(gd.x - 3.0 * gm.x, gd.y - 3.0 * gm.y);
       const double2 vq = make_double2(gs.x + gt.x, gs.y + gt.y);
