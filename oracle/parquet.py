@@ -22,10 +22,16 @@ Pinned by (tests/test_oracle_parquet.py):
 * crossing: Nl=1, NR=1 against a brute-force frequency crossing written from
   the index maps of each routine (every target entry reached at most once per
   routine pair), and the linearity of calculate_PAE in PhiR.
-The Nl>1 M tensors and crossing rest on sym_op_L / FF_inv, whose law is
-pinned in test_oracle_lattice.py; beyond those identities Nl>1 PAE parity is
-"parity unpinned" (no reference vectors exist; the reference cannot be built
-here).
+* calc_phir + calc_m_tensors + calculate_pae together, for Nl in {1, 5, 9}
+  (tests/test_oracle_crossing_physics.py): with NR = every IBZ shell the
+  truncated-unity sum is exact, and the result must equal the SU(2) parquet
+  crossing relations (d/m/s/t, Bickers 2004; Rohringer et al., PRB 86, 125114
+  (2012)) evaluated directly on the momentum/frequency-space vertex built
+  from C4v rotations and the conjugation symmetry
+  V(k nu, k' nu'; Q, -omega) = conj V(k -nu, k' -nu'; Q, omega)
+  (tests/parquet_bruteforce.py, which uses none of the maps, tables or
+  tensors above).  Planted mistakes in the maps, the factor table, the
+  time-reversal conj/signs and the M tensors are each caught.
 """
 from __future__ import annotations
 
@@ -79,17 +85,19 @@ def calc_phir(lat: Lattice, phi: np.ndarray) -> np.ndarray:
     """PhiR_r(Nz*Nz, Nw, NR) from Phi_r(Nz, Nz, Nq); SRC/parquet_PhiR.f90:35-109, 196-258.
 
     One rank: PhiQ slot == global q (indexR_from_indexQ is the identity).
+    The l-block permutation of symop_arr does not depend on Q, so it is applied
+    after the sum over Q (to the Nz^2 x NR product) instead of to every PhiQ.
     """
     nz2 = lat.nz * lat.nz
     out = np.zeros((nz2, lat.nw, lat.nr), dtype=np.complex128, order="F")
     for w1 in range(lat.nw):
         qs = np.arange(lat.nibz) * lat.nw + w1
-        phiq = phi[:, :, qs]  # (Nz, Nz, NIBZ)
-        acc = np.zeros((nz2, lat.nr), dtype=np.complex128)
+        phiq = phi[:, :, qs].reshape((nz2, lat.nibz), order="F")
+        acc = np.zeros((lat.nz, lat.nz, lat.nr), dtype=np.complex128, order="F")
         for s in range(1, NS + 1):
-            ps = sym_permute(lat, phiq, s).reshape((nz2, lat.nibz), order="F")
-            acc += (1.0 / lat.nx) * (ps @ lat.expSqR[:, :, s - 1])
-        out[:, w1, :] = acc
+            t = (phiq @ lat.expSqR[:, :, s - 1]).reshape((lat.nz, lat.nz, lat.nr), order="F")
+            acc += (1.0 / lat.nx) * sym_permute(lat, t, s)
+        out[:, w1, :] = acc.reshape((nz2, lat.nr), order="F")
     return out
 
 
@@ -226,17 +234,18 @@ def routine_maps(routine: str, tr: bool, nf: int, qw: int, w2: int):
 
 
 def add_contribution(lat: Lattice, F: list, phir_work: np.ndarray, M: np.ndarray, routine: str,
-                     tr: bool, w2: int, fac1: float, fac2: float):
+                     tr: bool, w2: int, fac1: float, fac2: float, qs=None):
     """One calc_* routine (vectorised over l1..l4 and nu1); F updated in place.
 
     phir_work: (Nz, Nz, Nw, NR) view of one channel's PhiR.
     M: (Nl,Nl,Nl,Nl,NIBZ,NR) tensor of this routine.
+    qs: target q (0-based) to update, default all (each target q is independent).
     """
     t1, t2 = TARGETS[routine]
     nf, nl = lat.nf, lat.nl
     pw = phir_work.reshape((nf, nl, nf, nl, lat.nw, lat.nr), order="F")
     sgn = np.outer(lat.signs, lat.signs).astype(np.float64)  # (l3, l4)
-    for q in range(lat.nq):
+    for q in (range(lat.nq) if qs is None else qs):
         qw = int(lat.qw[q])
         mq = lat.q_momentum_index(q)
         maps = routine_maps(routine, tr, nf, qw, w2)
@@ -258,11 +267,13 @@ def add_contribution(lat: Lattice, F: list, phir_work: np.ndarray, M: np.ndarray
                 fv2[nu1[i], :, nu2[i], :] += contrib[i] * fac2
 
 
-def calculate_pae(lat: Lattice, phi: list, lam: list, phir: list, mt: dict):
+def calculate_pae(lat: Lattice, phi: list, lam: list, phir: list, mt: dict, qs=None):
     """calculate_PAE: F_r = Phi_r + Lambda_r*Nc (l=1 block) + crossing; SRC/parquet_EQOuter.f90:16-118.
 
     phi: 4 arrays (Nz, Nz, Nq) (G_r_LL after BSE); lam: 4 arrays (Nf, Nf, Nw);
     phir: 4 arrays (Nz*Nz, Nw, NR); mt: output of calc_m_tensors.
+    qs: optional list of target q (0-based); only those slices get the crossing
+    (the others keep Phi + Lambda*Nc), for checks at large shapes on a sample of q.
     Returns the 4 new F_r arrays.
     """
     F = [np.array(p, dtype=np.complex128, order="F", copy=True) for p in phi]
@@ -276,10 +287,10 @@ def calculate_pae(lat: Lattice, phi: list, lam: list, phir: list, mt: dict):
         work = phir[channel].reshape((lat.nz, lat.nz, lat.nw, lat.nr), order="F")
         for w2 in range(1, lat.nw + 1):
             for (routine, f1, f2) in PAE_TABLE[channel]:
-                add_contribution(lat, F, work, mt[routine], routine, False, w2, f1, f2)
+                add_contribution(lat, F, work, mt[routine], routine, False, w2, f1, f2, qs)
             if w2 > 1:
                 for (routine, f1, f2) in PAE_TABLE[channel]:
-                    add_contribution(lat, F, work, mt[routine], routine, True, w2, f1, f2)
+                    add_contribution(lat, F, work, mt[routine], routine, True, w2, f1, f2, qs)
     return F
 
 

@@ -64,14 +64,15 @@ def test_gkw_grt_cal_chi_evaluates_once():
     assert rel_fro(grt, T.grt(lat, sigma, mu, BETA)) < TOL
 
 
-def test_gkw_grt_reports_no_convergence():
+@pytest.mark.parametrize("nx,nf,ngrain", [(4, 8, 1),     # one-workgroup search
+                                          (8, 32, 4)])   # multi-workgroup passes + tree walk
+def test_gkw_grt_reports_no_convergence(nx, nf, ngrain):
     """A density the bracket cannot reach (n = 5 > 2) ends after 64 evaluations with
     TUPS_ENOCONV, where the reference would loop on; every launch exits normally."""
     from tups_amd import TupsContext, TupsError
 
-    nx, nf = 4, 8
     with TupsContext(nx=nx, nf=nf, nl=1, beta=BETA, f_damping=0.5) as c:
-        c.set_sigma(np.zeros(nx * nx * nf, dtype=np.complex128), 0.0)
+        c.set_sigma(np.zeros(nx * nx * nf, dtype=np.complex128), 0.0, ngrain=ngrain)
         with pytest.raises(TupsError) as ei:
             c.gkw_grt(U, 5.0, 0.0)
         assert "ENOCONV" in str(ei.value)

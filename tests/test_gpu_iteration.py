@@ -162,3 +162,35 @@ def test_full_iteration_with_mu_search_equals_steps(nx, nf, nl, split, ngrain):
                     assert np.array_equal(x, e.download(which, r)), (it, which, r)
             assert np.isfinite(sigma).all()
             _say(f"  it {it}: mu {mu:.6f} after {steps} evaluations")
+
+
+def test_full_iteration_graph_follows_sigma_h_rule():
+    """The Sigma commit inside the full-iteration graph applies the Sigma_H rule in force at
+    capture (Sigma_H = Sigma(k, Nf) when set_sigma got no Sigma_H, else the explicit one).
+    Switching the rule between calls must re-capture: graph and eager stay bit-identical."""
+    from tups_amd import BSE_LADDER_FP64, F, GAMMA, TupsContext
+
+    nx, nf, nl, U, mu = 4, 8, 1, 2.0, 0.1
+    rng = np.random.default_rng(61)
+    sigma0 = crandn(rng, (nx * nx * nf,), 0.1).ravel()
+    sigh = crandn(rng, (nx, nx), 0.2)
+    with TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as b, \
+            TupsContext(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as e:
+        for c in (b, e):
+            _setup(c, 37)
+        it = 0
+        for explicit in (True, False, True):
+            s = b.get_sigma() if it else sigma0
+            for c in (b, e):
+                c.set_sigma(s, mu, sigma_h=sigh if explicit else None)
+            # two iterations per rule: the second one's bubbles read the Sigma_H that the
+            # first one's commit left
+            for _ in range(2):
+                _say(f"  sigma_h rule switch, iteration {it}: explicit Sigma_H = {explicit}")
+                b.iteration_full(U, BSE_LADDER_FP64, graph=True)
+                e.iteration_full(U, BSE_LADDER_FP64, graph=False)
+                assert np.array_equal(b.get_sigma(), e.get_sigma()), it
+                for which in (GAMMA, F):
+                    for r in range(4):
+                        assert np.array_equal(b.download(which, r), e.download(which, r)), (it, which, r)
+                it += 1

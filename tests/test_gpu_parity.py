@@ -266,3 +266,36 @@ def test_every_crossing_variant_nl1(nx, nf, nr):
             c.calculate_PAE()
             for r in range(4):
                 assert rel_fro(c.download(F, r), exp[r]) < TOL, (name, r)
+
+
+def test_rejected_tables_leave_context_unchanged():
+    """tups_set_tables with an invalid irrep list returns an error and changes no table,
+    even when the same call also carries new form factors and signs."""
+    import ctypes as C
+
+    from tups_amd import TupsError
+
+    with _ctx(nx=4, nf=4, nl=5) as c:
+        lib = c.lib
+        lat = Lattice(4, 4, 5)
+        ff0 = np.zeros_like(lat.ff, order="F")
+        signs0 = np.zeros(5, np.int32)
+        reps0 = np.zeros(5, np.int32)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        assert lib.tups_get_tables(c._h, dp(ff0), None, ip(signs0), ip(reps0), None, None) == 0
+        bad_ff = np.asfortranarray(ff0 * 2.0)
+        bad_signs = -signs0
+        bad_reps = reps0.copy()
+        bad_reps[1] = 9
+        assert lib.tups_set_tables(c._h, dp(bad_ff), None, ip(bad_signs), ip(bad_reps), None, None) != 0
+        ff1 = np.zeros_like(ff0)
+        signs1 = np.zeros(5, np.int32)
+        reps1 = np.zeros(5, np.int32)
+        assert lib.tups_get_tables(c._h, dp(ff1), None, ip(signs1), ip(reps1), None, None) == 0
+        assert np.array_equal(ff1, ff0) and np.array_equal(signs1, signs0) and np.array_equal(reps1, reps0)
+        assert np.abs(ff0 - lat.ff).max() < 1e-15
+        # the context still works after the rejected call
+        c.calc_M()
+        with pytest.raises(TupsError):
+            c.calculate_PAE()  # PhiR was never formed

@@ -345,23 +345,24 @@ int tups_set_tables(tups_ctx* c, const double* ff, const double* ff_inv, const i
                     const int32_t* reps, const double* exp_sqr, const double* exp_qsr) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
   Tables& t = c->tab;
+  if (reps) {  // validate before any table changes, so a rejected call leaves the context as it was
+    for (int l = 0; l < t.nl; ++l) TUPS_CHECK(reps[l] >= 1 && reps[l] <= 6, TUPS_EINVAL, "reps out of range");
+    Tables trial = t;
+    std::copy(reps, reps + t.nl, trial.reps.begin());
+    if (int rc = rebuild_sym_l(trial)) return rc;
+  }
   if (ff) std::memcpy(t.ff.data(), ff, t.ff.size() * sizeof(double));
   if (ff_inv) std::memcpy(t.ff_inv.data(), ff_inv, t.ff_inv.size() * sizeof(double));
   if (signs) std::copy(signs, signs + t.nl, t.signs.begin());
   if (reps) {
-    for (int l = 0; l < t.nl; ++l) TUPS_CHECK(reps[l] >= 1 && reps[l] <= 6, TUPS_EINVAL, "reps out of range");
-    std::vector<int> old(t.reps);
     std::copy(reps, reps + t.nl, t.reps.begin());
-    if (int rc = rebuild_sym_l(t)) {
-      t.reps = old;
-      rebuild_sym_l(t);
-      return rc;
-    }
+    (void)rebuild_sym_l(t);  // validated above
   }
   if (exp_sqr) std::memcpy(t.exp_sqr.data(), exp_sqr, t.exp_sqr.size() * sizeof(double));
   if (exp_qsr) std::memcpy(t.exp_qsr.data(), exp_qsr, t.exp_qsr.size() * sizeof(double));
   c->tables_dirty = true;
   c->m_built = false;
+  c->phir_valid = false;  // PhiR was formed with the old symmetry/Fourier tables
   return ensure_tables(c);
 }
 

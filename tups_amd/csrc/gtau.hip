@@ -410,63 +410,65 @@ __global__ __launch_bounds__(256) void u2_ggg(GtauArgs a) {
   if (threadIdx.x == 0) a.u2data[(size_t)kidx * nf + it] = red[0];
 }
 
-// FDfit with dt = delta (SRC/MATH_mod.f90:95-107) and the cubic spline's tridiagonal
-// system with those end slopes (nfourier, SRC/MATH_mod.f90:131-160): xm receives the
-// second derivatives at the L + 1 points r (one lane; qv, uv are its scratch)
-__device__ void spline_fit(const double* r, double* xm, double* qv, double* uv, int L, double delta) {
-  const double fd1 = (-25.0 * r[0] + 48.0 * r[1] - 36.0 * r[2] + 16.0 * r[3] - 3.0 * r[4]) / 12.0 / delta;
-  const double fd2 =
-      (25.0 * r[L] - 48.0 * r[L - 1] + 36.0 * r[L - 2] - 16.0 * r[L - 3] + 3.0 * r[L - 4]) / 12.0 / delta;
-  if (fd1 > 0.99e30) {
-    qv[0] = 0.0;
-    uv[0] = 0.0;
-  } else {
-    qv[0] = -0.5;
-    uv[0] = (3.0 / delta) * ((r[1] - r[0]) / delta - fd1);
-  }
+// Clamped cubic spline through r[0..L] on the uniform grid t_k = k*h (the method of
+// Burden & Faires, Numerical Analysis, sec. 3.5): the second derivatives m_k solve
+//   2 m_0 + m_1               = (6/h) ((r_1 - r_0)/h - s_0)
+//   m_{k-1} + 4 m_k + m_{k+1} = (6/h^2) (r_{k+1} - 2 r_k + r_{k-1}),   0 < k < L
+//   m_{L-1} + 2 m_L           = (6/h) (s_L - (r_L - r_{L-1})/h)
+// with the end slopes s_0, s_L from the five-point one-sided difference
+// f'(t_0) = (-25 f_0 + 48 f_1 - 36 f_2 + 16 f_3 - 3 f_4)/(12 h) + O(h^4) (and its mirror
+// at t_L) — the end condition of the reference's tau quadrature (FDfit and nfourier,
+// SRC/MATH_mod.f90:95-160).  The tridiagonal system is solved by the Thomas algorithm on
+// one lane: cp and dp hold the eliminated super-diagonal and right-hand side.
+__device__ void spline_fit(const double* r, double* m, double* cp, double* dp, int L, double h) {
+  const double s0 = (-25.0 * r[0] + 48.0 * r[1] - 36.0 * r[2] + 16.0 * r[3] - 3.0 * r[4]) / (12.0 * h);
+  const double sL = (25.0 * r[L] - 48.0 * r[L - 1] + 36.0 * r[L - 2] - 16.0 * r[L - 3] + 3.0 * r[L - 4]) / (12.0 * h);
+  const double g = 6.0 / (h * h);
+  // row 0: diagonal 2, super-diagonal 1
+  cp[0] = 0.5;
+  dp[0] = 0.5 * (6.0 / h) * ((r[1] - r[0]) / h - s0);
+  // rows 1..L-1: sub-diagonal 1, diagonal 4, super-diagonal 1
   for (int k = 1; k < L; ++k) {
-    const double p = qv[k - 1] / 2.0 + 2.0;
-    qv[k] = -1.0 / 2.0 / p;
-    const double t = 3.0 / (delta * delta) * (r[k + 1] + r[k - 1] - 2.0 * r[k]);
-    uv[k] = (t - uv[k - 1] / 2.0) / p;
+    const double piv = 4.0 - cp[k - 1];
+    cp[k] = 1.0 / piv;
+    dp[k] = (g * (r[k + 1] - 2.0 * r[k] + r[k - 1]) - dp[k - 1]) / piv;
   }
-  double qn, un;
-  if (fd2 > 0.99e30) {
-    qn = 0.0;
-    un = 0.0;
-  } else {
-    qn = 0.5;
-    un = (3.0 / delta) * (fd2 - (r[L] - r[L - 1]) / delta);
-  }
-  xm[L] = (un - qn * uv[L - 1]) / (qn * qv[L - 1] + 1.0);
-  for (int k = L - 1; k >= 0; --k) xm[k] = qv[k] * xm[k + 1] + uv[k];
+  // row L: sub-diagonal 1, diagonal 2
+  const double dL = (6.0 / h) * (sL - (r[L] - r[L - 1]) / h);
+  m[L] = (dL - dp[L - 1]) / (2.0 - cp[L - 1]);
+  for (int k = L - 1; k >= 0; --k) m[k] = dp[k] - cp[k] * m[k + 1];
 }
 
-// the spline's exact integral against e^{i om tau} over the intervals j = j0, j0 + js, ...
-// <= L (SRC/MATH_mod.f90:170-207), added to (re, im)
-__device__ void spline_integral(const double* r, const double* xm, int L, double delta, double om, int j0, int js,
+// Exact integral of the spline against e^{i om t} over the intervals j = j0, j0 + js, ...
+// <= L (interval j spans [t_{j-1}, t_j]), added to (re, im).  On interval j the spline is
+// p(u) = a + b u + c u^2 + d u^3, u = t - t_{j-1}, with
+//   a = r_{j-1}, b = (r_j - r_{j-1})/h - h (2 m_{j-1} + m_j)/6, c = m_{j-1}/2, d = (m_j - m_{j-1})/(6h),
+// and repeated integration by parts gives the antiderivative
+//   int p e^{i om t} dt = e^{i om t} [ (p1/om^2 - p3/om^4) + i (p2/om^3 - p/om) ]
+// (pk the k-th derivative of p), evaluated at u = h and u = 0; for om -> 0 the plain
+// polynomial integral.
+__device__ void spline_integral(const double* r, const double* m, int L, double h, double om, int j0, int js,
                                 double& re, double& im) {
-  const double o2 = om * om, o3 = o2 * om, o4 = o2 * o2;
+  const bool zero = fabs(om) < 1e-6;
+  const double i1 = zero ? 0.0 : 1.0 / om, i2 = i1 * i1, i3 = i2 * i1, i4 = i2 * i2;
   for (int j = j0; j <= L; j += js) {
-    const double aj = r[j - 1], cj = xm[j - 1] / 2.0;
-    const double bj = (r[j] - r[j - 1]) / delta - (2.0 * xm[j - 1] + xm[j]) * delta / 6.0;
-    const double dj = (xm[j] - xm[j - 1]) / (6.0 * delta);
-    if (fabs(om) < 1e-6) {
-      re += aj * delta + 0.5 * bj * delta * delta + cj * delta * delta * delta / 3.0 +
-            dj * delta * delta * delta * delta / 4.0;
+    const double a = r[j - 1], c = 0.5 * m[j - 1];
+    const double b = (r[j] - r[j - 1]) / h - h * (2.0 * m[j - 1] + m[j]) / 6.0;
+    const double d = (m[j] - m[j - 1]) / (6.0 * h);
+    if (zero) {
+      re += h * (a + h * (b / 2.0 + h * (c / 3.0 + h * d / 4.0)));
       continue;
     }
+    // p and its derivatives at the interval's end (u = h) and start (u = 0)
+    const double p_h = a + h * (b + h * (c + h * d)), p1_h = b + h * (2.0 * c + 3.0 * h * d);
+    const double p2_h = 2.0 * c + 6.0 * h * d, p3 = 6.0 * d;
+    const double f1r = p1_h * i2 - p3 * i4, f1i = p2_h * i3 - p_h * i1;
+    const double f0r = b * i2 - p3 * i4, f0i = 2.0 * c * i3 - a * i1;
     double s1, c1, s0, c0;
-    sincos(om * delta * j, &s1, &c1);
-    sincos(om * delta * (j - 1), &s0, &c0);
-    // e^{i w delta j} (P1r + i P1i) + e^{i w delta (j-1)} (P0r + i P0i)
-    const double p1r = -6.0 * dj / o4 + (bj + 2.0 * delta * cj + 3.0 * delta * delta * dj) / o2;
-    const double p1i = (2.0 * cj + 6.0 * delta * dj) / o3 +
-                       (-aj - delta * bj - delta * delta * cj - delta * delta * delta * dj) / om;
-    const double p0r = 6.0 * dj / o4 - bj / o2;
-    const double p0i = -2.0 * cj / o3 + aj / om;
-    re += c1 * p1r - s1 * p1i + c0 * p0r - s0 * p0i;
-    im += c1 * p1i + s1 * p1r + c0 * p0i + s0 * p0r;
+    sincos(om * h * j, &s1, &c1);
+    sincos(om * h * (j - 1), &s0, &c0);
+    re += (c1 * f1r - s1 * f1i) - (c0 * f0r - s0 * f0i);
+    im += (c1 * f1i + s1 * f1r) - (c0 * f0i + s0 * f0r);
   }
 }
 

@@ -150,6 +150,7 @@ struct tups_ctx {
   double conv_fmax_prev[4] = {};  // max |F_r| of the previous convergence check
   double full_n_particle = -1.0; // full-iteration graphs: nParticle and Ngrain they were captured with
   int full_ngrain = 0;
+  int full_sigh_default = -1;    // and the Sigma_H rule of the Sigma commit (sigh_default)
   hipGraphExec_t graph_full[4] = {};  // full iteration, per (ladder mode, buffer parity)
   int full_f_range = -1, full_use_u2 = 0;  // settings the full-iteration graphs were captured with
   double full_u = 0.0;

@@ -165,14 +165,16 @@ extern "C" int tups_iteration_full(tups_ctx* c, int32_t mode, int32_t use_graph,
     if (!rc) c->phir_valid = c->sigma_old_valid = true;
     return rc;
   }
+  const int sigh = c->sigh_default ? 1 : 0;
   if (c->full_f_range != f_range || c->full_u != u || c->full_use_u2 != (use_u2 ? 1 : 0) ||
-      c->full_n_particle != n_particle || c->full_ngrain != c->green_ngrain) {
+      c->full_n_particle != n_particle || c->full_ngrain != c->green_ngrain || c->full_sigh_default != sigh) {
     drop_graphs(c);  // the settings are part of the captured launches
     c->full_f_range = f_range;
     c->full_u = u;
     c->full_use_u2 = use_u2 ? 1 : 0;
     c->full_n_particle = n_particle;
     c->full_ngrain = c->green_ngrain;
+    c->full_sigh_default = sigh;
   }
   const int slot = 2 * mode + c->parity;
   if (!c->graph_full[slot]) {
