@@ -27,10 +27,13 @@ Pinned by (tests/test_oracle_sde.py):
   IBZ equals a plain sum over every Q of the zone with the vertex of Q's IBZ
   representative, for random vertices -- this pins PhChi and PPChi's place in
   the sum;
-* linearity in the vertices.
-Parity unpinned (no reference output exists): the orientation (row, column)
-of G_r_LL inside PhChi/PPChi beyond Nl = 1, and the pickNu clamp for
-f_range > 0, follow the reference loops literally.
+* linearity in the vertices;
+* the Schwinger-Dyson equation evaluated directly in momentum and frequency
+  space (tests/test_oracle_sde_physics.py) for C4v-symmetric vertices in the
+  span of the Nl form factors at Nl = 1, 5, 9 and f_range = 0, 1, 2: this pins
+  the (row, column) orientation of G_r_LL in PhChi / PPChi (the external line on
+  the row), FF_inv per operation, the form-factor signs of the time-reversed
+  half, the pp term and the enlarged frequency sums with the pickNu clamp.
 """
 from __future__ import annotations
 

@@ -208,3 +208,82 @@ def symmetric_phi(lat: Lattice, rng) -> list:
         _, ref = ks.symmetric_full_zone(pl)
         out.append(ref)
     return out
+
+
+def symmetric_phi_full(lat: Lattice, rng):
+    """As symmetric_phi, also returning the full-zone k-space vertices
+    ((Nc[Q], Nw, Nc, Nf, Nc, Nf) each)."""
+    ks = KSpace(lat)
+    fulls, refs = [], []
+    for _ in range(4):
+        pl = (rng.standard_normal((lat.nf, lat.nl, lat.nf, lat.nl, lat.nibz, lat.nw))
+              + 1j * rng.standard_normal((lat.nf, lat.nl, lat.nf, lat.nl, lat.nibz, lat.nw)))
+        full, ref = ks.symmetric_full_zone(pl)
+        fulls.append(full)
+        refs.append(ref)
+    return fulls, refs
+
+
+def sde_kspace(lat: Lattice, fulls, lam_d, lam_m, sigma_k, sigma_h_k, mu, beta, U, f_range=0):
+    """The Schwinger-Dyson self-energy evaluated directly in momentum/frequency space.
+
+    Sigma(k, nu) = -U/(2 beta^2 Nc^2) sum_{Q in BZ} sum_{|omega| < Nw}
+        [ G(k+Q, nu+omega) sum_{k', nu' in box} ((L_d - L_m)(nu, nu'; omega)
+                 + Phi_d(k nu, k' nu'; Q omega) - 3 Phi_m(k nu, k' nu'; Q omega)) G(k', nu') G(k'+Q, nu'+omega)
+        + G(Q-k, omega-nu) sum_{k', nu' in box} (Phi_s + Phi_t)(k nu, k' nu'; Q omega) G(k', nu') G(Q-k', omega-nu') ]
+
+    The SU(2) SDE with one bare U vertex, F_updown = (Lambda + ph + ph-bar + pp)
+    parts, the ph-bar part relabelled into the ph form (hence Phi_d - 3 Phi_m,
+    Eckhardt, Honerkamp, Held, Kauch, arXiv:1912.07469, the paper the
+    reference's README cites), the
+    local parquet Lambda entering as a k-independent vertex.  The external line k
+    attaches to the vertex's first pair; the bubble to its second.  Negative omega
+    through V(k nu, k' nu'; Q, -omega) = conj V(k -nu, k' -nu'; Q, omega) for the
+    vertices and Lambda.  G(k, nu) = 1 / (i nu - eps_k + mu - Sigma(k, nu)) with
+    Sigma_H(k) outside the box.  fulls: symmetric_phi_full's k-space vertices;
+    lam_*: (Nf, Nf, Nw) as stored; sigma_k: (Nc, Nf) with k = kx + Nx ky,
+    sigma_h_k: (Nc,).  f_range > 0 extends the nu' sums to (2 f_range + 1) Nf
+    frequencies: there the vertices take their value at the nearest box frequency
+    (the reference's asymptotics, pickNu) and Lambda its bare limit U_d - U_m = 2U.
+    Returns Sigma (Nc, Nf)."""
+    ks = KSpace(lat)
+    nx, nf, nw, nc = lat.nx, lat.nf, lat.nw, ks.nc
+    kx = 2.0 * np.pi * ks.kvec[:, 0] / nx
+    ky = 2.0 * np.pi * ks.kvec[:, 1] / nx
+    eps = -2.0 * np.cos(kx) - 2.0 * np.cos(ky)
+
+    def G(k, n):
+        """k: k index array, n: box index array (any integer; outside the box Sigma_H)."""
+        k, n = np.broadcast_arrays(k, n)
+        inside = (n >= 0) & (n < nf)
+        nu = (2.0 * (n - nf // 2) + 1.0) * np.pi / beta
+        sig = np.where(inside, sigma_k[k, np.clip(n, 0, nf - 1)], sigma_h_k[k])
+        return 1.0 / (1j * nu - eps[k] + mu - sig)
+
+    def vertex(full, Q, om):
+        if om >= 0:
+            return full[Q, om]
+        return np.conj(full[Q, -om][:, ::-1, :, ::-1])
+
+    def lam(L, om):
+        return L[:, :, om] if om >= 0 else np.conj(L[::-1, ::-1, -om])
+
+    kk = np.arange(nc)[:, None]
+    nn = np.arange(nf)[None, :]
+    ne = np.arange(-f_range * nf, (f_range + 1) * nf)[None, :]   # the nu' range
+    ce = np.clip(ne[0], 0, nf - 1)                                # its nearest box frequency
+    inside = (ne[0] >= 0) & (ne[0] < nf)
+    out = np.zeros((nc, nf), dtype=np.complex128)
+    for Q in range(nc):
+        for om in range(-(nw - 1), nw):
+            b_ph = G(kk, ne) * G(ks.add(kk, Q), ne + om)                     # (k', n')
+            b_pp = G(kk, ne) * G(ks.add(np.full_like(kk, Q), kk, -1), om + nf - 1 - ne)
+            v_ph = vertex(fulls[0], Q, om) - 3.0 * vertex(fulls[1], Q, om)    # (k, n, k', n' in box)
+            v_pp = vertex(fulls[2], Q, om) + vertex(fulls[3], Q, om)
+            x_ph = np.einsum("anbm,bm->an", v_ph[:, :, :, ce], b_ph)
+            lam_e = np.where(inside[None, :], (lam(lam_d, om) - lam(lam_m, om))[:, ce], 2.0 * U)
+            x_ph += np.einsum("nm,bm->n", lam_e, b_ph)[None, :]
+            x_pp = np.einsum("anbm,bm->an", v_pp[:, :, :, ce], b_pp)
+            out += G(ks.add(kk, Q), nn + om) * x_ph
+            out += G(ks.add(np.full_like(kk, Q), kk, -1), om + nf - 1 - nn) * x_pp
+    return -U / (2.0 * beta * beta * nc * nc) * out
