@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nq", type=int, default=1024)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="CPU baseline budget of the ladder sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the D=576, Nq=4096 (261 GB) share")
     ap.add_argument("--moduli", type=int, default=12, help="CRT moduli of the split form")
@@ -58,21 +58,14 @@ def profile_traffic(name):
     return None
 
 
-def cpu_baseline(nx, nf, nl, budget_s):
-    """Oracle ladder (4 ZGEMMs per channel, host BLAS) on a bounded sample of q."""
+def _ladder_rate(lat, budget_s):
+    """q-points/s of the oracle ladder (4 ZGEMMs per channel, host BLAS) over about budget_s."""
     import numpy as np
 
     from oracle import bse as B
-    from oracle.lattice import Lattice
 
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
-    except Exception:
-        threads = os.cpu_count() or 1
-    lat = Lattice(nx, nf, nl)
     rng = np.random.default_rng(1)
-    D = lat.nz
+    D, nl, nf = lat.nz, lat.nl, lat.nf
     nq = 2
     mk = lambda: np.asfortranarray(rng.standard_normal((D, D, nq)) + 1j * rng.standard_normal((D, D, nq)))
     gam = [mk() for _ in range(4)]
@@ -86,10 +79,61 @@ def cpu_baseline(nx, nf, nl, budget_s):
         done += nq
         el = time.perf_counter() - t0
         if el >= budget_s:
-            break
-    out = {"value": done / el, "unit": "q-points/s", "cores": int(threads), "kind": "port",
-           "sample": f"{done} q-points of the D={D} ladder (4 channels x 4 ZGEMM + mixing each, "
+            return done, el
+
+
+def _crossing_cpu(nx, nf, nl, ws_sample, q_sample):
+    """Oracle calc_PhiR (whole call, or extrapolated from a sample of bosonic frequencies)
+    and calculate_PAE per target q (a sample of q) at one shape, one host process."""
+    import numpy as np
+
+    from oracle import parquet as P
+    from oracle.lattice import Lattice
+
+    lat = Lattice(nx, nf, nl)
+    # the work does not depend on the values: constant inputs of the right shape
+    phi = [np.full((lat.nz, lat.nz, lat.nq), 0.5 + 0.25j * r, dtype=np.complex128, order="F") for r in range(4)]
+    lam = [np.full((nf, nf, lat.nw), 0.1 * r, dtype=np.complex128, order="F") for r in range(4)]
+    ws = list(range(min(ws_sample, lat.nw)))
+    t0 = time.perf_counter()
+    phir = [P.calc_phir(lat, p, ws=ws) for p in phi]
+    t_phir = (time.perf_counter() - t0) * lat.nw / len(ws)
+    mt = P.calc_m_tensors(lat)
+    qs = list(range(q_sample))
+    t0 = time.perf_counter()
+    P.calculate_pae(lat, phi, lam, phir, mt, qs=qs)
+    t_pae = time.perf_counter() - t0
+    return {"shape": f"Nx={nx} Nf={nf} Nl={nl} NR={lat.nr} Nq={lat.nq}",
+            "phir_ms_per_call": round(t_phir * 1e3, 1),
+            "phir_sample": f"{len(ws)} of {lat.nw} bosonic frequencies, scaled to the whole call",
+            "pae_q_per_s": round(q_sample / t_pae, 3), "pae_ms_per_call": round(t_pae / q_sample * lat.nq * 1e3, 1),
+            "pae_sample": f"{q_sample} target q of {lat.nq}, scaled to the whole call"}
+
+
+def cpu_baseline(nx, nf, nl, budget_s):
+    """The oracle restatement on the host cores: the ladder on a bounded sample of q with
+    every BLAS thread and with one, calc_PhiR and calculate_PAE at the crossing shapes, and
+    the config-1 iteration.  One process (the reference's MPI split collapses to one rank)."""
+    from oracle.lattice import Lattice
+
+    try:
+        from threadpoolctl import threadpool_info, threadpool_limits
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        threadpool_limits = None
+        threads = os.cpu_count() or 1
+    lat = Lattice(nx, nf, nl)
+    done, el = _ladder_rate(lat, budget_s)
+    out = {"value": done / el, "unit": "q-points/s", "cores": int(threads), "mpi_ranks": 1, "kind": "port",
+           "sample": f"{done} q-points of the D={lat.nz} ladder (4 channels x 4 ZGEMM + mixing each, "
                      f"oracle restatement of SRC/parquet_BSE.f90:62-132 on host BLAS), {el:.1f} s"}
+    if threadpool_limits is not None:
+        with threadpool_limits(limits=1, user_api="blas"):
+            d1, e1 = _ladder_rate(lat, budget_s / 2)
+        out["one_thread"] = {"value": d1 / e1, "unit": "q-points/s", "cores": 1,
+                             "sample": f"{d1} q-points, {e1:.1f} s"}
+    out["crossing_nl1"] = _crossing_cpu(16, 64, 1, 32, 2)
+    out["crossing_nl5"] = _crossing_cpu(8, 64, 5, 1, 1)
     out["config1_iteration"] = config1_cpu()
     return out
 
@@ -227,6 +271,27 @@ def large_share(moduli):
     return out
 
 
+def phir_leg(c):
+    """calc_PhiR on a full-IBZ context: HBM bytes (Phi read once per eight shells,
+    PhiR written once) per call against 8 TB/s."""
+    from tups_amd.runtime import K_PHIR
+
+    c.calc_PhiR(0)
+    c.set_timing(True)
+    for _ in range(5):
+        c.calc_PhiR(0)
+    ms, n = c.kernel_time(K_PHIR)
+    c.set_timing(False)
+    nz2 = float(c.nz) ** 2
+    nbytes = 16.0 * (4 * nz2 * c.nq * ((c.nr + 7) // 8) + 4 * nz2 * c.nw * c.nr)
+    gbs = nbytes / (ms / n / 1e3) / 1e9
+    print(f"[bench] PhiR Nx={c.nx} Nl={c.nl}: {ms / n:.3f} ms per call, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "avg_launch_ms": round(ms / n, 4), "bytes_per_call": nbytes,
+            "kernel": "phir_block (one launch per symmetry-block shape)",
+            "workload": f"calc_PhiR Nx={c.nx} Nf={c.nf} Nl={c.nl} NR={c.nr}, Nq={c.nq}, 4 channels"}
+
+
 def main():
     import numpy as np
 
@@ -236,8 +301,8 @@ def main():
               "of this path is a design note, not a run", file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from tups_amd import BSE_CLOSED_LU, BSE_LADDER_FP64, BSE_LADDER_SPLIT, GAMMA, TupsContext
-    from tups_amd.runtime import K_BSE_CLOSED, K_BSE_LADDER, K_SPLIT_GEMM, K_SPLIT_PREP
+    from tups_amd import BSE_CLOSED_IR, BSE_CLOSED_LU, BSE_LADDER_FP64, BSE_LADDER_SPLIT, GAMMA, TupsContext
+    from tups_amd.runtime import K_BSE_CLOSED, K_BSE_IR, K_BSE_LADDER, K_SPLIT_GEMM, K_SPLIT_PREP
     from tups_amd.synth import batched_bse_inputs
 
     nx, nf, nl, U = 8, 64, 5, 2.0
@@ -259,6 +324,15 @@ def main():
         return ctx.elapsed_ms(0, 1)
 
     # --- FP64 form ---------------------------------------------------------------------
+    # the three-multiplication complex products first (reported beside), then the
+    # four-product variants (the reference's ZGEMM arithmetic): the plain path
+    ctx.set_product_forms(True)
+    var3, _ = ctx.autotune()
+    ms3 = timed(BSE_LADDER_FP64)
+    qps3 = nq * args.steps / (ms3 / 1e3)
+    print(f"[bench] fp64 with three-multiplication products ({var3}): {qps3:.1f} q-points/s", file=sys.stderr,
+          flush=True)
+    ctx.set_product_forms(False)
     variant, tune_ms = ctx.autotune()
     print(f"[bench] autotuner picked {variant} ({tune_ms:.2f} ms/launch)", file=sys.stderr, flush=True)
     ms64 = timed(BSE_LADDER_FP64)
@@ -282,6 +356,9 @@ def main():
     # --- split form ----------------------------------------------------------------------
     ctx.set_split(args.moduli, args.qbatch)
     batched_bse_inputs(ctx, U=U, seed=0)
+    svar, _ = ctx.autotune(K_SPLIT_GEMM)
+    print(f"[bench] split GEMM tile: {svar}", file=sys.stderr, flush=True)
+    batched_bse_inputs(ctx, U=U, seed=0)
     ms_sp = timed(BSE_LADDER_SPLIT)
     qps = nq * args.steps / (ms_sp / 1e3)
     print(f"[bench] split: {args.steps} steps {ms_sp:.1f} ms -> {qps:.1f} q-points/s", file=sys.stderr, flush=True)
@@ -298,7 +375,7 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(ach_sp, 1), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                 "frac": round(ach_sp / INT8_PEAK_TOPS, 4),
                 "traffic": profile_traffic("bse_split_crt_D320_nq1024.md"),
-                "kernel": f"crt_gemm ({args.moduli} moduli)", "avg_launch_ms": round(gemm_ms / gemm_n, 4),
+                "kernel": f"split GEMM {svar} ({args.moduli} moduli)", "avg_launch_ms": round(gemm_ms / gemm_n, 4),
                 "ops_per_launch": ops_per_launch, "prep_ms_per_launch": round(prep_ms / prep_n, 4),
                 "launches_per_step": gemm_n // nprof}
     # --- closed form (fixed point by pivoted Gauss-Jordan) --------------------------------
@@ -318,6 +395,25 @@ def main():
               "unit_roofline": "TFLOP/s", "frac": round(cl_flops / (cl_step_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
               "flops_per_call": cl_flops, "kernel": "closed_build + closed_solve"}
     print(f"[bench] closed form: {cl_step_ms:.1f} ms per call", file=sys.stderr, flush=True)
+    # --- closed form by FP32 LU (Gauss-Jordan inverse) + FP64 iterative refinement -------
+    batched_bse_inputs(ctx, U=U, seed=0)
+    ctx.BSE(0, mode=BSE_CLOSED_IR)  # warm-up (allocates its workspace)
+    ctx.sync()
+    ctx.set_timing(True)
+    for _ in range(ncl):
+        ctx.BSE(0, mode=BSE_CLOSED_IR)
+    ir_ms, _ = ctx.kernel_time(K_BSE_IR)
+    ctx.set_timing(False)
+    ir_steps, ir_res = ctx.solve_info()
+    ir_step_ms = ir_ms / ncl
+    refined = {"value": round(nq / (ir_step_ms / 1e3), 2), "unit": "q-points/s", "ms_per_call": round(ir_step_ms, 3),
+               "refinement_steps": ir_steps, "residual": ir_res, "residual_bar": 1e-12,
+               "vs_direct_fp64": round(cl_step_ms / ir_step_ms, 3),
+               "kernel": "ir_build32 + closed_solve<float> (FP32 MFMA) + ir_gemm (FP32 / FP64 MFMA) + ir_chi + ir_check",
+               "flops_per_call": {"fp32": (12.0 + 8.0 * (1 + ir_steps)) * D ** 3 * 4 * nq,
+                                  "fp64": 8.0 * (ir_steps + 1) * D ** 3 * 4 * nq}}
+    print(f"[bench] refined closed form: {ir_step_ms:.1f} ms per call, {ir_steps} steps, residual {ir_res:.1e}",
+          file=sys.stderr, flush=True)
     # --- convergence check: max |F_r| over the four resident vertices (HBM read) -----------
     from tups_amd.runtime import K_CONV
     nt = nx * nx * nf
@@ -349,6 +445,13 @@ def main():
     for r in range(4):
         got = ec.download(GAMMA, r)
         errs.append(float(np.linalg.norm(got - ref[r]) / np.linalg.norm(ref[r])))
+    batched_bse_inputs(ec, U=U, seed=5)
+    ec.BSE(0, mode=BSE_CLOSED_LU)
+    ref_cl = [ec.download(GAMMA, r) for r in range(4)]
+    batched_bse_inputs(ec, U=U, seed=5)
+    ec.BSE(0, mode=BSE_CLOSED_IR)
+    refined["error_vs_fp64_direct"] = [float(np.linalg.norm(ec.download(GAMMA, r) - ref_cl[r])
+                                             / np.linalg.norm(ref_cl[r])) for r in range(4)]
     ec.close()
     print(f"[bench] split error per channel: {errs}", file=sys.stderr, flush=True)
 
@@ -378,6 +481,7 @@ def main():
                 "avg_launch_ms": round(pae_ms / pae_n, 4), "pregather_ms": round(gat_ms / max(1, gat_n), 4),
                 "bytes_per_call": cbytes,
                 "workload": f"calculate_PAE Nx=16 Nf=64 Nl=1 NR=1, Nq={cx.nq} (all IBZ q), 4 channels"}
+    phir1 = phir_leg(cx)
     cx.close()
 
     # crossing at Nl = 5 (Nx = 8, Nf = 64, NR = 4, all 480 IBZ q): the compute-heavy regime
@@ -402,6 +506,7 @@ def main():
                  "kernel": "pae_ybuild + pae_mm (FP64 MFMA GEMM per bosonic frequency)",
                  "workload": f"calculate_PAE Nx=8 Nf=64 Nl=5 NR={c5.nr}, Nq={c5.nq} (all IBZ q), 4 channels"}
     print(f"[bench] crossing Nl=5: {p5_ms / p5_n:.2f} ms per call", file=sys.stderr, flush=True)
+    phir5 = phir_leg(c5)
     c5.close()
 
     # batching form: 1024 q with Nf_q 64 -> 16 by bosonic frequency and Nl_q in {1, 5, 9}
@@ -426,6 +531,28 @@ def main():
                             f"Nl_q 1/5/9 by momentum, D {min(dims)}..{max(dims)}, FP64"}
     print(f"[bench] batching: {bt_ms:.2f} ms per launch", file=sys.stderr, flush=True)
     bt.close()
+
+    # uneven-box iteration (BSE + crossing + G = F - G) replayed as one HIP graph
+    from tups_amd.profile_workloads import uneven_context
+    un = uneven_context()
+    un.batch.autotune()
+    un.iteration(graph=True)
+    un.sync()
+    un.ctx.record(0)
+    for _ in range(args.steps):
+        un.iteration(graph=True)
+    un.ctx.record(1)
+    un.sync()
+    un_ms = un.ctx.elapsed_ms(0, 1) / args.steps
+    udims = un.batch.dims()
+    uneven = {"value": round(un.batch.nq / (un_ms / 1e3), 2), "unit": "q-points/s", "ms_per_iteration": round(un_ms, 3),
+              "iterations": args.steps,
+              "workload": f"Nx=8, Nq={un.batch.nq} (15 IBZ momenta x 32 bosonic freq), Nf_w 64->16 by frequency, "
+                          f"Nl_m 1/5/9 by momentum, D {min(udims)}..{max(udims)}; one iteration = batched BSE "
+                          f"(one launch) + crossing on the largest box (Nf=64, Nl=9, NR={un.ctx.nr}) + "
+                          f"G = F - G, as one replayed HIP graph"}
+    print(f"[bench] uneven-box iteration: {un_ms:.2f} ms", file=sys.stderr, flush=True)
+    un.close()
 
     # self-energy (SDE without the use_U2 term) at Nx=8, Nf=64, Nl=5, all 480 IBZ q
     from tups_amd import LAMBDA
@@ -542,10 +669,19 @@ def main():
         "fp64_path": {"value": round(qps64, 2), "unit": "q-points/s", "ms_per_step": round(ms64 / args.steps, 3),
                       "roofline": roof64},
         "split_vs_fp64": round(qps / qps64, 3),
+        "fp64_three_mult": {"value": round(qps3, 2), "unit": "q-points/s", "variant": var3,
+                            "split_vs": round(qps / qps3, 3),
+                            "note": "three-multiplication complex products: meets the complex relative "
+                                    "Frobenius bar, but loses relative accuracy on a small imaginary part "
+                                    "(7e-9 under cancellation); opt-in, not the parity path"},
         "closed_form": closed,
+        "closed_form_refined": refined,
+        "uneven_iteration": uneven,
         "roofline": roofline,
         "crossing_roofline": crossing,
         "crossing_nl5": crossing5,
+        "phir_nl1": phir1,
+        "phir_nl5": phir5,
         "batching": batching,
         "large_share": large,
         "sde": sde,

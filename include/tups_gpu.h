@@ -75,6 +75,11 @@ extern "C" {
                                    Phi = (I - kappa_r Gamma chi)^-1 Gamma - Gamma, kappa_r =
                                    2 c_r alpha, by pivoted Gauss-Jordan (FP64 MFMA updates);
                                    returns TUPS_ENONFINITE on a zero pivot column */
+#define TUPS_BSE_CLOSED_IR 3    /* the same closed form by FP32 LU (pivoted Gauss-Jordan on
+                                   [fp32(A) | I], FP32 MFMA) and FP64 iterative refinement
+                                   X += A32^-1 (Gamma - A X) until |R|_F / |Gamma|_F <= the
+                                   tolerance of tups_set_refinement (default 1e-12, at most
+                                   10 steps; TUPS_ENOCONV otherwise) */
 
 typedef struct tups_dims {
   int32_t nx;        /* Nx = Ny (even), SRC/parquet_ini.f90:304-305 */
@@ -153,6 +158,13 @@ int tups_upload_chi_dense(tups_ctx* ctx, int32_t pp, int32_t q0, int32_t nqs,
 /* --- the hot path (stream-ordered) */
 /* BSE(ite): Gamma_r <- Phi_r for r = d,m,s,t; SRC/parquet_BSE.f90:62-132 */
 int tups_bse(tups_ctx* ctx, int32_t mode);
+/* TUPS_BSE_CLOSED_IR: refine until |Gamma - A X|_F / |Gamma|_F <= tol for every
+ * (q, r), with at most max_steps refinement steps (defaults 1e-12, 10) */
+int tups_set_refinement(tups_ctx* ctx, double tol, int32_t max_steps);
+/* after a closed-form tups_bse: the refinement steps the last call took (the
+ * most over its passes) and the largest relative residual it reached; 0 and
+ * NaN after the direct mode, which computes no residual */
+int tups_bse_solve_info(const tups_ctx* ctx, int32_t* steps, double* residual);
 /* calc_chi for every q slot (SRC/parquet_chi.f90:438-523 with get_green_coarse,
  * SRC/parquet_util.f90:1303-1327): fills the resident TUPS_CHI bubbles (ph:
  * channel 0, pp: channel 1) from the self-energy.  sigma: Sigma(Nt) complex,
@@ -284,6 +296,32 @@ int tups_batch_sync(tups_batch* batch);
  * it and keeps the fastest; tups_variant_name(TUPS_K_BSE_BATCH, v) names v. */
 int tups_batch_autotune(tups_batch* batch, int32_t* variant, float* ms);
 int tups_batch_set_variant(tups_batch* batch, int32_t variant);
+int tups_batch_set_product_forms(tups_batch* batch, int32_t allow_three);
+
+/* --- uneven-box iteration (no counterpart in the reference, which fixes one Nf
+ * and Nl for every q): the IBZ q = (m, w) of an Nx lattice, q = m Nw + w as in
+ * the reference (Nw = nf_max / 2), with the fermionic box nf_w[w] (even, at
+ * most nf_max, nf_max for some w) per bosonic frequency and nl_m[m] in {1, 5, 9}
+ * per IBZ momentum.  It owns a batch (the per-q boxes; upload GAMMA, F and CHI
+ * there through tups_batch_upload) and a context for the largest box (Nf =
+ * nf_max, Nl = max nl_m, full IBZ; upload LAMBDA there; its M tensors are
+ * built at creation), both on one stream.  tups_uneven_parts hands them out
+ * (owned by the iteration: do not destroy them).  tups_uneven_iteration runs
+ * BSE (one persistent launch, SRC/parquet_BSE.f90:62-132 per q) -> crossing ->
+ * G = F - G (SRC/loop.f90:219-222) with the crossing defined as the reference's
+ * calc_PhiR + calculate_PAE on the largest box of the vertices embedded in it
+ * (form factors beyond Nl_q zero, frequencies outside q's box clamped to its
+ * edge, as pickNu does) and restricted to each q's box; on uniform boxes this
+ * is the reference's iteration.  use_graph != 0 captures it once per buffer
+ * parity as a HIP graph and replays it.  Stream-ordered: tups_uneven_sync
+ * waits. */
+typedef struct tups_uneven tups_uneven;
+int tups_uneven_create(int32_t nx, int32_t nf_max, const int32_t* nf_w, const int32_t* nl_m, double beta,
+                       double f_damping, tups_uneven** out);
+int tups_uneven_destroy(tups_uneven* u);
+int tups_uneven_parts(tups_uneven* u, tups_batch** batch, tups_ctx** ctx);
+int tups_uneven_iteration(tups_uneven* u, int32_t use_graph);
+int tups_uneven_sync(tups_uneven* u);
 
 /* calc_PhiR(ite): PhiR_r from Gamma_r (holding Phi_r); SRC/parquet_PhiR.f90:35-109 */
 int tups_phir(tups_ctx* ctx);
@@ -295,11 +333,20 @@ int tups_pae(tups_ctx* ctx);
 int tups_gamma_update(tups_ctx* ctx);
 
 /* Project autotuner: times the tile variants of `kernel` (TUPS_K_BSE_LADDER,
- * or TUPS_K_PAE for Nl = 1 after PhiR and M exist) on the context's resident
- * inputs and keeps the fastest for later calls.
+ * TUPS_K_SPLIT_GEMM — the GEMM tile of the split ladder, timed as whole split
+ * steps — or TUPS_K_PAE for Nl = 1 after PhiR and M exist) on the context's
+ * resident inputs and keeps the fastest for later calls.
  * Synchronous.  `variant` receives the chosen variant id, `ms` its time per
  * launch; tups_variant_name() names a variant id. */
 int tups_autotune(tups_ctx* ctx, int32_t kernel, int32_t* variant, float* ms);
+/* Whether the autotuner of the FP64 ladder (and of a batch,
+ * tups_batch_set_product_forms) may pick the three-multiplication complex
+ * products (Re = P1 - P2, Im = P3 - P1 - P2).  They meet the relative Frobenius
+ * bar of the complex result, but a small imaginary part next to a large real
+ * one loses relative accuracy (7e-9 where the four-product form, the
+ * reference's ZGEMM arithmetic, keeps 1e-15), so the default is 0: the four-
+ * product variants only.  tups_set_variant can still force any variant. */
+int tups_set_product_forms(tups_ctx* ctx, int32_t allow_three);
 const char* tups_variant_name(int32_t kernel, int32_t variant);
 /* forces a variant id (must match the context's Nl); -1 restores the default */
 int tups_set_variant(tups_ctx* ctx, int32_t kernel, int32_t variant);
@@ -330,7 +377,8 @@ int tups_event_elapsed(tups_ctx* ctx, int32_t a, int32_t b, float* ms);
 #define TUPS_K_GTAU 12        /* G(tau): mu bisection, fine-grid G(k, tau), 2D transform */
 #define TUPS_K_CONV 13        /* convergence check: Sigma change, vertex maxima */
 #define TUPS_K_SUS 14         /* susceptibilities: Green's-function pairs and their contraction */
-#define TUPS_K_COUNT 15
+#define TUPS_K_BSE_IR 15      /* refined closed form: FP32 inverse, products with it, FP64 residuals */
+#define TUPS_K_COUNT 16
 int tups_set_timing(tups_ctx* ctx, int32_t on);
 int tups_get_timing(const tups_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);
 /* scans the resident array for NaN/Inf; returns TUPS_ENONFINITE if any */

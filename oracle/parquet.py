@@ -81,16 +81,18 @@ def sym_permute(lat: Lattice, phi: np.ndarray, s: int) -> np.ndarray:
     return out.reshape(phi.shape, order="F")
 
 
-def calc_phir(lat: Lattice, phi: np.ndarray) -> np.ndarray:
+def calc_phir(lat: Lattice, phi: np.ndarray, ws=None) -> np.ndarray:
     """PhiR_r(Nz*Nz, Nw, NR) from Phi_r(Nz, Nz, Nq); SRC/parquet_PhiR.f90:35-109, 196-258.
 
     One rank: PhiQ slot == global q (indexR_from_indexQ is the identity).
     The l-block permutation of symop_arr does not depend on Q, so it is applied
     after the sum over Q (to the Nz^2 x NR product) instead of to every PhiQ.
+    ws: optional bosonic frequencies (0-based) to form; the others stay zero
+    (for timing a sample of a large call).
     """
     nz2 = lat.nz * lat.nz
     out = np.zeros((nz2, lat.nw, lat.nr), dtype=np.complex128, order="F")
-    for w1 in range(lat.nw):
+    for w1 in (range(lat.nw) if ws is None else ws):
         qs = np.arange(lat.nibz) * lat.nw + w1
         phiq = phi[:, :, qs].reshape((nz2, lat.nibz), order="F")
         acc = np.zeros((lat.nz, lat.nz, lat.nr), dtype=np.complex128, order="F")
@@ -301,3 +303,49 @@ def lambda_pa(lat: Lattice, U: float):
             np.full(shape, -U, dtype=np.complex128, order="F"),
             np.full(shape, 2.0 * U, dtype=np.complex128, order="F"),
             np.zeros(shape, dtype=np.complex128, order="F")]
+
+
+# ---- uneven boxes ------------------------------------------------------------------
+#
+# The reference fixes one Nf and one Nl for every q (SRC/parquet_ini.f90:289-348),
+# so a crossing between q points with different boxes has no reference semantics.
+# The library defines it as the uniform crossing on the largest box of the
+# vertices embedded in it (form factors beyond Nl_q zero, frequencies outside
+# q's box clamped to its edge, as pickNu does, SRC/parquet_EQContribs.f90:17-39),
+# restricted to each q's box.  These restate that definition; on uniform boxes
+# both are the identity.  Parity unpinned: no reference output exists.
+
+def uneven_embed(lat_max: Lattice, blocks: list, nf_q, nl_q) -> np.ndarray:
+    """Per-q blocks (D_q, D_q) -> (Nz, Nz, Nq) on the largest box lat_max."""
+    nz, nf = lat_max.nz, lat_max.nf
+    out = np.zeros((nz, nz, len(blocks)), dtype=np.complex128, order="F")
+    n = np.arange(nf) - nf // 2
+    for q, b in enumerate(blocks):
+        f, l = int(nf_q[q]), int(nl_q[q])
+        c = np.clip(n, -(f // 2), f // 2 - 1) + f // 2
+        src = np.concatenate([l1 * f + c for l1 in range(l)])
+        dst = np.concatenate([l1 * nf + np.arange(nf) for l1 in range(l)])
+        out[np.ix_(dst, dst, [q])] = np.asarray(b)[np.ix_(src, src)][:, :, None]
+    return out
+
+
+def uneven_extract(lat_max: Lattice, full: np.ndarray, nf_q, nl_q) -> list:
+    """(Nz, Nz, Nq) on the largest box -> per-q blocks (D_q, D_q)."""
+    nf = lat_max.nf
+    out = []
+    for q in range(full.shape[2]):
+        f, l = int(nf_q[q]), int(nl_q[q])
+        sh = nf // 2 - f // 2
+        rows = np.concatenate([l1 * nf + sh + np.arange(f) for l1 in range(l)])
+        out.append(np.asfortranarray(full[np.ix_(rows, rows, [q])][:, :, 0]))
+    return out
+
+
+def crossing_uneven(lat_max: Lattice, phi_blocks: list, lam: list, nf_q, nl_q, mt=None) -> list:
+    """F_r per q box from the per-q Phi_r blocks (4 lists): embed, calc_phir,
+    calculate_pae on the largest box, extract."""
+    emb = [uneven_embed(lat_max, phi_blocks[r], nf_q, nl_q) for r in range(4)]
+    phir = [calc_phir(lat_max, e) for e in emb]
+    mt = calc_m_tensors(lat_max) if mt is None else mt
+    f = calculate_pae(lat_max, emb, lam, phir, mt)
+    return [uneven_extract(lat_max, f[r], nf_q, nl_q) for r in range(4)]

@@ -98,3 +98,63 @@ def test_closed_form_reports_singular_system():
         c.upload(CHI, 1, np.asfortranarray(np.zeros((1, 1, nf, nq), complex)))
         with pytest.raises(TupsError):
             c.BSE(0, mode=BSE_CLOSED_LU)
+
+
+@pytest.mark.parametrize(
+    "nx,nf,nl,nq",
+    [(4, 16, 1, 0), (4, 8, 5, 0), (4, 6, 5, 3), (4, 8, 9, 2), (8, 64, 5, 2)],
+)
+def test_refined_closed_form_parity(nx, nf, nl, nq):
+    """FP32 LU (Gauss-Jordan inverse) + FP64 iterative refinement: the residual
+    |Gamma - A X|_F / |Gamma|_F reaches 1e-12 for every (q, r) and X matches the
+    oracle's FP64 LAPACK solve to cond(A) times that; the FP32 inverse alone
+    (no refinement step allowed) does not reach the tolerance and is reported."""
+    from tups_amd import BSE_CLOSED_IR, CHI, F, GAMMA, TupsError
+
+    _say(f"refined closed parity nx={nx} nf={nf} nl={nl} nq={nq}")
+    rng = np.random.default_rng(nx * 31 + nf + nl)
+    lat = Lattice(nx, nf, nl)
+    with _ctx(nx=nx, nf=nf, nl=nl, nq=nq, beta=2.0, f_damping=0.5) as c:
+        gam = random_vertices(rng, c.nz, c.nq, diag=2.0, scale=0.3)
+        chic = [random_chi_compact(rng, nl, nf, c.nq, 3.0) for _ in range(2)]
+        for p in range(2):
+            c.upload(CHI, p, chic[p])
+
+        def load():
+            for r in range(4):
+                c.upload(GAMMA, r, gam[r])
+                c.upload(F, r, gam[r])
+
+        load()
+        c.BSE(0, mode=BSE_CLOSED_IR)
+        steps, res = c.solve_info()
+        got = [c.download(GAMMA, r) for r in range(4)]
+        _say(f"  refinement steps {steps}, residual {res:.2e}")
+        assert 1 <= steps <= 10 and res <= 1e-12
+        load()
+        c.set_refinement(1e-12, 0)
+        with pytest.raises(TupsError, match="did not reach"):
+            c.BSE(0, mode=BSE_CLOSED_IR)
+        steps0, res0 = c.solve_info()
+        assert steps0 == 0 and 1e-12 < res0 < 1e-4
+    exp = B.bse_closed(lat, gam, [B.chi_compact_to_dense(lat, x) for x in chic], 2.0)
+    for r in range(4):
+        err = rel_fro(got[r], exp[r])
+        _say(f"  channel {r}: rel err {err:.2e}")
+        assert err < 1e-11
+
+
+def test_refined_closed_form_reports_singular_system():
+    from tups_amd import BSE_CLOSED_IR, CHI, GAMMA, TupsError
+
+    nx, nf, nl = 4, 8, 1
+    with _ctx(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as c:
+        D, nq = c.nz, c.nq
+        eye = np.asfortranarray(np.repeat(np.eye(D, dtype=complex)[:, :, None], nq, axis=2))
+        for r in range(4):
+            c.upload(GAMMA, r, eye)
+        kappa_d = 2 * 0.5 * c.alpha
+        c.upload(CHI, 0, np.asfortranarray(np.full((1, 1, nf, nq), 1.0 / kappa_d, complex)))
+        c.upload(CHI, 1, np.asfortranarray(np.zeros((1, 1, nf, nq), complex)))
+        with pytest.raises(TupsError):
+            c.BSE(0, mode=BSE_CLOSED_IR)

@@ -299,3 +299,50 @@ def test_rejected_tables_leave_context_unchanged():
         c.calc_M()
         with pytest.raises(TupsError):
             c.calculate_PAE()  # PhiR was never formed
+
+
+@pytest.mark.parametrize("nx,nf,nl,nq", [(8, 64, 5, 2), (4, 16, 1, 0), (4, 8, 9, 0)])
+def test_ladder_product_forms_under_cancellation(nx, nf, nl, nq):
+    """Gamma, F and chi nearly real (imaginary parts 1e-7 of the real ones) and
+    large: Im(Gamma chi F) is then a small difference for the three-multiplication
+    forms (Im = P3 - P1 - P2).  Every variant must still meet the parity bar on the
+    complex result (relative Frobenius <= 1e-12); the four-product forms (the
+    reference's ZGEMM arithmetic) must also give the small imaginary part alone to
+    1e-12, while the three-product forms' error on it is reported."""
+    from tups_amd import CHI, F, GAMMA
+
+    rng = np.random.default_rng(77 + nl)
+    eps = 1e-7
+
+    def near_real(shape, scale, diag):
+        re = rng.standard_normal(shape) * scale
+        im = rng.standard_normal(shape) * scale * eps
+        v = np.asfortranarray(re + 1j * im)
+        if diag:
+            for q in range(shape[2]):
+                v[:, :, q] += diag * np.eye(shape[0])
+        return v
+
+    with _ctx(nx=nx, nf=nf, nl=nl, nq=nq, beta=2.0, f_damping=0.5) as c:
+        lat = Lattice(nx, nf, nl)
+        gam = [near_real((c.nz, c.nz, c.nq), 300.0, 2000.0) for _ in range(4)]
+        fv = [near_real((c.nz, c.nz, c.nq), 300.0, 2000.0) for _ in range(4)]
+        chic = [np.asfortranarray(near_real((nl, nl, nf, c.nq), 3.0, 0.0)) for _ in range(2)]
+        exp = B.bse_ladder(lat, gam, fv, [B.chi_compact_to_dense(lat, x) for x in chic], 2.0, 0.5)
+        for p in range(2):
+            c.upload(CHI, p, chic[p])
+        for v, name in c.variants():
+            for r in range(4):
+                c.upload(GAMMA, r, gam[r])
+                c.upload(F, r, fv[r])
+            c.set_variant(v)
+            c.BSE(0)
+            worst_c = worst_im = 0.0
+            for r in range(4):
+                got = c.download(GAMMA, r)
+                worst_c = max(worst_c, rel_fro(got, exp[r]))
+                worst_im = max(worst_im, rel_fro(got.imag, exp[r].imag))
+            _say(f"  {name}: complex rel err {worst_c:.2e}, imaginary part alone {worst_im:.2e}")
+            assert worst_c < TOL, name
+            if not name.endswith(("_3m", "_3r")):
+                assert worst_im < TOL, name

@@ -152,3 +152,34 @@ def test_split_matches_fp64_on_batched_workload():
             _say(f"  moduli {n}: " + " ".join(f"{e:.2e}" for e in errs[n]))
     assert max(errs[12]) < SPLIT_TOL
     assert max(errs[16]) < 1e-13
+
+
+@pytest.mark.parametrize("nx,nf,nl,nq", [(8, 64, 5, 2), (4, 8, 9, 0), (16, 64, 9, 1)])
+def test_every_split_gemm_tile_and_autotune(nx, nf, nl, nq):
+    """The 64 x 64, 160 x 160 and 192 x 192 GEMM tiles (each with its own padding
+    of D) and the autotuner's pick all meet the split bar."""
+    from tups_amd.runtime import K_SPLIT_GEMM
+
+    rng = np.random.default_rng(nx + nf + nl + 5)
+    lat = Lattice(nx, nf, nl)
+    with _ctx(nx=nx, nf=nf, nl=nl, nq=nq, beta=2.0, f_damping=0.5) as c:
+        gam = random_vertices(rng, c.nz, c.nq, diag=2.0, scale=0.3)
+        fv = random_vertices(rng, c.nz, c.nq, diag=2.0, scale=0.3)
+        chic = [random_chi_compact(rng, nl, nf, c.nq, 3.0) for _ in range(2)]
+        exp = B.bse_ladder(lat, gam, fv, [B.chi_compact_to_dense(lat, x) for x in chic], 2.0, 0.5)
+        variants = c.variants(K_SPLIT_GEMM)
+        assert len(variants) == 3
+        for v, name in variants + [(None, "autotuned")]:
+            _say(f"split tile {name} nx={nx} nf={nf} nl={nl} nq={c.nq}")
+            if v is None:
+                for r in range(4):
+                    from tups_amd import F, GAMMA
+                    c.upload(GAMMA, r, gam[r])
+                    c.upload(F, r, fv[r])
+                name, ms = c.autotune(K_SPLIT_GEMM)
+                assert ms > 0
+            else:
+                c.set_variant(v, K_SPLIT_GEMM)
+            got = _run(c, gam, fv, chic, 12)
+            for r in range(4):
+                assert rel_fro(got[r], exp[r]) < SPLIT_TOL, (name, r)

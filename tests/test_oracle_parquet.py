@@ -233,3 +233,34 @@ def test_pae_nl1_gather_form():
                         exp[tb][nu1 - 1, nu2 - 1, q] += (cb[0] * x0 + cb[1] * x1) / lat.nx
     for r in range(4):
         assert rel_fro(got[r], exp[r]) < 1e-13, r
+
+
+def test_uneven_crossing_reduces_to_calculate_pae_on_uniform_boxes():
+    """The uneven-box crossing (embed in the largest box, calculate_pae, extract) is
+    calculate_pae itself when every q has the largest box."""
+    lat = Lattice(4, 6, 5)
+    rng = np.random.default_rng(5)
+    phi = [crandn(rng, (lat.nz, lat.nz, lat.nq)) for _ in range(4)]
+    lam = [crandn(rng, (lat.nf, lat.nf, lat.nw)) for _ in range(4)]
+    nf_q, nl_q = [lat.nf] * lat.nq, [lat.nl] * lat.nq
+    blocks = [[phi[r][:, :, q] for q in range(lat.nq)] for r in range(4)]
+    for r in range(4):
+        assert np.array_equal(P.uneven_embed(lat, blocks[r], nf_q, nl_q), phi[r])
+    got = P.crossing_uneven(lat, blocks, lam, nf_q, nl_q)
+    exp = P.calculate_pae(lat, phi, lam, [P.calc_phir(lat, p) for p in phi], P.calc_m_tensors(lat))
+    for r in range(4):
+        assert all(np.array_equal(got[r][q], exp[r][:, :, q]) for q in range(lat.nq))
+
+
+def test_uneven_embed_clamps_and_truncates():
+    """A q with a smaller box: frequencies outside it take the edge value (pickNu),
+    form factors beyond its Nl are zero, and extract recovers the block exactly."""
+    lat = Lattice(4, 8, 5)
+    rng = np.random.default_rng(6)
+    b = crandn(rng, (1 * 4, 1 * 4))           # Nl_q = 1, Nf_q = 4
+    full = P.uneven_embed(lat, [b], [4], [1])[:, :, 0]
+    # box-centred n = -4..3 on the largest box -> clamp to -2..1 on q's box
+    c = np.clip(np.arange(8) - 4, -2, 1) + 2
+    assert np.array_equal(full[:8, :8], b[np.ix_(c, c)])
+    assert not full[8:, :].any() and not full[:, 8:].any()
+    assert np.array_equal(P.uneven_extract(lat, full[:, :, None], [4], [1])[0], b)

@@ -27,7 +27,7 @@ ARCH = "gfx950"
 
 SOURCES = ["context.cpp", "tables.cpp", "iteration.cpp", "prms.cpp", "bse.hip", "phir.hip", "eqmatrix.hip",
            "crossing.hip", "vertex_ops.hip", "bse_split.hip", "bse_closed.hip", "chi.hip", "sde.hip", "gtau.hip",
-           "conv.hip", "sus.hip"]
+           "conv.hip", "sus.hip", "uneven.hip"]
 
 # Packed FP32 instructions are disabled for every device compile.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -288,6 +288,10 @@ struct BseVariant {
   int nl;
   BseKernel fn;
   const char* name;
+  bool four() const {  // the four-product complex form (the reference's ZGEMM arithmetic)
+    const size_t n = std::char_traits<char>::length(name);
+    return !(n > 3 && name[n - 3] == '_' && name[n - 2] == '3');
+  }
 };
 
 // tile variants per form-factor count: frequency chunk G and LDS double buffering
@@ -364,6 +368,7 @@ int autotune_bse(tups_ctx* c, int reps, int* chosen, float* best_ms) {
   int arg = -1;
   for (int v = 0; v < kNumBseVariants; ++v) {
     if (kBseVariants[v].nl != c->sz.nl) continue;
+    if (!c->three_mult && !kBseVariants[v].four()) continue;
     if (int rc = launch_variant(c, v, a)) return rc;  // warm-up
     TUPS_HIP(hipEventRecord(c->tev[0], c->stream));
     for (int i = 0; i < reps; ++i)
@@ -515,6 +520,7 @@ struct BatchVariant {
 static const BatchVariant kBatchKernels[] = {
 #define TUPS_BATCH_VARIANT(K1, K5, K9, name) \
   {bse_batch_kernel<K1, K5, K9>, 8 * batch_lds_doubles<K1, K5, K9>(), batch_occupancy<K1, K5, K9>(), name}
+    TUPS_BATCH_VARIANT(0, 0, 0, "nl1_4m+nl5_4m+nl9_4m"),
     TUPS_BATCH_VARIANT(0, 1, 0, "nl1_4m+nl5_3m+nl9_4m"),
     TUPS_BATCH_VARIANT(0, 1, 2, "nl1_4m+nl5_3m+nl9_3r"),
     TUPS_BATCH_VARIANT(2, 2, 2, "nl1_3r+nl5_3r+nl9_3r"),
@@ -541,27 +547,6 @@ const char* batch_variant_name(int v) {
 
 using namespace tups;
 
-struct tups_batch {
-  int nx = 0, nq = 0;
-  double alpha = 0, f_damping = 0;
-  std::vector<int> nf, nl;
-  std::vector<long long> voff, coff;  // offsets, with totals at [nq]
-  hipStream_t stream = nullptr;
-  double2* gamma[4] = {};
-  double2* fvert[4] = {};
-  double2* spare[4] = {};
-  double2* chi[2] = {};
-  int4* d_items = nullptr;
-  long long* d_voff = nullptr;
-  long long* d_coff = nullptr;
-  int* d_nf = nullptr;
-  int* d_nl = nullptr;
-  int* d_tickets = nullptr;
-  int nitems = 0;  // work items (tiles)
-  int len = 0;     // entries per XCD list, padding included
-  int variant = 0;
-};
-
 extern "C" int tups_batch_destroy(tups_batch* b) {
   if (!b) return TUPS_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
@@ -581,7 +566,7 @@ extern "C" int tups_batch_destroy(tups_batch* b) {
   f(b->d_nf);
   f(b->d_nl);
   f(b->d_tickets);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->stream && b->own_stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return TUPS_OK;
 }
@@ -752,7 +737,7 @@ static BatchArgs batch_args(const tups_batch* b, int dyn) {
 }
 
 // one launch of variant v into the spare buffers (no swap)
-static int batch_launch(tups_batch* b, int v) {
+int tups::batch_launch(tups_batch* b, int v) {
   const BatchVariant& k = kBatchKernels[v / kBatchOrders];
   const int order = v % kBatchOrders;
   const int flat = order == 1, dyn = order == 2;
@@ -795,6 +780,12 @@ extern "C" int tups_batch_bse(tups_batch* b, float* ms) {
   return TUPS_OK;
 }
 
+extern "C" int tups_batch_set_product_forms(tups_batch* b, int32_t allow_three) {
+  TUPS_CHECK(b, TUPS_EINVAL, "null batch");
+  b->three_mult = allow_three != 0;
+  return TUPS_OK;
+}
+
 extern "C" int tups_batch_set_variant(tups_batch* b, int32_t variant) {
   TUPS_CHECK(b, TUPS_EINVAL, "null batch");
   TUPS_CHECK(variant >= 0 && variant < batch_variant_count(), TUPS_EINVAL, "no such batch variant");
@@ -812,6 +803,7 @@ extern "C" int tups_batch_autotune(tups_batch* b, int32_t* variant, float* ms) {
   float best = 0.f;
   int arg = -1, rc = TUPS_OK;
   for (int v = 0; v < batch_variant_count() && !rc; ++v) {
+    if (!b->three_mult && v / kBatchOrders != 0) continue;  // kernel 0 is the four-product one
     if ((rc = batch_launch(b, v))) break;  // warm-up
     if (hipEventRecord(e0, b->stream) != hipSuccess) rc = TUPS_EHIP;
     for (int i = 0; i < 2 && !rc; ++i) rc = batch_launch(b, v);

@@ -1,68 +1,136 @@
-// Closed form of the BSE ladder.
+// Closed form of the BSE ladder, direct (FP64) and by FP32 inverse with FP64
+// iterative refinement.
 //
 // For fixed Gamma the damped update of BSE(ite) (SRC/parquet_BSE.f90:62-132),
 //   Phi = c_r f alpha (Gamma chi F + F chi Gamma) + (1 - f)(F - Gamma),
 // has the fixed point F = (I - kappa_r Gamma chi)^-1 Gamma, kappa_r = 2 c_r alpha
 // (push-through identity: Gamma chi F = F chi Gamma there), where Phi = F - Gamma
-// for every damping f.  This mode solves (I - kappa_r Gamma chi) X = Gamma per
-// (q, r) and returns Phi = X - Gamma: the converged ladder in one call.  The
+// for every damping f.  Both modes solve A X = Gamma, A = I - kappa_r Gamma chi,
+// per (q, r) and return Phi = X - Gamma: the converged ladder in one call.  The
 // reference has no such solve; it iterates the update.
 //
-// Method: blocked Gauss-Jordan elimination with partial pivoting on the
-// augmented matrix [I - kappa Gamma chi | Gamma] (D x 2D, complex128), one
-// workgroup per matrix.  Per panel of NB columns:
+// Direct mode (TUPS_BSE_CLOSED_LU): blocked Gauss-Jordan elimination with
+// partial pivoting on the augmented matrix [A | Gamma] (D x 2D, complex128),
+// one workgroup per matrix.  Per panel of NB columns:
 //   1. LU of the panel rows k0..D-1 with row pivoting (rows swapped over all
 //      columns from k0 on, so the right-hand side follows the equations);
 //   2. Y = L11^-1 Z on the panel rows for every trailing column Z;
-//   3. rows below the panel: Z -= L21 Y                    (FP64 MFMA);
+//   3. rows below the panel: Z -= L21 Y                    (MFMA);
 //   4. X = U11^-1 Y on the panel rows;
-//   5. rows above the panel: Z -= P_above X                (FP64 MFMA);
+//   5. rows above the panel: Z -= P_above X                (MFMA);
 // after the last panel the right-hand block holds the solution.  Work per
-// (q, r): 12 D^3 real flops (Gauss-Jordan on D right-hand sides).
+// (q, r): 12 D^3 real flops.
+//
+// Refinement mode (TUPS_BSE_CLOSED_IR): the same elimination in FP32 on
+// [fp32(A) | I] gives the inverse of the FP32-rounded A (its LU with partial
+// pivoting applied to the identity, FP32 MFMA updates); then
+//   X  = fp64(A32^-1 fp32(Gamma))                         (FP32 MFMA GEMM)
+//   repeat: R = Gamma - A X = Gamma - X + kappa Gamma (chi X)   (FP64 MFMA GEMM)
+//           stop when max over (q, r) of |R|_F / |Gamma|_F <= tol
+//           X += fp64(A32^-1 fp32(R))                     (FP32 MFMA GEMM)
+// FP64 is used only for A's assembly, the residual and the update of X.  Each
+// step shrinks the error by about cond(A) 2^-24, so two or three steps reach
+// 1e-12.  Work per (q, r): 12 D^3 FP32 (inverse) + 8 D^3 FP32 per product with
+// the inverse + 8 D^3 FP64 per residual.
 #include "internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <type_traits>
 
 namespace tups {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int CL_NB = 16;  // panel width
 constexpr int CL_T = 64;   // update tile (rows and columns)
+
+__device__ __forceinline__ double2 mkc(double x, double y) { return make_double2(x, y); }
+__device__ __forceinline__ float2 mkc(float x, float y) { return make_float2(x, y); }
+
+// Precision of an elimination or product: complex type, MFMA accumulator and
+// the 16 x 16 x 4 MFMA output layout (element e of lane l sits at column l % 16
+// and the row given by orow(l / 16, e)).
+template <class R>
+struct Prec;
+template <>
+struct Prec<double> {
+  using C = double2;
+  using V = d4;
+  __device__ static V fma_add(double a, double b, V acc) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+  __device__ static V fma_sub(double a, double b, V acc) {  // A negated by the MFMA modifier
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 1);
+  }
+  __device__ static int orow(int lk, int e) { return lk + 4 * e; }
+};
+template <>
+struct Prec<float> {
+  using C = float2;
+  using V = f4;
+  __device__ static V fma_add(float a, float b, V acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  }
+  __device__ static V fma_sub(float a, float b, V acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(-a, b, acc, 0, 0, 0);
+  }
+  __device__ static int orow(int lk, int e) { return 4 * lk + e; }
+};
+
+template <class C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  return mkc(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class C>
+__device__ __forceinline__ C csub(C a, C b) {
+  return mkc(a.x - b.x, a.y - b.y);
+}
+template <class C>
+__device__ __forceinline__ C cinv(C a) {
+  // 1 / a with the larger component scaled out (Smith's method)
+  if (fabs(a.x) >= fabs(a.y)) {
+    const auto r = a.y / a.x, d = a.x + a.y * r;
+    return mkc(1 / d, -r / d);
+  }
+  const auto r = a.x / a.y, d = a.y + a.x * r;
+  return mkc(r / d, -1 / d);
+}
 
 struct ClosedArgs {
   const double2* gam[4];
   double2* out[4];
   const double2* chi[2];
-  double2* W;   // [mat][2D columns][D rows], column-major
+  void* W;      // [mat][2D columns][D rows], column-major: double2 (direct) or float2 (inverse)
   int* info;    // first failing pivot (1-based column) over all matrices, 0 if none
   int D, nf, nl, q0, nqb;
   double kappa[4];
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cinv(double2 a) {
-  // 1 / a with the larger component scaled out (Smith's method)
-  if (fabs(a.x) >= fabs(a.y)) {
-    const double r = a.y / a.x, d = a.x + a.y * r;
-    return make_double2(1.0 / d, -r / d);
-  }
-  const double r = a.x / a.y, d = a.y + a.x * r;
-  return make_double2(r / d, -1.0 / d);
-}
-
 __global__ void closed_clear(int* info) { *info = 0; }
 
-// [I - kappa Gamma chi | Gamma]; column j of the left block is (l2, nu) = (j / Nf, j % Nf),
-// (Gamma chi)(:, (l2, nu)) = sum_l1 Gamma(:, (l1, nu)) chi_nu(l1, l2).
-// grid: (2D, 4 * nqb); one thread per row.
+// (Gamma chi)(row, (l2, nu)) = sum_l1 Gamma(row, (l1, nu)) chi_nu(l1, l2)
+__device__ __forceinline__ double2 gamma_chi(const double2* G, const double2* cb, int D, int nf, int nl, int row,
+                                             int l2, int nu) {
+  double sr = 0.0, si = 0.0;
+  for (int l1 = 0; l1 < nl; ++l1) {
+    const double2 g = G[(size_t)(l1 * nf + nu) * D + row];
+    const double2 c = cb[l1 + nl * l2];
+    sr += g.x * c.x - g.y * c.y;
+    si += g.x * c.y + g.y * c.x;
+  }
+  return make_double2(sr, si);
+}
+
+// Direct mode: [I - kappa Gamma chi | Gamma]; column j of the left block is
+// (l2, nu) = (j / Nf, j % Nf).  grid: (2D, 4 * nqb); one thread per row.
 __global__ __launch_bounds__(256) void closed_build(ClosedArgs a) {
   const int j = blockIdx.x, mat = blockIdx.y;
   const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
   const int D = a.D, nf = a.nf, nl = a.nl;
   const double2* G = a.gam[r] + (size_t)D * D * q;
-  double2* Wc = a.W + (size_t)mat * 2 * D * D + (size_t)j * D;
+  double2* Wc = static_cast<double2*>(a.W) + (size_t)mat * 2 * D * D + (size_t)j * D;
   if (j >= D) {
     for (int row = threadIdx.x; row < D; row += 256) Wc[row] = G[(size_t)(j - D) * D + row];
     return;
@@ -71,50 +139,68 @@ __global__ __launch_bounds__(256) void closed_build(ClosedArgs a) {
   const double2* cb = a.chi[r >> 1] + (size_t)nf * nl * nl * q + (size_t)nu * nl * nl;
   const double kap = a.kappa[r];
   for (int row = threadIdx.x; row < D; row += 256) {
-    double sr = 0.0, si = 0.0;
-    for (int l1 = 0; l1 < nl; ++l1) {
-      const double2 g = G[(size_t)(l1 * nf + nu) * D + row];
-      const double2 c = cb[l1 + nl * l2];
-      sr += g.x * c.x - g.y * c.y;
-      si += g.x * c.y + g.y * c.x;
-    }
-    Wc[row] = make_double2((row == j ? 1.0 : 0.0) - kap * sr, -kap * si);
+    const double2 s = gamma_chi(G, cb, D, nf, nl, row, l2, nu);
+    Wc[row] = make_double2((row == j ? 1.0 : 0.0) - kap * s.x, -kap * s.y);
+  }
+}
+
+// Refinement mode: [fp32(I - kappa Gamma chi) | I], A formed in FP64 and rounded.
+__global__ __launch_bounds__(256) void ir_build32(ClosedArgs a) {
+  const int j = blockIdx.x, mat = blockIdx.y;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  const int D = a.D, nf = a.nf, nl = a.nl;
+  const double2* G = a.gam[r] + (size_t)D * D * q;
+  float2* Wc = static_cast<float2*>(a.W) + (size_t)mat * 2 * D * D + (size_t)j * D;
+  if (j >= D) {
+    for (int row = threadIdx.x; row < D; row += 256) Wc[row] = make_float2(row == j - D ? 1.f : 0.f, 0.f);
+    return;
+  }
+  const int l2 = j / nf, nu = j - l2 * nf;
+  const double2* cb = a.chi[r >> 1] + (size_t)nf * nl * nl * q + (size_t)nu * nl * nl;
+  const double kap = a.kappa[r];
+  for (int row = threadIdx.x; row < D; row += 256) {
+    const double2 s = gamma_chi(G, cb, D, nf, nl, row, l2, nu);
+    Wc[row] = make_float2((float)((row == j ? 1.0 : 0.0) - kap * s.x), (float)(-kap * s.y));
   }
 }
 
 // Z(r, c) -= sum_t W(r, k0 + t) W(k0 + t, c) for r in [r0, r1), c in [c0, c1),
-// t < nb, by 64 x 64 tiles; four waves of 2 x 2 blocks of v_mfma_f64_16x16x4f64.
-// A fragments: lane l holds A[l % 16][l / 16]; B: B[l / 16][l % 16]; output
-// element e of lane l: row l / 16 + 4 e, column l % 16.
-__device__ void closed_update(double2* W, int D, int r0, int r1, int c0, int c1, int k0, int nb,
-                              double2 (*As)[CL_T + 1], double2 (*Bs)[CL_T + 1]) {
+// t < nb, by 64 x 64 tiles; four waves of 2 x 2 blocks of 16 x 16 x 4 MFMAs.
+// A fragments: lane l holds A[l % 16][l / 16]; B: B[l / 16][l % 16].
+template <class R>
+__device__ void closed_update(typename Prec<R>::C* W, int D, int r0, int r1, int c0, int c1, int k0, int nb,
+                              typename Prec<R>::C (*As)[CL_T + 1], typename Prec<R>::C (*Bs)[CL_T + 1]) {
+  using P = Prec<R>;
+  using C = typename P::C;
+  using V = typename P::V;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
+  const C zero = mkc((R)0, (R)0);
   for (int tr = r0; tr < r1; tr += CL_T)
     for (int tc = c0; tc < c1; tc += CL_T) {
       for (int u = tid; u < CL_NB * CL_T; u += 256) {
         const int row = u & (CL_T - 1), ka = u >> 6;   // A: rows contiguous
-        As[ka][row] = (tr + row < r1 && ka < nb) ? W[(size_t)(k0 + ka) * D + tr + row] : make_double2(0.0, 0.0);
+        As[ka][row] = (tr + row < r1 && ka < nb) ? W[(size_t)(k0 + ka) * D + tr + row] : zero;
         const int kb = u % CL_NB, col = u / CL_NB;     // B: k contiguous in a column
-        Bs[kb][col] = (tc + col < c1 && kb < nb) ? W[(size_t)(tc + col) * D + k0 + kb] : make_double2(0.0, 0.0);
+        Bs[kb][col] = (tc + col < c1 && kb < nb) ? W[(size_t)(tc + col) * D + k0 + kb] : zero;
       }
       __syncthreads();
-      d4 ar[2][2], ai[2][2];
+      V ar[2][2], ai[2][2];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            const int row = tr + wr + i * 16 + lk + 4 * e, col = tc + wc + j * 16 + li;
-            const double2 z = (row < r1 && col < c1) ? W[(size_t)col * D + row] : make_double2(0.0, 0.0);
+            const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
+            const C z = (row < r1 && col < c1) ? W[(size_t)col * D + row] : zero;
             ar[i][j][e] = z.x;
             ai[i][j][e] = z.y;
           }
 #pragma unroll
       for (int kk = 0; kk < CL_NB; kk += 4) {
-        double2 fa[2], fb[2];
+        C fa[2], fb[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) fa[i] = As[kk + lk][wr + i * 16 + li];
 #pragma unroll
@@ -123,11 +209,11 @@ __device__ void closed_update(double2* W, int D, int r0, int r1, int c0, int c1,
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
-            // Z - A B: re -= ar br - ai bi, im -= ar bi + ai br (blgp 1 negates A)
-            ar[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].x, fb[j].x, ar[i][j], 0, 0, 1);
-            ar[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].y, fb[j].y, ar[i][j], 0, 0, 0);
-            ai[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].x, fb[j].y, ai[i][j], 0, 0, 1);
-            ai[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].y, fb[j].x, ai[i][j], 0, 0, 1);
+            // Z - A B: re -= ar br - ai bi, im -= ar bi + ai br
+            ar[i][j] = P::fma_sub(fa[i].x, fb[j].x, ar[i][j]);
+            ar[i][j] = P::fma_add(fa[i].y, fb[j].y, ar[i][j]);
+            ai[i][j] = P::fma_sub(fa[i].x, fb[j].y, ai[i][j]);
+            ai[i][j] = P::fma_sub(fa[i].y, fb[j].x, ai[i][j]);
           }
       }
 #pragma unroll
@@ -136,8 +222,8 @@ __device__ void closed_update(double2* W, int D, int r0, int r1, int c0, int c1,
         for (int j = 0; j < 2; ++j)
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            const int row = tr + wr + i * 16 + lk + 4 * e, col = tc + wc + j * 16 + li;
-            if (row < r1 && col < c1) W[(size_t)col * D + row] = make_double2(ar[i][j][e], ai[i][j][e]);
+            const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
+            if (row < r1 && col < c1) W[(size_t)col * D + row] = mkc(ar[i][j][e], ai[i][j][e]);
           }
       __syncthreads();
     }
@@ -145,50 +231,57 @@ __device__ void closed_update(double2* W, int D, int r0, int r1, int c0, int c1,
 
 // Z(k0 + i, c) <- sum_t M[i][t] Z(k0 + t, c) for i, t < nb and every column c in
 // [c0, c1): one thread per column, M (NB x NB, identity-padded) in LDS.
-__device__ void panel_rows_apply(double2* W, int D, int k0, int nb, int c0, int c1,
-                                 const double2 (*M)[CL_NB]) {
+template <class R>
+__device__ void panel_rows_apply(typename Prec<R>::C* W, int D, int k0, int nb, int c0, int c1,
+                                 const typename Prec<R>::C (*M)[CL_NB]) {
+  using C = typename Prec<R>::C;
   for (int col = c0 + (int)threadIdx.x; col < c1; col += 256) {
-    double2* zc = W + (size_t)col * D + k0;
-    double2 z[CL_NB];
+    C* zc = W + (size_t)col * D + k0;
+    C z[CL_NB];
 #pragma unroll
-    for (int i = 0; i < CL_NB; ++i) z[i] = i < nb ? zc[i] : make_double2(0.0, 0.0);
+    for (int i = 0; i < CL_NB; ++i) z[i] = i < nb ? zc[i] : mkc((R)0, (R)0);
 #pragma unroll 1
     for (int i = 0; i < CL_NB; ++i) {
-      double yr = 0.0, yi = 0.0;
+      R yr = 0, yi = 0;
 #pragma unroll
       for (int t = 0; t < CL_NB; ++t) {
-        const double2 m = M[i][t];
+        const C m = M[i][t];
         yr = fma(m.x, z[t].x, fma(-m.y, z[t].y, yr));
         yi = fma(m.x, z[t].y, fma(m.y, z[t].x, yi));
       }
-      if (i < nb) zc[i] = make_double2(yr, yi);
+      if (i < nb) zc[i] = mkc(yr, yi);
     }
   }
 }
 
-// one workgroup per (q, r) matrix; grid: 4 * nqb
+// One workgroup per (q, r) matrix; grid: 4 * nqb.  PHI: the direct mode's
+// final Phi = X - Gamma from the right-hand block (else the right-hand block,
+// the inverse, stays in W).
+template <class R, bool PHI>
 __global__ __launch_bounds__(256, 2) void closed_solve(ClosedArgs a) {
-  __shared__ double2 As[CL_NB][CL_T + 1];
-  __shared__ double2 Bs[CL_NB][CL_T + 1];
-  __shared__ double2 Ls[CL_NB][CL_NB];
-  __shared__ double2 Li[CL_NB][CL_NB];
-  __shared__ double2 Ui[CL_NB][CL_NB];
-  __shared__ double red_v[256];
+  using C = typename Prec<R>::C;
+  __shared__ C As[CL_NB][CL_T + 1];
+  __shared__ C Bs[CL_NB][CL_T + 1];
+  __shared__ C Ls[CL_NB][CL_NB];
+  __shared__ C Li[CL_NB][CL_NB];
+  __shared__ C Ui[CL_NB][CL_NB];
+  __shared__ R red_v[256];
   __shared__ int red_i[256];
   const int mat = blockIdx.x;
   const int D = a.D, D2 = 2 * D;
-  double2* W = a.W + (size_t)mat * 2 * D * D;
+  C* W = static_cast<C*>(a.W) + (size_t)mat * 2 * D * D;
   const int tid = threadIdx.x;
+  const C one = mkc((R)1, (R)0), zero = mkc((R)0, (R)0);
   int info = 0;
   for (int k0 = 0; k0 < D; k0 += CL_NB) {
     const int nb = min(CL_NB, D - k0);
     // 1. panel LU, rows k0..D-1, columns k0..k0+nb-1
     for (int c = k0; c < k0 + nb; ++c) {
-      double best = -1.0;
+      R best = -1;
       int bi = c;
       for (int row = c + tid; row < D; row += 256) {
-        const double2 v = W[(size_t)c * D + row];
-        const double m = fabs(v.x) + fabs(v.y);
+        const C v = W[(size_t)c * D + row];
+        const R m = fabs(v.x) + fabs(v.y);
         if (m > best) {
           best = m;
           bi = row;
@@ -199,7 +292,7 @@ __global__ __launch_bounds__(256, 2) void closed_solve(ClosedArgs a) {
       __syncthreads();
       for (int s = 128; s > 0; s >>= 1) {
         if (tid < s) {
-          const double o = red_v[tid + s];
+          const R o = red_v[tid + s];
           const int oi = red_i[tid + s];
           if (o > red_v[tid] || (o == red_v[tid] && oi < red_i[tid])) {
             red_v[tid] = o;
@@ -209,23 +302,23 @@ __global__ __launch_bounds__(256, 2) void closed_solve(ClosedArgs a) {
         __syncthreads();
       }
       const int p = red_i[0];
-      const double pmax = red_v[0];
+      const R pmax = red_v[0];
       __syncthreads();
-      if (!(pmax > 0.0)) {  // zero or NaN pivot column: the system is singular
+      if (!(pmax > 0)) {  // zero or NaN pivot column: the system is singular
         if (info == 0) info = c + 1;
         continue;
       }
       if (p != c) {
         for (int col = k0 + tid; col < D2; col += 256) {
-          const double2 x = W[(size_t)col * D + c];
+          const C x = W[(size_t)col * D + c];
           W[(size_t)col * D + c] = W[(size_t)col * D + p];
           W[(size_t)col * D + p] = x;
         }
       }
       __syncthreads();
-      const double2 inv = cinv(W[(size_t)c * D + c]);
+      const C inv = cinv(W[(size_t)c * D + c]);
       for (int row = c + 1 + tid; row < D; row += 256) {
-        const double2 l = cmul(W[(size_t)c * D + row], inv);
+        const C l = cmul(W[(size_t)c * D + row], inv);
         W[(size_t)c * D + row] = l;
         for (int col = c + 1; col < k0 + nb; ++col)
           W[(size_t)col * D + row] = csub(W[(size_t)col * D + row], cmul(l, W[(size_t)col * D + c]));
@@ -234,7 +327,7 @@ __global__ __launch_bounds__(256, 2) void closed_solve(ClosedArgs a) {
     }
     for (int u = tid; u < CL_NB * CL_NB; u += 256) {
       const int i = u % CL_NB, j = u / CL_NB;
-      Ls[i][j] = (i < nb && j < nb) ? W[(size_t)(k0 + j) * D + k0 + i] : make_double2(0.0, 0.0);
+      Ls[i][j] = (i < nb && j < nb) ? W[(size_t)(k0 + j) * D + k0 + i] : zero;
     }
     __syncthreads();
     // inverses of the unit lower L11 and of the upper U11, padded to NB x NB with
@@ -242,13 +335,13 @@ __global__ __launch_bounds__(256, 2) void closed_solve(ClosedArgs a) {
     if (tid < CL_NB) {
       const int t = tid;
       for (int i = 0; i < CL_NB; ++i) {
-        double2 x = make_double2(i == t ? 1.0 : 0.0, 0.0);
+        C x = i == t ? one : zero;
         if (i < nb && t < nb)
           for (int s2 = t; s2 < i; ++s2) x = csub(x, cmul(Ls[i][s2], Li[s2][t]));
         Li[i][t] = x;
       }
       for (int i = CL_NB - 1; i >= 0; --i) {
-        double2 x = make_double2(i == t ? 1.0 : 0.0, 0.0);
+        C x = i == t ? one : zero;
         if (i < nb && t < nb) {
           for (int s2 = i + 1; s2 <= t; ++s2) x = csub(x, cmul(Ls[i][s2], Ui[s2][t]));
           x = cmul(x, cinv(Ls[i][i]));
@@ -259,31 +352,215 @@ __global__ __launch_bounds__(256, 2) void closed_solve(ClosedArgs a) {
     __syncthreads();
     const int j0 = k0 + nb;
     // 2. Y = L11^-1 Z on the panel rows
-    panel_rows_apply(W, D, k0, nb, j0, D2, Li);
+    panel_rows_apply<R>(W, D, k0, nb, j0, D2, Li);
     __syncthreads();
     // 3. rows below the panel
-    closed_update(W, D, k0 + nb, D, j0, D2, k0, nb, As, Bs);
+    closed_update<R>(W, D, k0 + nb, D, j0, D2, k0, nb, As, Bs);
     // 4. X = U11^-1 Y on the panel rows
-    panel_rows_apply(W, D, k0, nb, j0, D2, Ui);
+    panel_rows_apply<R>(W, D, k0, nb, j0, D2, Ui);
     __syncthreads();
     // 5. rows above the panel
-    closed_update(W, D, 0, k0, j0, D2, k0, nb, As, Bs);
+    closed_update<R>(W, D, 0, k0, j0, D2, k0, nb, As, Bs);
   }
   if (info && tid == 0) atomicCAS(a.info, 0, info);
-  // Phi = X - Gamma
-  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
-  const double2* G = a.gam[r] + (size_t)D * D * q;
-  double2* O = a.out[r] + (size_t)D * D * q;
-  for (size_t u = tid; u < (size_t)D * D; u += 256) O[u] = csub(W[(size_t)D * D + u], G[u]);
+  if constexpr (PHI) {
+    // Phi = X - Gamma
+    const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+    const double2* G = a.gam[r] + (size_t)D * D * q;
+    double2* O = a.out[r] + (size_t)D * D * q;
+    for (size_t u = tid; u < (size_t)D * D; u += 256) O[u] = csub(W[(size_t)D * D + u], G[u]);
+  }
 }
 
-int launch_bse_closed(tups_ctx* c) {
+// ---- refinement products ------------------------------------------------------------
+
+struct IrArgs {
+  const double2* gam[4];
+  double2* x[4];             // X_r, the solution, in the output arrays
+  const double2* chi[2];
+  const float2* inv;         // [mat][2D][D]: the inverse is columns D..2D-1
+  const double2* b;          // [mat][D][D] FP64 right-hand side of an apply (Gamma when null)
+  double2* rres;             // [mat][D][D] residual R
+  double2* y;                // [mat][D][D] chi X
+  double* part;              // [mat][tiles][2]: sum |R|^2, sum |Gamma|^2 per tile
+  double* res;               // [mat] |R|_F / |Gamma|_F
+  unsigned long long* rmax;  // bits of the largest res (non-negative doubles order as integers)
+  int D, nf, nl, q0, nqb, tiles_n;
+  double kappa[4];
+};
+
+__global__ void ir_clear(unsigned long long* rmax) { *rmax = 0ull; }
+
+// Y = chi X: Y((l1, nu), c) = sum_l2 chi_nu(l1, l2) X((l2, nu), c).  grid (D, 4 nqb)
+__global__ __launch_bounds__(256) void ir_chi(IrArgs a) {
+  const int col = blockIdx.x, mat = blockIdx.y;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  const int D = a.D, nf = a.nf, nl = a.nl;
+  const double2* X = a.x[r] + (size_t)D * D * q + (size_t)col * D;
+  double2* Y = a.y + (size_t)mat * D * D + (size_t)col * D;
+  const double2* cq = a.chi[r >> 1] + (size_t)nf * nl * nl * q;
+  for (int row = threadIdx.x; row < D; row += 256) {
+    const int l1 = row / nf, nu = row - l1 * nf;
+    const double2* cb = cq + (size_t)nu * nl * nl;
+    double sr = 0.0, si = 0.0;
+    for (int l2 = 0; l2 < nl; ++l2) {
+      const double2 c = cb[l1 + nl * l2], x = X[l2 * nf + nu];
+      sr += c.x * x.x - c.y * x.y;
+      si += c.x * x.y + c.y * x.x;
+    }
+    Y[row] = make_double2(sr, si);
+  }
+}
+
+// 64 x 64 tile of a batched complex product per workgroup (four waves of 2 x 2
+// blocks of 16 x 16 x 4 MFMAs), K in chunks of 16 through LDS.
+//   MODE 0: X  = fp64(A32^-1 fp32(B))    MODE 1: X += fp64(A32^-1 fp32(B))
+//   MODE 2: R  = Gamma - X + kappa Gamma Y (FP64), with per-tile |R|^2, |Gamma|^2
+// grid: tiles_n^2 * 4 nqb
+template <int MODE>
+__global__ __launch_bounds__(256) void ir_gemm(IrArgs a) {
+  using R = typename std::conditional<MODE == 2, double, float>::type;
+  using P = Prec<R>;
+  using C = typename P::C;
+  using V = typename P::V;
+  __shared__ C As[CL_NB][CL_T + 1];
+  __shared__ C Bs[CL_NB][CL_T + 1];
+  __shared__ double red[2][256];
+  const int tiles = a.tiles_n * a.tiles_n;
+  const int mat = blockIdx.x / tiles, t = blockIdx.x - mat * tiles;
+  const int tr = (t / a.tiles_n) * CL_T, tc = (t % a.tiles_n) * CL_T;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  const int D = a.D;
+  const size_t DD = (size_t)D * D;
+  const double2* G = a.gam[r] + DD * q;
+  double2* X = a.x[r] + DD * q;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const C zero = mkc((R)0, (R)0);
+  V ar[2][2], ai[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ar[i][j][e] = ai[i][j][e] = 0;
+  const double2* Bsrc = MODE == 2 ? a.y + DD * mat : (a.b ? a.b + DD * mat : G);
+  for (int k0 = 0; k0 < D; k0 += CL_NB) {
+    for (int u = tid; u < CL_NB * CL_T; u += 256) {
+      const int row = u & (CL_T - 1), ka = u >> 6;
+      const int kb = u % CL_NB, col = u / CL_NB;
+      const bool aok = tr + row < D && k0 + ka < D, bok = tc + col < D && k0 + kb < D;
+      if (MODE == 2) {
+        As[ka][row] = aok ? reinterpret_cast<const C*>(G)[(size_t)(k0 + ka) * D + tr + row] : zero;
+        Bs[kb][col] = bok ? reinterpret_cast<const C*>(Bsrc)[(size_t)(tc + col) * D + k0 + kb] : zero;
+      } else {
+        As[ka][row] = aok ? reinterpret_cast<const C*>(a.inv + 2 * DD * mat + DD)[(size_t)(k0 + ka) * D + tr + row]
+                          : zero;
+        const double2 v = bok ? Bsrc[(size_t)(tc + col) * D + k0 + kb] : make_double2(0.0, 0.0);
+        Bs[kb][col] = mkc((R)v.x, (R)v.y);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < CL_NB; kk += 4) {
+      C fa[2], fb[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) fa[i] = As[kk + lk][wr + i * 16 + li];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) fb[j] = Bs[kk + lk][wc + j * 16 + li];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          ar[i][j] = P::fma_add(fa[i].x, fb[j].x, ar[i][j]);
+          ar[i][j] = P::fma_sub(fa[i].y, fb[j].y, ar[i][j]);
+          ai[i][j] = P::fma_add(fa[i].x, fb[j].y, ai[i][j]);
+          ai[i][j] = P::fma_add(fa[i].y, fb[j].x, ai[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+  double rr = 0.0, gg = 0.0;
+  const double kap = a.kappa[r];
+  double2* Rm = a.rres + DD * mat;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
+        if (row >= D || col >= D) continue;
+        const size_t idx = (size_t)col * D + row;
+        if (MODE == 0) {
+          X[idx] = make_double2((double)ar[i][j][e], (double)ai[i][j][e]);
+        } else if (MODE == 1) {
+          const double2 x = X[idx];
+          X[idx] = make_double2(x.x + (double)ar[i][j][e], x.y + (double)ai[i][j][e]);
+        } else {
+          const double2 g = G[idx], x = X[idx];
+          const double2 v = make_double2(g.x - x.x + kap * ar[i][j][e], g.y - x.y + kap * ai[i][j][e]);
+          Rm[idx] = v;
+          rr += v.x * v.x + v.y * v.y;
+          gg += g.x * g.x + g.y * g.y;
+        }
+      }
+  if (MODE == 2) {
+    red[0][tid] = rr;
+    red[1][tid] = gg;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) {
+        red[0][tid] += red[0][tid + s];
+        red[1][tid] += red[1][tid + s];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      a.part[2 * ((size_t)mat * tiles + t)] = red[0][0];
+      a.part[2 * ((size_t)mat * tiles + t) + 1] = red[1][0];
+    }
+  }
+}
+
+// per matrix |R|_F / |Gamma|_F from the tile partials (fixed order), and the
+// largest over the pass; grid: ceil(4 nqb / 256)
+__global__ __launch_bounds__(256) void ir_check(IrArgs a, int mats) {
+  const int mat = blockIdx.x * 256 + threadIdx.x;
+  if (mat >= mats) return;
+  const int tiles = a.tiles_n * a.tiles_n;
+  double rr = 0.0, gg = 0.0;
+  for (int t = 0; t < tiles; ++t) {
+    rr += a.part[2 * ((size_t)mat * tiles + t)];
+    gg += a.part[2 * ((size_t)mat * tiles + t) + 1];
+  }
+  // a NaN residual reports as infinity, so the pass never counts as converged
+  const double v = gg > 0.0 ? sqrt(rr / gg) : (rr > 0.0 ? INFINITY : 0.0);
+  const double s = v == v ? v : INFINITY;
+  a.res[mat] = s;
+  atomicMax(a.rmax, (unsigned long long)__double_as_longlong(s));
+}
+
+// Phi = X - Gamma in place (X sits in the output array); grid (ceil(D^2 / 256), 4 nqb)
+__global__ __launch_bounds__(256) void ir_final(IrArgs a) {
+  const int mat = blockIdx.y;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  const size_t DD = (size_t)a.D * a.D;
+  const size_t u = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (u >= DD) return;
+  double2* X = a.x[r] + DD * q;
+  const double2 g = a.gam[r][DD * q + u], x = X[u];
+  X[u] = make_double2(x.x - g.x, x.y - g.y);
+}
+
+// ---- host -------------------------------------------------------------------------
+
+static int closed_reserve(tups_ctx* c, size_t per_q, size_t extra, int* qbatch) {
   const tups_sizes& s = c->sz;
-  const size_t D = s.nz;
-  const size_t per_q = 4 * 2 * D * D * sizeof(double2);
-  const size_t want = ((size_t)4 << 30) / per_q;  // about 4 GiB of augmented matrices per pass
-  const int qbatch = (int)std::max<size_t>(1, std::min<size_t>(s.nq, want));
-  const size_t need = per_q * qbatch + 256;
+  const size_t want = ((size_t)4 << 30) / per_q;  // about 4 GiB of workspace per pass
+  *qbatch = (int)std::max<size_t>(1, std::min<size_t>(s.nq, want));
+  const size_t need = per_q * *qbatch + extra + 256;
   if (c->closed_ws_bytes < need) {
     if (c->closed_ws) TUPS_HIP(hipFree(c->closed_ws));
     c->closed_ws = nullptr;
@@ -291,7 +568,11 @@ int launch_bse_closed(tups_ctx* c) {
     TUPS_HIP(hipMalloc(&c->closed_ws, need));
     c->closed_ws_bytes = need;
   }
-  ClosedArgs a{};
+  return TUPS_OK;
+}
+
+static void closed_common(tups_ctx* c, ClosedArgs& a) {
+  const tups_sizes& s = c->sz;
   for (int r = 0; r < 4; ++r) {
     a.gam[r] = c->gamma[r];
     a.out[r] = c->spare[r];
@@ -299,12 +580,21 @@ int launch_bse_closed(tups_ctx* c) {
   a.chi[0] = c->chi[0];
   a.chi[1] = c->chi[1];
   a.info = static_cast<int*>(c->closed_ws);
-  a.W = reinterpret_cast<double2*>(static_cast<char*>(c->closed_ws) + 256);
+  a.W = static_cast<char*>(c->closed_ws) + 256;
   a.D = s.nz;
   a.nf = s.nf;
   a.nl = s.nl;
   const double cr[4] = {0.5, 0.5, -0.25, 0.25};
   for (int r = 0; r < 4; ++r) a.kappa[r] = 2.0 * cr[r] * s.alpha;
+}
+
+int launch_bse_closed(tups_ctx* c) {
+  const tups_sizes& s = c->sz;
+  const size_t D = s.nz;
+  int qbatch = 0;
+  if (int rc = closed_reserve(c, 4 * 2 * D * D * sizeof(double2), 0, &qbatch)) return rc;
+  ClosedArgs a{};
+  closed_common(c, a);
   hipLaunchKernelGGL(closed_clear, dim3(1), dim3(1), 0, c->stream, a.info);
   TUPS_HIP(hipGetLastError());
   for (int q0 = 0; q0 < s.nq; q0 += qbatch) {
@@ -314,7 +604,7 @@ int launch_bse_closed(tups_ctx* c) {
     if (int rc = timer_begin(c)) return rc;
     hipLaunchKernelGGL(closed_build, dim3((unsigned)(2 * D), mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(closed_solve, dim3(mats), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((closed_solve<double, true>), dim3(mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_BSE_CLOSED)) return rc;
   }
@@ -322,6 +612,100 @@ int launch_bse_closed(tups_ctx* c) {
   TUPS_HIP(hipMemcpyAsync(&info, a.info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   TUPS_HIP(hipStreamSynchronize(c->stream));
   TUPS_CHECK(info == 0, TUPS_ENONFINITE, "closed-form BSE: singular I - kappa Gamma chi (zero pivot column)");
+  c->ir_steps = 0;
+  c->ir_residual = NAN;
+  return TUPS_OK;
+}
+
+int launch_bse_closed_ir(tups_ctx* c) {
+  const tups_sizes& s = c->sz;
+  const size_t D = s.nz, DD = D * D;
+  const int tiles_n = (int)((D + CL_T - 1) / CL_T);
+  // per matrix: [A32 | I] (2 D^2 float2), R and chi X (D^2 double2 each), tile partials, res
+  const size_t per_mat = 2 * DD * sizeof(float2) + 2 * DD * sizeof(double2) +
+                         (size_t)tiles_n * tiles_n * 2 * sizeof(double) + sizeof(double) + 64;
+  int qbatch = 0;
+  if (int rc = closed_reserve(c, 4 * per_mat, 256, &qbatch)) return rc;
+  ClosedArgs a{};
+  closed_common(c, a);
+  IrArgs b{};
+  for (int r = 0; r < 4; ++r) {
+    b.gam[r] = c->gamma[r];
+    b.x[r] = c->spare[r];
+    b.kappa[r] = a.kappa[r];
+  }
+  b.chi[0] = c->chi[0];
+  b.chi[1] = c->chi[1];
+  b.D = a.D;
+  b.nf = a.nf;
+  b.nl = a.nl;
+  b.tiles_n = tiles_n;
+  const size_t mats_max = 4 * (size_t)qbatch;
+  char* base = static_cast<char*>(a.W);
+  b.inv = reinterpret_cast<const float2*>(base);
+  base += mats_max * 2 * DD * sizeof(float2);
+  b.rres = reinterpret_cast<double2*>(base);
+  base += mats_max * DD * sizeof(double2);
+  b.y = reinterpret_cast<double2*>(base);
+  base += mats_max * DD * sizeof(double2);
+  b.part = reinterpret_cast<double*>(base);
+  base += mats_max * tiles_n * tiles_n * 2 * sizeof(double);
+  b.res = reinterpret_cast<double*>(base);
+  base += mats_max * sizeof(double);
+  b.rmax = reinterpret_cast<unsigned long long*>((reinterpret_cast<uintptr_t>(base) + 63) / 64 * 64);
+  hipLaunchKernelGGL(closed_clear, dim3(1), dim3(1), 0, c->stream, a.info);
+  TUPS_HIP(hipGetLastError());
+  int steps_max = 0;
+  double res_max = 0.0;
+  for (int q0 = 0; q0 < s.nq; q0 += qbatch) {
+    a.q0 = b.q0 = q0;
+    a.nqb = b.nqb = std::min(qbatch, s.nq - q0);
+    const unsigned mats = 4u * (unsigned)a.nqb;
+    const unsigned gblk = (unsigned)(tiles_n * tiles_n) * mats;
+    if (int rc = timer_begin(c)) return rc;
+    hipLaunchKernelGGL(ir_build32, dim3((unsigned)(2 * D), mats), dim3(256), 0, c->stream, a);
+    TUPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL((closed_solve<float, false>), dim3(mats), dim3(256), 0, c->stream, a);
+    TUPS_HIP(hipGetLastError());
+    b.b = nullptr;  // X = A32^-1 Gamma
+    hipLaunchKernelGGL(ir_gemm<0>, dim3(gblk), dim3(256), 0, c->stream, b);
+    TUPS_HIP(hipGetLastError());
+    int steps = 0;
+    double res = INFINITY;
+    for (;;) {
+      hipLaunchKernelGGL(ir_chi, dim3((unsigned)D, mats), dim3(256), 0, c->stream, b);
+      TUPS_HIP(hipGetLastError());
+      hipLaunchKernelGGL(ir_clear, dim3(1), dim3(1), 0, c->stream, b.rmax);
+      TUPS_HIP(hipGetLastError());
+      hipLaunchKernelGGL(ir_gemm<2>, dim3(gblk), dim3(256), 0, c->stream, b);
+      TUPS_HIP(hipGetLastError());
+      hipLaunchKernelGGL(ir_check, dim3((mats + 255) / 256), dim3(256), 0, c->stream, b, (int)mats);
+      TUPS_HIP(hipGetLastError());
+      unsigned long long bits = 0;
+      TUPS_HIP(hipMemcpyAsync(&bits, b.rmax, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+      TUPS_HIP(hipStreamSynchronize(c->stream));
+      std::memcpy(&res, &bits, sizeof(res));
+      if (res <= c->ir_tol || steps >= c->ir_max_steps) break;
+      b.b = b.rres;  // X += A32^-1 R
+      hipLaunchKernelGGL(ir_gemm<1>, dim3(gblk), dim3(256), 0, c->stream, b);
+      TUPS_HIP(hipGetLastError());
+      ++steps;
+    }
+    hipLaunchKernelGGL(ir_final, dim3((unsigned)((DD + 255) / 256), mats), dim3(256), 0, c->stream, b);
+    TUPS_HIP(hipGetLastError());
+    if (int rc = timer_end(c, TUPS_K_BSE_IR)) return rc;
+    steps_max = std::max(steps_max, steps);
+    res_max = std::max(res_max, res);
+    if (!(res <= c->ir_tol)) break;
+  }
+  int info = 0;
+  TUPS_HIP(hipMemcpyAsync(&info, a.info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  TUPS_HIP(hipStreamSynchronize(c->stream));
+  c->ir_steps = steps_max;
+  c->ir_residual = res_max;
+  TUPS_CHECK(info == 0, TUPS_ENONFINITE, "refined closed-form BSE: singular fp32(I - kappa Gamma chi) (zero pivot)");
+  TUPS_CHECK(res_max <= c->ir_tol, TUPS_ENOCONV,
+             "refined closed-form BSE: the residual did not reach the tolerance within the step limit");
   return TUPS_OK;
 }
 

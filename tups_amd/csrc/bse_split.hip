@@ -20,7 +20,8 @@
 //     GEMM; its INT32 sums are exact for 2D * 120^2 < 2^31 (D <= 1024).
 //  3. Re C' = (P+ + P-)/2 and Im C' = (P+ - P-)/(2j) (mod m); the Chinese
 //     remainder theorem gives C' = sum_m u_m W_m (mod M), W_m = M/m ((M/m)^-1
-//     mod m), M = prod m.  ta + tb <= log2 M - log2(8D) - 1 keeps |C'| < M/4,
+//     mod m), M = prod m; the factors 1/2 and 1/(2j) are folded into the
+//     weights, so only the residues of P+ + P- and P+ - P- are formed.  ta + tb <= log2 M - log2(8D) - 1 keeps |C'| < M/4,
 //     so C' is the representative nearest zero.  The sum is held as an exact
 //     high part (W rounded to 41 significant bits, a multiple of 2^L) plus a
 //     low part, so C' = (S_hi - q M_hi) + (S_lo - q M_lo), q = rint(S / M),
@@ -65,7 +66,10 @@ struct CrtArgs {
   double cf[4];
   double omf;
   double m[CRT_MAX], inv_m[CRT_MAX], j[CRT_MAX], c26[CRT_MAX];
-  double inv2[CRT_MAX], inv2j[CRT_MAX], whi[CRT_MAX], wlo[CRT_MAX];
+  // CRT weights of the Re and Im residues with the factors 1/2 and 1/(2j) of
+  // Re = (P+ + P-)/2, Im = (P+ - P-)/(2j) folded in, exact high part + low part
+  double wrh[CRT_MAX], wrl[CRT_MAX], wih[CRT_MAX], wil[CRT_MAX];
+  unsigned nm24[CRT_MAX];       // (-m) mod 2^24
   double mhi, mlo, inv_mm;
 };
 
@@ -85,6 +89,21 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int nblk) {
 // symmetric residue of an integer-valued y (|y| < 2^50): exact, |r| <= (m - 1) / 2
 __device__ __forceinline__ double sym_mod(double y, double m, double inv_m) {
   return fma(-m, rint(y * inv_m), y);
+}
+
+// symmetric residue of an INT32 y, as the low byte of the result: rint(y/m) is
+// read from the low mantissa bits of y/m + 1.5 2^52 (exact: y/m is never within
+// 2^-12 of a half integer) and y - m rint(y/m) is formed on 24-bit integers
+__device__ __forceinline__ int sym_mod_lo(int y, double inv_m, unsigned nm24) {
+  const double t = fma((double)y, inv_m, 0x1.8p52);
+  return (int)((unsigned)y + __umul24((unsigned)__double2loint(t), nm24));
+}
+
+// four low bytes packed into one dword, byte e from x_e
+__device__ __forceinline__ int pack4(int x0, int x1, int x2, int x3) {
+  const unsigned lo = __builtin_amdgcn_perm((unsigned)x1, (unsigned)x0, 0x0c0c0400u);
+  const unsigned hi = __builtin_amdgcn_perm((unsigned)x3, (unsigned)x2, 0x0c0c0400u);
+  return (int)__builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
 // x (integer valued, |x| <= 2^52) = hi 2^26 + lo, 0 <= lo < 2^26
@@ -365,8 +384,8 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
       }
     if (++kstep == nk) {
       // residues of this modulus -> CRT sums
-      const double m = a.m[mod], im = a.inv_m[mod], h2 = a.inv2[mod], h2j = a.inv2j[mod];
-      const double wh = a.whi[mod], wl = a.wlo[mod];
+      const double m = a.m[mod], im = a.inv_m[mod];
+      const double wrh = a.wrh[mod], wrl = a.wrl[mod], wih = a.wih[mod], wil = a.wil[mod];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -375,12 +394,12 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
           for (int e = 0; e < 4; ++e) {
             const int x = 8 * i + 4 * j + e;
             const int p = accp[i][j][e], n = accm[i][j][e];
-            const double ur = sym_mod((double)(p + n) * h2, m, im);
-            const double ui = sym_mod((double)(p - n) * h2j, m, im);
-            shr[x] = fma(ur, wh, shr[x]);
-            slr[x] = fma(ur, wl, slr[x]);
-            shi[x] = fma(ui, wh, shi[x]);
-            sli[x] = fma(ui, wl, sli[x]);
+            const double ur = sym_mod((double)(p + n), m, im);
+            const double ui = sym_mod((double)(p - n), m, im);
+            shr[x] = fma(ur, wrh, shr[x]);
+            slr[x] = fma(ur, wrl, slr[x]);
+            shi[x] = fma(ui, wih, shi[x]);
+            sli[x] = fma(ui, wil, sli[x]);
           }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -541,35 +560,33 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_big(CrtArgs a) {
     }
     const int md = step / (2 * nk), rem = step - md * 2 * nk;
     if (rem == nk - 1) {  // + product complete: keep its residues
-      const double m = a.m[md], im = a.inv_m[md];
+      const double im = a.inv_m[md];
+      const unsigned nm = a.nm24[md];
 #pragma unroll
       for (int i = 0; i < TB; ++i)
 #pragma unroll
         for (int j = 0; j < TB; ++j) {
-          int w = 0;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) w |= ((int)sym_mod((double)acc[i][j][e], m, im) & 0xff) << (8 * e);
-          pk[i][j] = w;
+          pk[i][j] = pack4(sym_mod_lo(acc[i][j][0], im, nm), sym_mod_lo(acc[i][j][1], im, nm),
+                           sym_mod_lo(acc[i][j][2], im, nm), sym_mod_lo(acc[i][j][3], im, nm));
           acc[i][j] = v4i{0, 0, 0, 0};
         }
-    } else if (rem == 2 * nk - 1) {  // - product complete: (Re, Im) residues to the scratch
-      const double m = a.m[md], im = a.inv_m[md], h2 = a.inv2[md], h2j = a.inv2j[md];
+    } else if (rem == 2 * nk - 1) {  // - product complete: residues of P+ + P-, P+ - P- to the scratch
+      const double im = a.inv_m[md];
+      const unsigned nm = a.nm24[md];
 #pragma unroll
       for (int i = 0; i < TB; ++i)
 #pragma unroll
         for (int j = 0; j < TB; ++j) {
-          int w0 = 0, w1 = 0;
+          int r[4], d[4];
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            const double p = (double)(signed char)(pk[i][j] >> (8 * e)), n = (double)acc[i][j][e];
-            const int ur = (int)sym_mod((p + n) * h2, m, im) & 0xff;
-            const int ui = (int)sym_mod((p - n) * h2j, m, im) & 0xff;
-            const int pair = ur | (ui << 8);
-            if (e < 2)
-              w0 |= pair << (16 * e);
-            else
-              w1 |= pair << (16 * (e - 2));
+            const int p = (int)(signed char)(pk[i][j] >> (8 * e)), n = acc[i][j][e];
+            r[e] = sym_mod_lo(p + n, im, nm);
+            d[e] = sym_mod_lo(p - n, im, nm);
           }
+          // dword 0: (Re, Im) bytes of elements 0, 1; dword 1: of elements 2, 3
+          const int w0 = pack4(r[0], d[0], r[1], d[1]);
+          const int w1 = pack4(r[2], d[2], r[3], d[3]);
           const int x = i * TB + j;  // block index; dwords 2x, 2x + 1 hold elements e = 0,1 and 2,3
           scr[((size_t)md * DW + 2 * x) * 256] = w0;
           scr[((size_t)md * DW + 2 * x + 1) * 256] = w1;
@@ -639,17 +656,18 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_big(CrtArgs a) {
       for (int k = 0; k < 4; ++k) {
         const bool live = m0 + k < a.nmod;
         const int mk = min(m0 + k, CRT_MAX - 1);
-        const double wh = live ? a.whi[mk] : 0.0, wl = live ? a.wlo[mk] : 0.0;
+        const double wrh = live ? a.wrh[mk] : 0.0, wrl = live ? a.wrl[mk] : 0.0;
+        const double wih = live ? a.wih[mk] : 0.0, wil = live ? a.wil[mk] : 0.0;
 #pragma unroll
         for (int g = 0; g < TB; ++g)
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             const int d = dv[k][g];
             const double ur = (double)(signed char)(d >> (16 * h)), ui = (double)(signed char)(d >> (16 * h + 8));
-            shr[g][h] = fma(ur, wh, shr[g][h]);
-            slr[g][h] = fma(ur, wl, slr[g][h]);
-            shi[g][h] = fma(ui, wh, shi[g][h]);
-            sli[g][h] = fma(ui, wl, sli[g][h]);
+            shr[g][h] = fma(ur, wrh, shr[g][h]);
+            slr[g][h] = fma(ur, wrl, slr[g][h]);
+            shi[g][h] = fma(ui, wih, shi[g][h]);
+            sli[g][h] = fma(ui, wil, sli[g][h]);
           }
       }
     }
@@ -720,15 +738,19 @@ static void crt_constants(CrtArgs& a, int nmod, int D) {
     while ((j * j + 1) % m) ++j;
     const u128 Mi = M / (u128)m;
     const long long yi = inv_mod((long long)(Mi % (u128)m), m);
-    const u128 W = Mi * (u128)yi;  // < M
     a.m[i] = (double)m;
     a.inv_m[i] = 1.0 / (double)m;
+    a.nm24[i] = (unsigned)((1u << 24) - (unsigned)m);
     a.j[i] = (double)j;
     a.c26[i] = (double)(((long long)1 << 26) % m);
-    a.inv2[i] = (double)((m + 1) / 2);
-    a.inv2j[i] = (double)inv_mod(2 * j, m);
-    a.whi[i] = std::ldexp((double)(unsigned long long)(W >> L), L);  // < 2^41 significant
-    a.wlo[i] = u128_to_double(W - ((W >> L) << L));
+    // residue u of Re (Im) = residue of (P+ + P-) (of P+ - P-) times 1/2 (1/(2j)) mod m;
+    // its CRT term u (M/m) yi is the residue times Wr (Wi) below, all < M
+    const long long h2 = (m + 1) / 2, h2j = inv_mod(2 * j, m);
+    const u128 Wr = Mi * (u128)((h2 * yi) % m), Wi = Mi * (u128)((h2j * yi) % m);
+    a.wrh[i] = std::ldexp((double)(unsigned long long)(Wr >> L), L);  // < 2^41 significant
+    a.wrl[i] = u128_to_double(Wr - ((Wr >> L) << L));
+    a.wih[i] = std::ldexp((double)(unsigned long long)(Wi >> L), L);
+    a.wil[i] = u128_to_double(Wi - ((Wi >> L) << L));
   }
   a.mhi = std::ldexp((double)(unsigned long long)(M >> L), L);
   a.mlo = u128_to_double(M - ((M >> L) << L));
@@ -742,28 +764,40 @@ static void crt_constants(CrtArgs& a, int nmod, int D) {
 
 // GEMM tile for an order D: 6 selects crt_gemm_big<6> (192 x 192 tiles) when
 // padding D to 192 costs at most 7% over the 64-row padding, else 0 (the
-// 64 x 64 kernel).  Measured per 256 q: D = 576 24.2 ms (big) vs 28.7 ms;
-// at D = 320 the 160-row variant (TB = 5) lost to the 64 x 64 kernel
-// (24.0 vs 22.6 ms: its per-modulus reductions and scratch epilogue weigh
-// more against the shorter inner dimension), so it is not selected.  The
-// padded order Dp is a multiple of the tile and of 64.
+// 64 x 64 kernel).  Measured per 256 q: D = 576 24.2 ms (big) vs 28.7 ms.  The
+// autotuner (TUPS_K_SPLIT_GEMM) times every tile on the resident inputs and
+// may pick another.  The padded order Dp is a multiple of the tile and of 64.
 static int crt_big_tb(int D) {
   const long long d64 = (D + 63) / 64 * 64;
   const long long L = std::lcm(32 * 6, 64);
   return (D + L - 1) / L * L * 100 <= d64 * 107 ? 6 : 0;
 }
 
-static size_t crt_padded(int D) {
-  const int tb = crt_big_tb(D);
+// tile variants: 0 = the 64 x 64 kernel, else crt_gemm_big<tb> (32 tb x 32 tb tiles)
+struct SplitVariant {
+  int tb;
+  const char* name;
+};
+static const SplitVariant kSplitVariants[] = {{0, "crt_64x64"}, {5, "crt_160x160"}, {6, "crt_192x192"}};
+constexpr int kNumSplitVariants = sizeof(kSplitVariants) / sizeof(kSplitVariants[0]);
+
+int split_variant_count() { return kNumSplitVariants; }
+const char* split_variant_name(int v) { return v >= 0 && v < kNumSplitVariants ? kSplitVariants[v].name : ""; }
+
+static int split_tb(const tups_ctx* c) {
+  const int v = c->split_variant;
+  return v >= 0 && v < kNumSplitVariants ? kSplitVariants[v].tb : crt_big_tb(c->sz.nz);
+}
+
+static size_t crt_padded(int D, int tb) {
   const size_t L = tb ? std::lcm(32 * tb, 64) : 64;
   return ((size_t)D + L - 1) / L * L;
 }
 
 // scratch bytes per matrix of the large-tile GEMM
-static size_t crt_scratch_per_mat(int D, int nmod) {
-  const int tb = crt_big_tb(D);
+static size_t crt_scratch_per_mat(int D, int nmod, int tb) {
   if (!tb) return 0;
-  const size_t tn = crt_padded(D) / (32 * tb);
+  const size_t tn = crt_padded(D, tb) / (32 * tb);
   return tn * tn * (size_t)nmod * (2 * tb * tb) * 256 * sizeof(int);
 }
 
@@ -785,11 +819,12 @@ int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
   const int nmod = c->split_moduli;
   TUPS_CHECK(nmod >= 8 && nmod <= CRT_MAX, TUPS_EINVAL, "split moduli must be in 8..16");
   TUPS_CHECK(s.nz <= 1024, TUPS_EINVAL, "split path needs Nz <= 1024 (exact INT32 sums)");
-  const size_t Dp = crt_padded(s.nz);
+  const int tb = split_tb(c);
+  const size_t Dp = crt_padded(s.nz, tb);
   const size_t Kp = (2 * (size_t)s.nz + 63) / 64 * 64;
   // per q point: 4 channels x (A and B) x 2 signs x nmod residue planes + exponents + GEMM scratch
   const size_t per_q = 4 * (4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int) + CRT_KB * Dp * sizeof(double) +
-                            crt_scratch_per_mat(s.nz, nmod) + 16);
+                            crt_scratch_per_mat(s.nz, nmod, tb) + 16);
   // default pass: about 4 GiB of residues, at most an eighth of the free memory
   // (fewer, larger passes measured 3-5% faster than 1 GiB ones); fixed at
   // first use, so a later call (possibly under stream capture) finds the same
@@ -826,9 +861,9 @@ int launch_bse_split(tups_ctx* c) {
   const int nmod = c->split_moduli;
   int qbatch = 0;
   if (int rc = bse_split_reserve(c, &qbatch)) return rc;
-  const size_t Dp = crt_padded(s.nz);
+  const int tb = split_tb(c);
+  const size_t Dp = crt_padded(s.nz, tb);
   const size_t Kp = (2 * (size_t)s.nz + 63) / 64 * 64;
-  const int tb = crt_big_tb(s.nz);
   CrtArgs a{};
   for (int r = 0; r < 4; ++r) {
     a.gam[r] = c->gamma[r];
@@ -875,11 +910,43 @@ int launch_bse_split(tups_ctx* c) {
     const long long nblk = (long long)a.tiles_n * a.tiles_n * mats;
     TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "split grid too large");
     if (int rc = timer_begin(c)) return rc;
-    void (*gemm)(CrtArgs) = tb == 6 ? crt_gemm_big<6> : crt_gemm;
+    void (*gemm)(CrtArgs) = tb == 6 ? crt_gemm_big<6> : (tb == 5 ? crt_gemm_big<5> : crt_gemm);
     hipLaunchKernelGGL(gemm, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_SPLIT_GEMM)) return rc;
   }
+  return TUPS_OK;
+}
+
+// Times one split step (preparation + GEMM, outputs to the ping-pong buffer) per
+// tile variant on the resident inputs after a warm-up, keeps the fastest.
+int autotune_split(tups_ctx* c, int* chosen, float* best_ms) {
+  float best = 0.f;
+  int arg = -1;
+  const int saved = c->split_variant;
+  for (int v = 0; v < kNumSplitVariants; ++v) {
+    c->split_variant = v;
+    if (int rc = launch_bse_split(c)) {
+      c->split_variant = saved;
+      return rc;
+    }
+    TUPS_HIP(hipEventRecord(c->tev[0], c->stream));
+    if (int rc = launch_bse_split(c)) {
+      c->split_variant = saved;
+      return rc;
+    }
+    TUPS_HIP(hipEventRecord(c->tev[1], c->stream));
+    TUPS_HIP(hipEventSynchronize(c->tev[1]));
+    float ms = 0.f;
+    TUPS_HIP(hipEventElapsedTime(&ms, c->tev[0], c->tev[1]));
+    if (arg < 0 || ms < best) {
+      best = ms;
+      arg = v;
+    }
+  }
+  c->split_variant = arg;
+  if (chosen) *chosen = arg;
+  if (best_ms) *best_ms = best;
   return TUPS_OK;
 }
 

@@ -112,12 +112,17 @@ int upload_tables(tups_ctx* c) {
   if ((rc = upload_vec(&c->d_signs, t.signs, c->stream))) return rc;
   PhirTable pt;
   build_phir_table(t, pt);
-  if ((rc = upload_vec(&c->d_ph_cnt, pt.cnt, c->stream))) return rc;
-  if ((rc = upload_vec(&c->d_ph_src, pt.src, c->stream))) return rc;
+  PhirBlocks pb;
+  if ((rc = build_phir_blocks(t, pt, pb))) return rc;
+  if ((rc = upload_vec(&c->d_ph_blk, pb.blk, c->stream))) return rc;
+  for (int k = 0; k < 4; ++k) {
+    c->ph_first[k] = pb.first[k];
+    c->ph_count[k] = pb.count[k];
+  }
   if (c->d_ph_coef) hipFree(c->d_ph_coef);
   c->d_ph_coef = nullptr;
-  TUPS_HIP(hipMalloc((void**)&c->d_ph_coef, std::max<size_t>(1, pt.coef.size()) * sizeof(double)));
-  TUPS_HIP(hipMemcpyAsync(c->d_ph_coef, pt.coef.data(), pt.coef.size() * sizeof(double),
+  TUPS_HIP(hipMalloc((void**)&c->d_ph_coef, std::max<size_t>(1, pb.coef.size()) * sizeof(double)));
+  TUPS_HIP(hipMemcpyAsync(c->d_ph_coef, pb.coef.data(), pb.coef.size() * sizeof(double),
                           hipMemcpyHostToDevice, c->stream));
   TUPS_HIP(hipStreamSynchronize(c->stream));
   c->tables_dirty = false;
@@ -186,7 +191,7 @@ int tups_create(const tups_dims* d, tups_ctx** out) {
   const int nx_ibz = (d->nx + 2) / 2;
   const int nibz = nx_ibz * (nx_ibz + 1) / 2;
   const int nr = d->nr > 0 ? d->nr : nr_default(d->nl);
-  TUPS_CHECK(nr <= nibz && nr <= kMaxNR, TUPS_EINVAL, "NR exceeds the IBZ size or the kernel limit (8)");
+  TUPS_CHECK(nr <= nibz && nr <= kMaxNR, TUPS_EINVAL, "NR exceeds the IBZ size or the kernel limit (16)");
   const int nw = d->nf / 2;
   const int nq_full = nibz * nw;
   const int nq = d->nq > 0 ? d->nq : nq_full;
@@ -307,8 +312,7 @@ int tups_destroy(tups_ctx* c) {
   f(c->d_qw);
   f(c->d_ibz);
   f(c->d_signs);
-  f(c->d_ph_cnt);
-  f(c->d_ph_src);
+  f(c->d_ph_blk);
   f(c->d_ph_coef);
   f(c->d_flag);
   f(c->pae_y);
@@ -433,17 +437,41 @@ int tups_upload_chi_dense(tups_ctx* c, int32_t pp, int32_t q0, int32_t nqs, cons
 
 int tups_bse(tups_ctx* c, int32_t mode) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
-  TUPS_CHECK(mode == TUPS_BSE_LADDER_FP64 || mode == TUPS_BSE_LADDER_SPLIT || mode == TUPS_BSE_CLOSED_LU,
+  TUPS_CHECK(mode == TUPS_BSE_LADDER_FP64 || mode == TUPS_BSE_LADDER_SPLIT || mode == TUPS_BSE_CLOSED_LU ||
+                 mode == TUPS_BSE_CLOSED_IR,
              TUPS_EINVAL, "unknown BSE mode");
   int rc = ensure_tables(c);
   if (rc) return rc;
   rc = mode == TUPS_BSE_LADDER_SPLIT ? launch_bse_split(c)
        : mode == TUPS_BSE_CLOSED_LU  ? launch_bse_closed(c)
+       : mode == TUPS_BSE_CLOSED_IR  ? launch_bse_closed_ir(c)
                                      : launch_bse_ladder(c);
   if (rc) return rc;
   // Phi overwrites G_r_LL (SRC/parquet_BSE.f90:81): swap the ping-pong pair
   swap_vertex_buffers(c);
   c->phir_valid = false;
+  return TUPS_OK;
+}
+
+int tups_set_product_forms(tups_ctx* c, int32_t allow_three) {
+  TUPS_CHECK(c, TUPS_EINVAL, "null context");
+  c->three_mult = allow_three != 0;
+  return TUPS_OK;
+}
+
+int tups_set_refinement(tups_ctx* c, double tol, int32_t max_steps) {
+  TUPS_CHECK(c, TUPS_EINVAL, "null context");
+  TUPS_CHECK(tol > 0.0 && max_steps >= 0 && max_steps <= 100, TUPS_EINVAL,
+             "refinement tolerance must be positive and the step limit in 0..100");
+  c->ir_tol = tol;
+  c->ir_max_steps = max_steps;
+  return TUPS_OK;
+}
+
+int tups_bse_solve_info(const tups_ctx* c, int32_t* steps, double* residual) {
+  TUPS_CHECK(c, TUPS_EINVAL, "null context");
+  if (steps) *steps = c->ir_steps;
+  if (residual) *residual = c->ir_residual;
   return TUPS_OK;
 }
 
@@ -511,8 +539,8 @@ int tups_gamma_update(tups_ctx* c) {
 
 int tups_autotune(tups_ctx* c, int32_t kernel, int32_t* variant, float* ms) {
   TUPS_CHECK(c, TUPS_EINVAL, "null context");
-  TUPS_CHECK(kernel == TUPS_K_BSE_LADDER || kernel == TUPS_K_PAE, TUPS_EINVAL,
-             "only the BSE ladder and the crossing have tile variants");
+  TUPS_CHECK(kernel == TUPS_K_BSE_LADDER || kernel == TUPS_K_PAE || kernel == TUPS_K_SPLIT_GEMM, TUPS_EINVAL,
+             "only the BSE ladder, its split GEMM and the crossing have tile variants");
   int rc = ensure_tables(c);
   if (rc) return rc;
   int v = -1;
@@ -521,6 +549,8 @@ int tups_autotune(tups_ctx* c, int32_t kernel, int32_t* variant, float* ms) {
   if (kernel == TUPS_K_PAE) {
     TUPS_CHECK(c->m_built && c->phir_valid, TUPS_ESTATE, "crossing autotune needs M and PhiR");
     rc = autotune_pae(c, 3, &v, &t);
+  } else if (kernel == TUPS_K_SPLIT_GEMM) {
+    rc = autotune_split(c, &v, &t);
   } else {
     rc = autotune_bse(c, 2, &v, &t);
   }
@@ -532,6 +562,7 @@ int tups_autotune(tups_ctx* c, int32_t kernel, int32_t* variant, float* ms) {
 
 const char* tups_variant_name(int32_t kernel, int32_t variant) {
   if (kernel == TUPS_K_BSE_BATCH) return batch_variant_name(variant);
+  if (kernel == TUPS_K_SPLIT_GEMM) return split_variant_name(variant);
   return kernel == TUPS_K_BSE_LADDER ? bse_variant_name(variant)
                                      : (kernel == TUPS_K_PAE ? pae_variant_name(variant) : "");
 }
@@ -544,6 +575,11 @@ int tups_set_variant(tups_ctx* c, int32_t kernel, int32_t variant) {
     c->pae_variant = variant;
     return TUPS_OK;
   }
+  if (kernel == TUPS_K_SPLIT_GEMM) {
+    TUPS_CHECK(variant >= -1 && variant < split_variant_count(), TUPS_EINVAL, "no such split GEMM variant");
+    c->split_variant = variant;
+    return TUPS_OK;
+  }
   TUPS_CHECK(kernel == TUPS_K_BSE_LADDER, TUPS_EINVAL, "kernel has no tile variants");
   TUPS_CHECK(variant == -1 || bse_variant_nl(variant) == c->sz.nl, TUPS_EINVAL,
              "variant does not exist for this Nl");
@@ -553,6 +589,7 @@ int tups_set_variant(tups_ctx* c, int32_t kernel, int32_t variant) {
 
 int tups_variant_count(int32_t kernel) {
   if (kernel == TUPS_K_BSE_BATCH) return batch_variant_count();
+  if (kernel == TUPS_K_SPLIT_GEMM) return split_variant_count();
   return kernel == TUPS_K_BSE_LADDER ? bse_variant_count() : (kernel == TUPS_K_PAE ? pae_variant_count() : 0);
 }
 

@@ -12,7 +12,7 @@
 namespace tups {
 
 constexpr int kNs = 8;       // C4v operations, SRC/parquet_ini.f90:26
-constexpr int kMaxNR = 8;    // register budget of the PhiR / crossing kernels
+constexpr int kMaxNR = 16;   // real-space shells (the PhiR kernel takes them eight at a time)
 constexpr int kEvents = 16;
 
 void set_error(const std::string& msg);
@@ -61,6 +61,20 @@ struct PhirTable {
   std::vector<double> coef;   // [((... *4 + slot)*nr + R)] complex interleaved, includes 1/Nx
 };
 void build_phir_table(const Tables& t, PhirTable& out);
+
+// The same table by symmetry blocks: the form factors fall into partner groups
+// of one or two (the C4v operations map l only inside its group), so for
+// groups Ga, Gb the outputs (l1, l2) in Ga x Gb draw only on sources in
+// Ga x Gb.  Per block: la0, la1, lb0, lb1 (la1 = la0 for a group of one),
+// na, nb, and the offset of its coefficients, laid out [m][R][o][s] with
+// o = i + na j for (l1, l2) = (Ga[i], Gb[j]) and s likewise for the source.
+// Blocks are ordered by shape (na, nb) = (1,1), (2,1), (1,2), (2,2).
+struct PhirBlocks {
+  std::vector<int> blk;        // [block][8]
+  std::vector<double> coef;    // complex interleaved
+  int first[4] = {0, 0, 0, 0}, count[4] = {0, 0, 0, 0};
+};
+int build_phir_blocks(const Tables& t, const PhirTable& pt, PhirBlocks& out);
 
 // Loads and stores of data touched once per kernel, marked non-temporal: a
 // plain G = F - G stream reaches 81% of HBM peak with them against 73% without.
@@ -113,9 +127,9 @@ struct tups_ctx {
   int* d_qw = nullptr;          // bosonic frequency of each q (1-based)
   int* d_ibz = nullptr;         // IBZ momenta: x[0..nibz), then y[0..nibz) (1-based)
   int* d_signs = nullptr;
-  int* d_ph_cnt = nullptr;
-  int* d_ph_src = nullptr;
-  double2* d_ph_coef = nullptr;
+  int* d_ph_blk = nullptr;        // PhiR symmetry blocks (PhirBlocks::blk)
+  double2* d_ph_coef = nullptr;   // their coefficients
+  int ph_first[4] = {0, 0, 0, 0}, ph_count[4] = {0, 0, 0, 0};
   int* d_flag = nullptr;
   double2* d_sigma = nullptr;   // calc_chi: Sigma(Nx Ny Nf)
   double2* d_sigh = nullptr;    // calc_chi: Sigma_H(Nx, Ny)
@@ -124,10 +138,12 @@ struct tups_ctx {
 
 
   int bse_variant = -1;          // kernel variant chosen by the autotuner (-1: default)
+  bool three_mult = false;       // the autotuner may pick the three-multiplication product forms
   int pae_variant = -1;
   int split_moduli = 12;         // split ladder: CRT moduli (8..16)
   int split_qbatch = 0;          // split ladder: q points per pass (0: default)
   int split_auto_q = 0;          // split ladder: the default, fixed at first use until set_split
+  int split_variant = -1;        // split ladder: GEMM tile variant (-1: chosen by the matrix order)
   void* split_ws = nullptr;      // split ladder: residues and exponents of one pass
   void* sde_ws = nullptr;        // SDE: auxiliary bubbles, contracted vertices, Sigma
   size_t sde_ws_bytes = 0;
@@ -157,6 +173,10 @@ struct tups_ctx {
   size_t split_ws_bytes = 0;
   void* closed_ws = nullptr;     // closed form: augmented matrices of one pass
   size_t closed_ws_bytes = 0;
+  double ir_tol = 1e-12;         // refined closed form: residual bound |R|_F / |Gamma|_F
+  int ir_max_steps = 10;         // and the most refinement steps
+  int ir_steps = 0;              // last closed-form call: refinement steps taken
+  double ir_residual = 0.0;      // and the largest |R|_F / |Gamma|_F reached (NaN for the direct mode)
   bool timing = false;
   hipEvent_t tev[2] = {};
   double t_ms[TUPS_K_COUNT] = {};
@@ -179,6 +199,7 @@ void swap_vertex_buffers(tups_ctx* c);
 int ensure_tables(tups_ctx* c);  // rebuild and upload the tables when stale
 void drop_graphs(tups_ctx* c);
 int launch_bse_closed(tups_ctx* c);
+int launch_bse_closed_ir(tups_ctx* c);
 int launch_calc_chi(tups_ctx* c, const double* sigma, const double* sigma_h, double mu, int ngrain);
 struct GreenParams;
 int upload_sigma(tups_ctx* c, const double* sigma, const double* sigma_h, GreenParams* g, double mu, int ngrain);
@@ -201,6 +222,9 @@ int bse_variant_nl(int v);
 const char* batch_variant_name(int v);
 int batch_variant_count();
 int autotune_pae(tups_ctx* c, int reps, int* chosen, float* best_ms);
+int autotune_split(tups_ctx* c, int* chosen, float* best_ms);
+int split_variant_count();
+const char* split_variant_name(int v);
 int pae_variant_count();
 const char* pae_variant_name(int v);
 bool pae_variant_ok(const tups_ctx* c, int v);
@@ -215,4 +239,31 @@ int upload_tables(tups_ctx* c);
 // bracket one kernel launch when per-kernel timing is on
 int timer_begin(tups_ctx* c);
 int timer_end(tups_ctx* c, int kernel);
+// one launch of the batching form's ladder, variant v, into the spare buffers (bse.hip)
+int batch_launch(tups_batch* b, int v);
 }  // namespace tups
+
+// the batching form (tups_batch_*): q points with their own Nf_q, Nl_q, packed
+// q after q; defined here so the uneven-box iteration can drive it
+struct tups_batch {
+  int nx = 0, nq = 0;
+  double alpha = 0, f_damping = 0;
+  std::vector<int> nf, nl;
+  std::vector<long long> voff, coff;  // offsets, with totals at [nq]
+  hipStream_t stream = nullptr;
+  bool own_stream = true;             // false when the stream belongs to an uneven-box iteration
+  double2* gamma[4] = {};
+  double2* fvert[4] = {};
+  double2* spare[4] = {};
+  double2* chi[2] = {};
+  int4* d_items = nullptr;
+  long long* d_voff = nullptr;
+  long long* d_coff = nullptr;
+  int* d_nf = nullptr;
+  int* d_nl = nullptr;
+  int* d_tickets = nullptr;
+  int nitems = 0;  // work items (tiles)
+  int len = 0;     // entries per XCD list, padding included
+  int variant = 0;
+  bool three_mult = false;            // the autotuner may pick the three-multiplication product forms
+};

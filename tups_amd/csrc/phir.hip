@@ -10,74 +10,108 @@
 // omega), so the kernel reads Phi_r(q) where the BSE left it.
 //
 // The reference zeroes and rewrites the whole PhiQ array once per symmetry and
-// frequency; here the eight operations are folded on the host into at most
-// four (source l-pair, coefficient[R]) entries per (l1, l2, Q), so every Phi
-// element is read once per distinct source and PhiR is written once: the
-// kernel is HBM-bound on the read of Phi.
+// frequency.  Here the eight operations are folded on the host into
+// coefficients per symmetry block: the form factors fall into partner groups
+// of one or two, and the outputs (l1, l2) of groups Ga x Gb draw only on the
+// Phi entries of the same Ga x Gb.  One thread owns one (nu1, nu2, w) of one
+// block and channel: per IBZ momentum it loads the block's |Ga||Gb| entries
+// (lanes along nu1, so every load is a coalesced row) and accumulates all
+// |Ga||Gb| outputs for eight shells R at once.  Every Phi element is read
+// exactly once per eight shells and PhiR is written once, so the kernel is
+// HBM-bound: bytes per launch = 16 (4 Nz^2 Nq ceil(NR / 8) + 4 Nz^2 Nw NR).
 #include "internal.h"
 
 namespace tups {
 
-__global__ __launch_bounds__(256) void phir_kernel(const double2* __restrict__ phi0,
-                                                   const double2* __restrict__ phi1,
-                                                   const double2* __restrict__ phi2,
-                                                   const double2* __restrict__ phi3,
-                                                   double2* out0, double2* out1, double2* out2,
-                                                   double2* out3, const int* __restrict__ cnt,
-                                                   const int* __restrict__ src,
-                                                   const double2* __restrict__ coef, int nf,
-                                                   int nl, int nw, int nibz, int nr) {
-  const int nz = nf * nl;
+constexpr int kPhirR = 8;  // shells per thread
+
+struct PhirArgs {
+  const double2* phi[4];
+  double2* out[4];
+  const int* blk;         // [block][8]: la0, la1, lb0, lb1, na, nb, coefficient offset, -
+  const double2* coef;    // per block [m][R][o][s]
+  int nf, nl, nw, nibz, nr, first;
+};
+
+// grid: (ceil(Nf^2 Nw / 256), 4 * blocks of this shape, ceil(NR / 8))
+template <int NA, int NB>
+__global__ __launch_bounds__(256) void phir_block(PhirArgs a) {
+  constexpr int NO = NA * NB;
+  const int nf = a.nf, nz = nf * a.nl;
   const size_t nz2 = (size_t)nz * nz;
-  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= nz2 * nw) return;
-  const int ch = blockIdx.y;
-  const double2* phi = ch == 0 ? phi0 : ch == 1 ? phi1 : ch == 2 ? phi2 : phi3;
-  double2* out = ch == 0 ? out0 : ch == 1 ? out1 : ch == 2 ? out2 : out3;
-  const size_t z = gid % nz2;
-  const int w = (int)(gid / nz2);
-  const int row = (int)(z % nz), col = (int)(z / nz);
-  const int l1 = row / nf, nu1 = row - l1 * nf;
-  const int l2 = col / nf, nu2 = col - l2 * nf;
-
-  double ar[kMaxNR], ai[kMaxNR];
+  const int ch = blockIdx.y & 3, b = a.first + (blockIdx.y >> 2);
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nf * nf * a.nw) return;
+  const int nu1 = t % nf, nu2 = (t / nf) % nf, w = t / (nf * nf);
+  const int* bk = a.blk + 8 * b;
+  const int la[2] = {bk[0], bk[1]}, lb[2] = {bk[2], bk[3]};
+  const double2* cf0 = a.coef + bk[6];
+  size_t off[NO];
 #pragma unroll
-  for (int r = 0; r < kMaxNR; ++r) ar[r] = ai[r] = 0.0;
-
-  const size_t e0 = ((size_t)l1 * nl + l2) * nibz;
-  for (int m = 0; m < nibz; ++m) {
-    const size_t e = e0 + m;
-    const double2* base = phi + nz2 * ((size_t)m * nw + w);
-    const int n = cnt[e];
-    for (int j = 0; j < n; ++j) {
-      const int key = src[e * 4 + j];
-      const int l1s = key % nl, l2s = key / nl;
-      const double2 v = base[(size_t)(l1s * nf + nu1) + (size_t)nz * (l2s * nf + nu2)];
-      const double2* cf = coef + (e * 4 + j) * nr;
+  for (int j = 0; j < NB; ++j)
 #pragma unroll
-      for (int r = 0; r < kMaxNR; ++r) {
-        if (r < nr) {
-          const double2 c = cf[r];
-          ar[r] += c.x * v.x - c.y * v.y;
-          ai[r] += c.x * v.y + c.y * v.x;
-        }
+    for (int i = 0; i < NA; ++i) off[i + NA * j] = (size_t)(la[i] * nf + nu1) + (size_t)nz * (lb[j] * nf + nu2);
+  const int r0 = blockIdx.z * kPhirR, nrc = min(kPhirR, a.nr - r0);
+  const double2* phi = a.phi[ch] + nz2 * w;
+  const size_t mstride = nz2 * a.nw;
+  double ar[NO][kPhirR], ai[NO][kPhirR];
+#pragma unroll
+  for (int o = 0; o < NO; ++o)
+#pragma unroll
+    for (int r = 0; r < kPhirR; ++r) ar[o][r] = ai[o][r] = 0.0;
+#pragma unroll 2
+  for (int m = 0; m < a.nibz; ++m) {
+    double2 v[NO];
+#pragma unroll
+    for (int s = 0; s < NO; ++s) v[s] = ld_stream(phi + mstride * m + off[s]);
+    const double2* cf = cf0 + ((size_t)m * a.nr + r0) * NO * NO;
+#pragma unroll
+    for (int r = 0; r < kPhirR; ++r) {
+      if (r < nrc) {
+#pragma unroll
+        for (int o = 0; o < NO; ++o)
+#pragma unroll
+          for (int s = 0; s < NO; ++s) {
+            const double2 c = cf[(r * NO + o) * NO + s];
+            ar[o][r] = fma(c.x, v[s].x, fma(-c.y, v[s].y, ar[o][r]));
+            ai[o][r] = fma(c.x, v[s].y, fma(c.y, v[s].x, ai[o][r]));
+          }
       }
     }
   }
+  double2* out = a.out[ch];
 #pragma unroll
-  for (int r = 0; r < kMaxNR; ++r)
-    if (r < nr) out[z + nz2 * ((size_t)w + (size_t)nw * r)] = make_double2(ar[r], ai[r]);
+  for (int o = 0; o < NO; ++o)
+#pragma unroll
+    for (int r = 0; r < kPhirR; ++r)
+      if (r < nrc)
+        st_stream(out + off[o] + nz2 * ((size_t)w + (size_t)a.nw * (r0 + r)), make_double2(ar[o][r], ai[o][r]));
 }
 
 int launch_phir(tups_ctx* c) {
   const tups_sizes& s = c->sz;
-  const size_t n = (size_t)s.nz * s.nz * s.nw;
-  dim3 g((unsigned)((n + 255) / 256), 4);
+  PhirArgs a{};
+  for (int r = 0; r < 4; ++r) {
+    a.phi[r] = c->gamma[r];
+    a.out[r] = c->phir[r];
+  }
+  a.blk = c->d_ph_blk;
+  a.coef = c->d_ph_coef;
+  a.nf = s.nf;
+  a.nl = s.nl;
+  a.nw = s.nw;
+  a.nibz = s.nibz;
+  a.nr = s.nr;
+  const unsigned gx = (unsigned)(((size_t)s.nf * s.nf * s.nw + 255) / 256);
+  const unsigned gz = (unsigned)((s.nr + kPhirR - 1) / kPhirR);
+  void (*kern[4])(PhirArgs) = {phir_block<1, 1>, phir_block<2, 1>, phir_block<1, 2>, phir_block<2, 2>};
   if (int rc = timer_begin(c)) return rc;
-  hipLaunchKernelGGL(phir_kernel, g, dim3(256), 0, c->stream, c->gamma[0], c->gamma[1],
-                     c->gamma[2], c->gamma[3], c->phir[0], c->phir[1], c->phir[2], c->phir[3],
-                     c->d_ph_cnt, c->d_ph_src, c->d_ph_coef, s.nf, s.nl, s.nw, s.nibz, s.nr);
-  TUPS_HIP(hipGetLastError());
+  for (int sh = 0; sh < 4; ++sh) {
+    if (!c->ph_count[sh]) continue;
+    a.first = c->ph_first[sh];
+    hipLaunchKernelGGL(kern[sh], dim3(gx, 4u * (unsigned)c->ph_count[sh], gz), dim3(256), 0, c->stream, a);
+    TUPS_HIP(hipGetLastError());
+  }
   if (int rc = timer_end(c, TUPS_K_PHIR)) return rc;
   return TUPS_OK;
 }

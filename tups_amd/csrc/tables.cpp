@@ -313,4 +313,61 @@ void build_phir_table(const Tables& t, PhirTable& out) {
       }
 }
 
+int build_phir_blocks(const Tables& t, const PhirTable& pt, PhirBlocks& out) {
+  const int nl = t.nl, nm = t.nibz, nr = t.nr;
+  // partner groups: the images of l under the eight operations
+  std::vector<int> gid(nl, -1);
+  std::vector<std::vector<int>> groups;
+  for (int l = 0; l < nl; ++l) {
+    if (gid[l] >= 0) continue;
+    std::vector<int> g;
+    for (int s = 0; s < kNs; ++s) {
+      const int ls = t.lsym[s * nl + l];
+      if (std::find(g.begin(), g.end(), ls) == g.end()) g.push_back(ls);
+    }
+    std::sort(g.begin(), g.end());
+    TUPS_CHECK(g.size() <= 2, TUPS_EINVAL, "PhiR: a form-factor partner group larger than two");
+    for (int x : g) {
+      TUPS_CHECK(gid[x] < 0, TUPS_EINVAL, "PhiR: form-factor partner groups overlap");
+      gid[x] = (int)groups.size();
+    }
+    groups.push_back(g);
+  }
+  out.blk.clear();
+  out.coef.clear();
+  for (int sh = 0; sh < 4; ++sh) {
+    const int na = 1 + (sh & 1), nb = 1 + (sh >> 1), no = na * nb;
+    out.first[sh] = (int)(out.blk.size() / 8);
+    out.count[sh] = 0;
+    for (const auto& ga : groups)
+      for (const auto& gb : groups) {
+        if ((int)ga.size() != na || (int)gb.size() != nb) continue;
+        const size_t off = out.coef.size() / 2;
+        out.blk.insert(out.blk.end(), {ga[0], ga[na - 1], gb[0], gb[nb - 1], na, nb, (int)off, 0});
+        out.coef.resize(out.coef.size() + (size_t)2 * nm * nr * no * no, 0.0);
+        for (int i = 0; i < na; ++i)
+          for (int j = 0; j < nb; ++j)
+            for (int m = 0; m < nm; ++m) {
+              const size_t e = ((size_t)ga[i] * nl + gb[j]) * nm + m;
+              const int o = i + na * j;
+              for (int slot = 0; slot < pt.cnt[e]; ++slot) {
+                const int key = pt.src[e * 4 + slot], l1s = key % nl, l2s = key / nl;
+                const int is = (int)(std::find(ga.begin(), ga.end(), l1s) - ga.begin());
+                const int js = (int)(std::find(gb.begin(), gb.end(), l2s) - gb.begin());
+                TUPS_CHECK(is < na && js < nb, TUPS_EINVAL, "PhiR: a source outside its symmetry block");
+                const int sidx = is + na * js;
+                for (int r = 0; r < nr; ++r) {
+                  const size_t ci = 2 * (off + (((size_t)m * nr + r) * no + o) * no + sidx);
+                  const size_t pi = ((e * 4 + slot) * nr + r) * 2;
+                  out.coef[ci] += pt.coef[pi];
+                  out.coef[ci + 1] += pt.coef[pi + 1];
+                }
+              }
+            }
+        ++out.count[sh];
+      }
+  }
+  return TUPS_OK;
+}
+
 }  // namespace tups

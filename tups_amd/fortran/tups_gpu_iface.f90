@@ -17,12 +17,12 @@ module tups_gpu_iface
   integer(c_int32_t), parameter :: TUPS_ARR_GAMMA = 0, TUPS_ARR_F = 1, TUPS_ARR_CHI = 2, TUPS_ARR_PHIR = 3, &
                                    TUPS_ARR_M = 4, TUPS_ARR_LAMBDA = 5
   integer(c_int32_t), parameter :: TUPS_BSE_LADDER_FP64 = 0, TUPS_BSE_LADDER_SPLIT = 1, &
-                                   TUPS_BSE_CLOSED_LU = 2
+                                   TUPS_BSE_CLOSED_LU = 2, TUPS_BSE_CLOSED_IR = 3
   integer(c_int32_t), parameter :: TUPS_K_BSE_LADDER = 0, TUPS_K_PHIR = 1, TUPS_K_BUILD_M = 2, &
                                    TUPS_K_PAE = 3, TUPS_K_GAMMA_UPDATE = 4, TUPS_K_PAE_GATHER = 5, &
                                    TUPS_K_SPLIT_PREP = 6, TUPS_K_SPLIT_GEMM = 7, TUPS_K_BSE_CLOSED = 8, &
                                    TUPS_K_CALC_CHI = 9, TUPS_K_BSE_BATCH = 10, TUPS_K_SDE = 11, &
-                                   TUPS_K_GTAU = 12, TUPS_K_CONV = 13, TUPS_K_SUS = 14
+                                   TUPS_K_GTAU = 12, TUPS_K_CONV = 13, TUPS_K_SUS = 14, TUPS_K_BSE_IR = 15
 
   type, bind(C) :: tups_dims
     integer(c_int32_t) :: nx, nf, nl, nr, nq, reserved
@@ -149,6 +149,22 @@ module tups_gpu_iface
       integer(c_int) :: rc
     end function tups_bse
 
+    function tups_set_refinement(ctx, tol, max_steps) bind(C, name="tups_set_refinement") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: tol
+      integer(c_int32_t), value :: max_steps
+      integer(c_int) :: rc
+    end function tups_set_refinement
+
+    function tups_bse_solve_info(ctx, steps, residual) bind(C, name="tups_bse_solve_info") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: steps
+      real(c_double), intent(out) :: residual
+      integer(c_int) :: rc
+    end function tups_bse_solve_info
+
     function tups_batch_create(nx, nq, nf, nl, beta, f_damping, batch) &
         bind(C, name="tups_batch_create") result(rc)
       import :: c_ptr, c_int32_t, c_int, c_double
@@ -217,6 +233,58 @@ module tups_gpu_iface
       integer(c_int) :: rc
     end function tups_batch_set_variant
 
+    ! uneven-box iteration: nf_w(Nw) boxes per bosonic frequency, nl_m(NIBZ) form factors per momentum
+    function tups_set_product_forms(ctx, allow_three) bind(C, name="tups_set_product_forms") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: allow_three
+      integer(c_int) :: rc
+    end function tups_set_product_forms
+
+    function tups_batch_set_product_forms(batch, allow_three) &
+        bind(C, name="tups_batch_set_product_forms") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: allow_three
+      integer(c_int) :: rc
+    end function tups_batch_set_product_forms
+
+    function tups_uneven_create(nx, nf_max, nf_w, nl_m, beta, f_damping, u) &
+        bind(C, name="tups_uneven_create") result(rc)
+      import :: c_ptr, c_int32_t, c_int, c_double
+      integer(c_int32_t), value :: nx, nf_max
+      integer(c_int32_t), intent(in) :: nf_w(*), nl_m(*)
+      real(c_double), value :: beta, f_damping
+      type(c_ptr), intent(out) :: u
+      integer(c_int) :: rc
+    end function tups_uneven_create
+
+    function tups_uneven_destroy(u) bind(C, name="tups_uneven_destroy") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: u
+      integer(c_int) :: rc
+    end function tups_uneven_destroy
+
+    function tups_uneven_parts(u, batch, ctx) bind(C, name="tups_uneven_parts") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: u
+      type(c_ptr), intent(out) :: batch, ctx
+      integer(c_int) :: rc
+    end function tups_uneven_parts
+
+    function tups_uneven_iteration(u, use_graph) bind(C, name="tups_uneven_iteration") result(rc)
+      import :: c_ptr, c_int32_t, c_int
+      type(c_ptr), value :: u
+      integer(c_int32_t), value :: use_graph
+      integer(c_int) :: rc
+    end function tups_uneven_iteration
+
+    function tups_uneven_sync(u) bind(C, name="tups_uneven_sync") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: u
+      integer(c_int) :: rc
+    end function tups_uneven_sync
+
     function tups_sde(ctx, sigma, sigma_h, mu, ngrain, f_range, u, use_u2, sigma_out) &
         bind(C, name="tups_sde") result(rc)
       import :: c_ptr, c_int32_t, c_int, c_double, c_double_complex
@@ -275,6 +343,7 @@ module tups_gpu_iface
       type(tups_conv_info), intent(out) :: info
       integer(c_int) :: rc
     end function tups_conv
+
 
     ! calc_sus (SRC/parquet_sus.f90:18): sus(Nq, 4, 4) = sus_X_IBZ(q, ff) for X = d, m, s, t
     function tups_calc_sus(ctx, u, f_range, after_update, sus) bind(C, name="tups_calc_sus") result(rc)

@@ -19,7 +19,7 @@ import sys
 
 import numpy as np
 
-from .runtime import BSE_LADDER_SPLIT, CHI, F, GAMMA, K_PAE, LAMBDA, TupsBatch, TupsContext
+from .runtime import BSE_LADDER_SPLIT, CHI, F, GAMMA, K_PAE, LAMBDA, TupsBatch, TupsContext, TupsUneven
 from .synth import batched_bse_inputs
 
 
@@ -100,6 +100,44 @@ def batch_context(nx=8, nw=16, U=2.0, seed=0):
     for p in range(2):
         b.upload_packed(CHI, p, rng.standard_normal(b.chi_elems) + 1j * rng.standard_normal(b.chi_elems))
     return b
+
+
+def uneven_shape(nx=8, nf_max=64, nf_min=16):
+    """Boxes of the uneven-box iteration: Nf_w shrinking linearly (kept even) from
+    nf_max at w = 0 to nf_min at the last bosonic frequency, Nl_m cycling 1, 5, 9
+    over the IBZ momenta."""
+    nw = nf_max // 2
+    nxi = (nx + 2) // 2
+    nibz = nxi * (nxi + 1) // 2
+    nf_w = [int(2 * round((nf_max - (nf_max - nf_min) * w / max(1, nw - 1)) / 2)) for w in range(nw)]
+    nl_m = [(1, 5, 9)[m % 3] for m in range(nibz)]
+    return nf_w, nl_m
+
+
+def uneven_context(nx=8, nf_max=64, nf_min=16, U=2.0, seed=0):
+    """TupsUneven of that shape: Gamma = F per q from N(0, (0.3U)^2/D_q) plus U on the
+    diagonal, bubble blocks of scale 0.1, the parquet-approximation Lambda
+    (U, -U, 2U, 0) on the largest box."""
+    nf_w, nl_m = uneven_shape(nx, nf_max, nf_min)
+    u = TupsUneven(nx, nf_max, nf_w, nl_m, beta=2.0, f_damping=0.5)
+    b = u.batch
+    rng = np.random.default_rng(seed)
+    d = np.array([f * l for f, l in zip(b.nf, b.nl)], dtype=np.int64)
+    sizes = d * d
+    off = np.concatenate(([0], np.cumsum(sizes)))
+    scale = np.repeat(0.3 * U / np.sqrt(d), sizes)
+    diag = np.concatenate([off[q] + np.arange(d[q]) * (d[q] + 1) for q in range(len(d))])
+    for r in range(4):
+        v = (rng.standard_normal(b.vertex_elems) + 1j * rng.standard_normal(b.vertex_elems)) * scale
+        v[diag] += U
+        b.upload_packed(GAMMA, r, v)
+        b.upload_packed(F, r, v)
+    for p in range(2):
+        b.upload_packed(CHI, p, 0.1 * (rng.standard_normal(b.chi_elems) + 1j * rng.standard_normal(b.chi_elems)))
+    nw = nf_max // 2
+    for r, val in enumerate((U, -U, 2.0 * U, 0.0)):
+        u.ctx.upload(LAMBDA, r, np.full((nf_max, nf_max, nw), val, dtype=np.complex128, order="F"))
+    return u
 
 
 def batch_flops(b: TupsBatch) -> float:

@@ -21,11 +21,12 @@ LIB_PATH = Path(__file__).resolve().parent / "libtups_gpu.so"
 
 GAMMA, F, CHI, PHIR, M, LAMBDA = range(6)
 (K_BSE_LADDER, K_PHIR, K_BUILD_M, K_PAE, K_GAMMA_UPDATE, K_PAE_GATHER, K_SPLIT_PREP, K_SPLIT_GEMM,
- K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH, K_SDE, K_GTAU, K_CONV, K_SUS) = range(15)
+ K_BSE_CLOSED, K_CALC_CHI, K_BSE_BATCH, K_SDE, K_GTAU, K_CONV, K_SUS, K_BSE_IR) = range(16)
 CH_D, CH_M, CH_S, CH_T = range(4)
 BSE_LADDER_FP64 = 0
 BSE_LADDER_SPLIT = 1
 BSE_CLOSED_LU = 2
+BSE_CLOSED_IR = 3
 M_NAMES = ("phbar_to_ph", "pp_to_ph", "ph_to_pp", "phbar_to_pp")
 
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE", -6: "ENOCONV"}
@@ -81,6 +82,8 @@ SIGNATURES = {
     "tups_download": ([_V, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_upload_chi_dense": ([_V, C.c_int32, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_bse": ([_V, C.c_int32], C.c_int),
+    "tups_set_refinement": ([_V, C.c_double, C.c_int32], C.c_int),
+    "tups_bse_solve_info": ([_V, C.POINTER(C.c_int32), _D], C.c_int),
     "tups_set_split": ([_V, C.c_int32, C.c_int32], C.c_int),
     "tups_calc_chi": ([_V, _D, _D, C.c_double, C.c_int32], C.c_int),
     "tups_sde": ([_V, _D, _D, C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_int32, _D], C.c_int),
@@ -104,6 +107,13 @@ SIGNATURES = {
     "tups_batch_sync": ([_V], C.c_int),
     "tups_batch_autotune": ([_V, _I, _P(C.c_float)], C.c_int),
     "tups_batch_set_variant": ([_V, C.c_int32], C.c_int),
+    "tups_batch_set_product_forms": ([_V, C.c_int32], C.c_int),
+    "tups_set_product_forms": ([_V, C.c_int32], C.c_int),
+    "tups_uneven_create": ([C.c_int32, C.c_int32, _I, _I, C.c_double, C.c_double, _P(_V)], C.c_int),
+    "tups_uneven_destroy": ([_V], C.c_int),
+    "tups_uneven_parts": ([_V, _P(_V), _P(_V)], C.c_int),
+    "tups_uneven_iteration": ([_V, C.c_int32], C.c_int),
+    "tups_uneven_sync": ([_V], C.c_int),
     "tups_phir": ([_V], C.c_int),
     "tups_build_m": ([_V], C.c_int),
     "tups_pae": ([_V], C.c_int),
@@ -246,11 +256,15 @@ class TupsContext:
     """HBM-resident state of one rank's share of the TUPS vertex hot path."""
 
     def __init__(self, nx: int, nf: int, nl: int, nr: int = 0, nq: int = 0, beta: float = 2.0,
-                 f_damping: float = 0.5):
+                 f_damping: float = 0.5, _handle=None):
         self.lib = load_library()
-        self._h = _V()
-        d = make_dims(nx, nf, nl, nr, nq, beta, f_damping)
-        _check(self.lib.tups_create(C.byref(d), C.byref(self._h)), "tups_create")
+        self._owned = _handle is None
+        if _handle is None:
+            self._h = _V()
+            d = make_dims(nx, nf, nl, nr, nq, beta, f_damping)
+            _check(self.lib.tups_create(C.byref(d), C.byref(self._h)), "tups_create")
+        else:
+            self._h = _handle
         s = Sizes()
         _check(self.lib.tups_query(self._h, C.byref(s)), "tups_query")
         self.sizes = s
@@ -264,7 +278,8 @@ class TupsContext:
     # --- lifetime -------------------------------------------------------------------
     def close(self):
         if self._h:
-            self.lib.tups_destroy(self._h)
+            if self._owned:
+                self.lib.tups_destroy(self._h)
             self._h = _V()
 
     def __enter__(self):
@@ -320,6 +335,17 @@ class TupsContext:
     def BSE(self, ite: int = 0, mode: int = BSE_LADDER_FP64):
         """BSE(ite), SRC/parquet_BSE.f90:19-135: G_r_LL <- Phi_r for r = d, m, s, t."""
         _check(self.lib.tups_bse(self._h, mode), "tups_bse")
+
+    def set_refinement(self, tol: float = 1e-12, max_steps: int = 10):
+        """Residual bound and step limit of the refined closed form (BSE_CLOSED_IR)."""
+        _check(self.lib.tups_set_refinement(self._h, float(tol), int(max_steps)), "tups_set_refinement")
+
+    def solve_info(self):
+        """(refinement steps, largest |R|_F / |Gamma|_F) of the last closed-form BSE."""
+        steps = C.c_int32(0)
+        res = C.c_double(0.0)
+        _check(self.lib.tups_bse_solve_info(self._h, C.byref(steps), C.byref(res)), "tups_bse_solve_info")
+        return steps.value, res.value
 
     def set_split(self, moduli: int = 12, qbatch: int = 0):
         """CRT moduli and q points per pass of the split (INT8 MFMA) ladder."""
@@ -477,6 +503,9 @@ class TupsContext:
         out = []
         for v in range(self.lib.tups_variant_count(kernel)):
             name = self.lib.tups_variant_name(kernel, v).decode()
+            if kernel == K_SPLIT_GEMM:  # every tile serves every Nl
+                out.append((v, name))
+                continue
             if not name.startswith(f"nl{self.nl}_"):
                 continue
             if kernel == K_PAE and "_nr" not in name and self.nr != 1:
@@ -486,6 +515,10 @@ class TupsContext:
 
     def set_variant(self, variant: int, kernel: int = K_BSE_LADDER):
         _check(self.lib.tups_set_variant(self._h, kernel, variant), "tups_set_variant")
+
+    def set_product_forms(self, allow_three: bool):
+        """Let the ladder autotuner pick the three-multiplication complex products."""
+        _check(self.lib.tups_set_product_forms(self._h, 1 if allow_three else 0), "tups_set_product_forms")
 
     # --- stream / timing ----------------------------------------------------------------
     def sync(self):
@@ -527,18 +560,22 @@ class TupsBatch:
     of per-q (Nl_q, Nl_q, Nf_q) blocks; they are packed q after q for the C ABI.
     """
 
-    def __init__(self, nx: int, nf, nl, beta: float = 2.0, f_damping: float = 0.5):
+    def __init__(self, nx: int, nf, nl, beta: float = 2.0, f_damping: float = 0.5, _handle=None):
         self.lib = load_library()
         self.nf = [int(x) for x in nf]
         self.nl = [int(x) for x in nl]
         self.nq = len(self.nf)
         self.nx = nx
         self.alpha = 1.0 / (beta * nx * nx)
-        h = _V()
-        nfa = np.ascontiguousarray(self.nf, dtype=np.int32)
-        nla = np.ascontiguousarray(self.nl, dtype=np.int32)
-        _check(self.lib.tups_batch_create(nx, self.nq, _iptr(nfa), _iptr(nla), beta, f_damping, C.byref(h)),
-               "tups_batch_create")
+        self._owned = _handle is None
+        if _handle is None:
+            h = _V()
+            nfa = np.ascontiguousarray(self.nf, dtype=np.int32)
+            nla = np.ascontiguousarray(self.nl, dtype=np.int32)
+            _check(self.lib.tups_batch_create(nx, self.nq, _iptr(nfa), _iptr(nla), beta, f_damping, C.byref(h)),
+                   "tups_batch_create")
+        else:
+            h = _handle
         self._h = h
         ve, ce, it = C.c_int64(), C.c_int64(), C.c_int32()
         _check(self.lib.tups_batch_query(self._h, C.byref(ve), C.byref(ce), C.byref(it)), "tups_batch_query")
@@ -546,7 +583,8 @@ class TupsBatch:
 
     def close(self):
         if getattr(self, "_h", None):
-            self.lib.tups_batch_destroy(self._h)
+            if self._owned:
+                self.lib.tups_batch_destroy(self._h)
             self._h = None
 
     def __enter__(self):
@@ -620,3 +658,61 @@ class TupsBatch:
 
     def set_variant(self, variant: int):
         _check(self.lib.tups_batch_set_variant(self._h, variant), "tups_batch_set_variant")
+
+    def set_product_forms(self, allow_three: bool):
+        """Let the batch autotuner pick the three-multiplication complex products."""
+        _check(self.lib.tups_batch_set_product_forms(self._h, 1 if allow_three else 0),
+               "tups_batch_set_product_forms")
+
+
+class TupsUneven:
+    """Uneven-box parquet iteration: IBZ q = (m, w), q = m Nw + w, with the box
+    nf_w[w] per bosonic frequency and nl_m[m] form factors per IBZ momentum; no
+    reference counterpart.  `batch` (a TupsBatch view) holds the per-q vertices
+    and bubbles, `ctx` (a TupsContext view of the largest box) Lambda; both are
+    owned by this object."""
+
+    def __init__(self, nx: int, nf_max: int, nf_w, nl_m, beta: float = 2.0, f_damping: float = 0.5):
+        self.lib = load_library()
+        self.nf_w = [int(x) for x in nf_w]
+        self.nl_m = [int(x) for x in nl_m]
+        self.nx, self.nf_max, self.nw = nx, nf_max, nf_max // 2
+        h = _V()
+        nfa = np.ascontiguousarray(self.nf_w, dtype=np.int32)
+        nla = np.ascontiguousarray(self.nl_m, dtype=np.int32)
+        _check(self.lib.tups_uneven_create(nx, nf_max, _iptr(nfa), _iptr(nla), beta, f_damping, C.byref(h)),
+               "tups_uneven_create")
+        self._h = h
+        hb, hc = _V(), _V()
+        _check(self.lib.tups_uneven_parts(self._h, C.byref(hb), C.byref(hc)), "tups_uneven_parts")
+        nibz = len(self.nl_m)
+        nf_q = [self.nf_w[q % self.nw] for q in range(nibz * self.nw)]
+        nl_q = [self.nl_m[q // self.nw] for q in range(nibz * self.nw)]
+        self.batch = TupsBatch(nx, nf_q, nl_q, beta, f_damping, _handle=hb)
+        self.ctx = TupsContext(nx, nf_max, max(self.nl_m), beta=beta, f_damping=f_damping, _handle=hc)
+
+    def iteration(self, graph: bool = True):
+        """BSE -> crossing (largest box, embedded and restricted) -> G = F - G."""
+        _check(self.lib.tups_uneven_iteration(self._h, 1 if graph else 0), "tups_uneven_iteration")
+
+    def sync(self):
+        _check(self.lib.tups_uneven_sync(self._h), "tups_uneven_sync")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.batch.close()
+            self.ctx.close()
+            self.lib.tups_uneven_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
