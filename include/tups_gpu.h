@@ -251,6 +251,19 @@ typedef struct tups_conv_info {
 } tups_conv_info;
 int tups_conv(tups_ctx* ctx, const double* sigma_old, const double* sigma, double f_sigma, double f_vert,
               tups_conv_info* info);
+/* tups_conv without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * errors as tups_conv; the block partials, the result and the staged host Sigma
+ * pair live in a fixed device array of this library (2.1 MiB, one copy per
+ * GPU, held until the process ends) instead of a workspace allocated at the
+ * first call.  Host Sigma arrays are limited to Nx Ny Nf <= 65536 elements
+ * (TUPS_EINVAL beyond; the device pair, NULL and NULL, has no limit).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (from two
+ * host threads, or on two contexts' streams) race and must be serialised by
+ * the caller.  Synchronous. */
+int tups_conv_noalloc(tups_ctx* ctx, const double* sigma_old, const double* sigma, double f_sigma, double f_vert,
+                      tups_conv_info* info);
 /* calc_sus (SRC/parquet_sus.f90:18-362) from the resident vertices and the
  * Sigma and mu on the device: sus_out receives sus_X_IBZ(q, ff) for X = d, m,
  * s, t, as [X][ff][q] complex (4 x 4 x Nq; ff the external form factors 1,

@@ -232,6 +232,10 @@ int tups_create(const tups_dims* d, tups_ctx** out) {
   }
 
   hipError_t he = hipGetDevice(&c->device);
+  if (he == hipSuccess && (rc = conv_noalloc_setup(c->device)) != TUPS_OK) {
+    tups_destroy(c);
+    return rc;
+  }
   if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   for (int i = 0; i < kEvents && he == hipSuccess; ++i) he = hipEventCreate(&c->ev[i]);
   for (int i = 0; i < 2 && he == hipSuccess; ++i) he = hipEventCreate(&c->tev[i]);

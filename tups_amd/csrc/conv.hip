@@ -14,6 +14,8 @@
 #include "internal.h"
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace tups {
 
@@ -151,26 +153,50 @@ __global__ __launch_bounds__(320) void conv_final(ConvArgs a) {
 
 }  // namespace
 
-}  // namespace tups
+// ---- scratch of tups_conv_noalloc --------------------------------------------------
+//
+// A fixed device array replaces the context's lazily allocated workspace: the
+// host Sigma pair (at most kConvNoallocMaxNt elements each), the block partials
+// and the result.  Each GPU holds its own copy; its address is taken once per
+// device at context creation and kept in conv_scratch_ptr.  Nothing in it must
+// start at zero: every call overwrites the partials and the result before
+// reading them.
+constexpr size_t kConvNoallocMaxNt = 65536;
+constexpr size_t kConvScratchDoubles = 4 * kConvNoallocMaxNt + 3 * kConvBlocksS + 3 * 4 * (size_t)kConvBlocksF + 8;
+static __device__ __align__(256) double scratch[kConvScratchDoubles];  // 2.1 MiB
 
-using namespace tups;
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex conv_scratch_mu;
+double* conv_scratch_ptr[kMaxDevices] = {};
+}  // namespace
 
-extern "C" int tups_conv(tups_ctx* c, const double* sigma_old, const double* sigma, double f_sigma, double f_vert,
-                         tups_conv_info* info) {
-  TUPS_CHECK(c && info, TUPS_EINVAL, "null argument");
-  TUPS_CHECK((sigma_old == nullptr) == (sigma == nullptr), TUPS_EINVAL,
-             "pass both Sigma_old and Sigma, or neither (the device pair of the last iteration)");
+// Called with `device` current (tups_create): the device address of this GPU's
+// copy of the scratch array.
+int conv_noalloc_setup(int device) {
+  TUPS_CHECK(device >= 0 && device < kMaxDevices, TUPS_EINVAL, "device index beyond the no-allocation table");
+  std::lock_guard<std::mutex> lock(conv_scratch_mu);
+  if (conv_scratch_ptr[device]) return TUPS_OK;
+  void* p = nullptr;
+  TUPS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  conv_scratch_ptr[device] = static_cast<double*>(p);
+  return TUPS_OK;
+}
+
+namespace {
+
+// The check over a workspace w: partials, result, then the host Sigma pair.
+int conv_run(tups_ctx* c, double* w, const double* sigma_old, const double* sigma, double f_sigma, double f_vert,
+             tups_conv_info* info) {
   const tups_sizes& s = c->sz;
   const size_t nt = (size_t)s.nx * s.ny * s.nf;
-  const size_t ws = (2 * nt * 2 + 3 * kConvBlocksS + 3 * 4 * (size_t)kConvBlocksF + 8) * sizeof(double);
-  if (!c->conv_ws) TUPS_HIP(hipMalloc(&c->conv_ws, ws));
   ConvArgs a{};
-  double* w = static_cast<double*>(c->conv_ws);
-  double2* h_old = reinterpret_cast<double2*>(w);
-  double2* h_cur = h_old + nt;
-  a.s_part = w + 4 * nt;
+  // partials and result first, so the device pair never reaches past them
+  a.s_part = w;
   a.f_part = a.s_part + 3 * kConvBlocksS;
   a.out = a.f_part + 3 * 4 * (size_t)kConvBlocksF;
+  double2* h_old = reinterpret_cast<double2*>(a.out + 8);
+  double2* h_cur = h_old + nt;
   if (sigma) {
     TUPS_HIP(hipMemcpyAsync(h_old, sigma_old, nt * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     TUPS_HIP(hipMemcpyAsync(h_cur, sigma, nt * sizeof(double2), hipMemcpyHostToDevice, c->stream));
@@ -211,4 +237,41 @@ extern "C" int tups_conv(tups_ctx* c, const double* sigma_old, const double* sig
   }
   info->converged = ok ? 1 : 0;
   return TUPS_OK;
+}
+
+int conv_args_ok(tups_ctx* c, const double* sigma_old, const double* sigma, tups_conv_info* info) {
+  TUPS_CHECK(c && info, TUPS_EINVAL, "null argument");
+  TUPS_CHECK((sigma_old == nullptr) == (sigma == nullptr), TUPS_EINVAL,
+             "pass both Sigma_old and Sigma, or neither (the device pair of the last iteration)");
+  return TUPS_OK;
+}
+
+}  // namespace
+
+}  // namespace tups
+
+using namespace tups;
+
+extern "C" int tups_conv(tups_ctx* c, const double* sigma_old, const double* sigma, double f_sigma, double f_vert,
+                         tups_conv_info* info) {
+  if (int rc = conv_args_ok(c, sigma_old, sigma, info)) return rc;
+  const size_t nt = (size_t)c->sz.nx * c->sz.ny * c->sz.nf;
+  const size_t ws = (2 * nt * 2 + 3 * kConvBlocksS + 3 * 4 * (size_t)kConvBlocksF + 8) * sizeof(double);
+  if (!c->conv_ws) TUPS_HIP(hipMalloc(&c->conv_ws, ws));
+  return conv_run(c, static_cast<double*>(c->conv_ws), sigma_old, sigma, f_sigma, f_vert, info);
+}
+
+extern "C" int tups_conv_noalloc(tups_ctx* c, const double* sigma_old, const double* sigma, double f_sigma,
+                                 double f_vert, tups_conv_info* info) {
+  if (int rc = conv_args_ok(c, sigma_old, sigma, info)) return rc;
+  const size_t nt = (size_t)c->sz.nx * c->sz.ny * c->sz.nf;
+  if (sigma && nt > kConvNoallocMaxNt) {
+    set_error("tups_conv_noalloc: the Sigma pair has " + std::to_string(nt) +
+              " elements (Nx Ny Nf), beyond the fixed scratch of " + std::to_string(kConvNoallocMaxNt) +
+              "; use tups_conv or pass the device pair (NULL, NULL)");
+    return TUPS_EINVAL;
+  }
+  TUPS_CHECK(c->device >= 0 && c->device < kMaxDevices && conv_scratch_ptr[c->device], TUPS_ESTATE,
+             "tups_conv_noalloc: no scratch address for this device (context not created by tups_create)");
+  return conv_run(c, conv_scratch_ptr[c->device], sigma_old, sigma, f_sigma, f_vert, info);
 }

@@ -238,6 +238,8 @@ int launch_check_finite(tups_ctx* c, const double2* p, size_t n);
 int upload_tables(tups_ctx* c);
 // bracket one kernel launch when per-kernel timing is on
 int timer_begin(tups_ctx* c);
+// device address of the scratch array of tups_conv_noalloc, once per device (conv.hip)
+int conv_noalloc_setup(int device);
 int timer_end(tups_ctx* c, int kernel);
 // one launch of the batching form's ladder, variant v, into the spare buffers (bse.hip)
 int batch_launch(tups_batch* b, int v);

@@ -344,6 +344,15 @@ module tups_gpu_iface
       integer(c_int) :: rc
     end function tups_conv
 
+    ! tups_conv with its partials in the library's fixed device array (no allocation)
+    function tups_conv_noalloc(ctx, sigma_old, sigma, f_sigma, f_vert, info) &
+        bind(C, name="tups_conv_noalloc") result(rc)
+      import :: c_ptr, c_int, c_double, tups_conv_info
+      type(c_ptr), value :: ctx, sigma_old, sigma
+      real(c_double), value :: f_sigma, f_vert
+      type(tups_conv_info), intent(out) :: info
+      integer(c_int) :: rc
+    end function tups_conv_noalloc
 
     ! calc_sus (SRC/parquet_sus.f90:18): sus(Nq, 4, 4) = sus_X_IBZ(q, ff) for X = d, m, s, t
     function tups_calc_sus(ctx, u, f_range, after_update, sus) bind(C, name="tups_calc_sus") result(rc)

@@ -94,6 +94,7 @@ SIGNATURES = {
     "tups_get_mu": ([_V, _D, C.POINTER(C.c_int32)], C.c_int),
     "tups_chi0": ([_V, _D, _D, _D, _D], C.c_int),
     "tups_conv": ([_V, _D, _D, C.c_double, C.c_double, _P(ConvInfo)], C.c_int),
+    "tups_conv_noalloc": ([_V, _D, _D, C.c_double, C.c_double, _P(ConvInfo)], C.c_int),
     "tups_read_prms": ([C.c_char_p, _P(Prms)], C.c_int),
     "tups_calc_sus": ([_V, C.c_double, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_prms_dims": ([_P(Prms), _P(Dims)], C.c_int),
@@ -419,18 +420,21 @@ class TupsContext:
                                      C.byref(steps), _dptr(grt) if grt is not None else None), "tups_gkw_grt")
         return m.value, steps.value, grt
 
-    def conv(self, f_sigma: float = 1e-8, f_vert: float = 1e-8, sigma_old=None, sigma=None) -> dict:
+    def conv(self, f_sigma: float = 1e-8, f_vert: float = 1e-8, sigma_old=None, sigma=None,
+             noalloc: bool = False) -> dict:
         """conv() (SRC/parquet_check.f90:39-150) on the device: the Sigma pair from the host, or
         (both None) the pair the last iteration_full left on the device; F from the resident
-        vertices; the previous F maxima are the context's."""
+        vertices; the previous F maxima are the context's.  noalloc calls tups_conv_noalloc
+        (the library's fixed scratch array instead of a workspace allocated at the first call)."""
         so = sn = None
         if sigma is not None:
             so = np.ascontiguousarray(np.asarray(sigma_old, dtype=np.complex128).ravel())
             sn = np.ascontiguousarray(np.asarray(sigma, dtype=np.complex128).ravel())
         info = ConvInfo()
-        _check(self.lib.tups_conv(self._h, _dptr(so) if so is not None else None,
-                                  _dptr(sn) if sn is not None else None, float(f_sigma), float(f_vert),
-                                  C.byref(info)), "tups_conv")
+        name = "tups_conv_noalloc" if noalloc else "tups_conv"
+        _check(getattr(self.lib, name)(self._h, _dptr(so) if so is not None else None,
+                                       _dptr(sn) if sn is not None else None, float(f_sigma), float(f_vert),
+                                       C.byref(info)), name)
         return {"sigma_avg": info.sigma_avg, "sigma_max": info.sigma_max, "kx": info.kx, "ky": info.ky,
                 "nu": info.nu, "converged": bool(info.converged), "f_max": list(info.f_max),
                 "f_rel_err": list(info.f_rel_err)}
