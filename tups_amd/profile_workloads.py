@@ -6,6 +6,8 @@
     python -m tups_amd.profile_workloads phir     --steps 5   # PhiR, Nl=1, Nx=16, Nf=64
     python -m tups_amd.profile_workloads calib    --steps 3   # G = F - G streaming kernel
     python -m tups_amd.profile_workloads batch    --steps 3   # uneven q batch, Nx=8, D 16..576
+    python -m tups_amd.profile_workloads uneven   --steps 3   # uneven-box iteration with the crossing
+    python -m tups_amd.profile_workloads closed_ir --steps 1  # FP32 LU + FP64 refinement, D=320
 
 `calib` runs a streaming kernel whose bytes are known exactly (per element:
 read 2 x 16 B, write 16 B) so hardware counters (FETCH_SIZE, WRITE_SIZE) can be
@@ -148,7 +150,7 @@ def batch_flops(b: TupsBatch) -> float:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("workload", choices=["bse", "split", "crossing", "phir", "calib", "batch", "gtau", "config1",
-                                         "conv", "sus"])
+                                         "conv", "sus", "closed_ir", "uneven"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--nx", type=int, default=16)
     ap.add_argument("--nf", type=int, default=64)
@@ -212,6 +214,17 @@ def main(argv=None):
         c.close()
         sys.stdout.flush()
         return
+    if args.workload == "uneven":
+        # uneven-box iteration (Nx=8, Nf_w 64 -> 16, Nl_m 1/5/9), graph replayed
+        u = uneven_context()
+        name, _ = u.batch.autotune()
+        for _ in range(args.steps):
+            u.iteration(graph=True)
+        u.sync()
+        print(f"uneven: batch variant {name}, {args.steps} graph-replayed iterations, Nq={u.batch.nq}")
+        u.close()
+        sys.stdout.flush()
+        return
     if args.workload == "batch":
         b = batch_context()
         if args.variant:
@@ -228,7 +241,7 @@ def main(argv=None):
         b.close()
         sys.stdout.flush()
         return
-    if args.workload in ("bse", "split", "calib"):
+    if args.workload in ("bse", "split", "calib", "closed_ir"):
         c = TupsContext(nx=8, nf=64, nl=5, nq=1024, beta=2.0, f_damping=0.5)
         batched_bse_inputs(c)
         if args.workload == "bse":
@@ -236,6 +249,11 @@ def main(argv=None):
             for _ in range(args.steps):
                 c.BSE(0)
             print(f"bse: variant {name}, {args.steps} launches, flops/launch {16.0 * c.nz ** 3 * 4 * c.nq:.4e}")
+        elif args.workload == "closed_ir":
+            from .runtime import BSE_CLOSED_IR
+            for _ in range(args.steps):
+                c.BSE(0, mode=BSE_CLOSED_IR)
+            print(f"closed_ir: {args.steps} calls, refinement (steps, residual) {c.solve_info()}")
         elif args.workload == "split":
             c.set_split(args.moduli)
             for _ in range(args.steps):
