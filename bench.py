@@ -409,7 +409,8 @@ def main():
     refined = {"value": round(nq / (ir_step_ms / 1e3), 2), "unit": "q-points/s", "ms_per_call": round(ir_step_ms, 3),
                "refinement_steps": ir_steps, "residual": ir_res, "residual_bar": 1e-12,
                "vs_direct_fp64": round(cl_step_ms / ir_step_ms, 3),
-               "kernel": "ir_build32 + closed_solve<float> (FP32 MFMA) + ir_gemm (FP32 / FP64 MFMA) + ir_chi + ir_check",
+               "kernel": "ir_build32 + closed_solve<float> (FP32 MFMA) + ir_gemm (FP32 / FP64 MFMA) + ir_chi "
+                         "+ ir_check",
                "flops_per_call": {"fp32": (12.0 + 8.0 * (1 + ir_steps)) * D ** 3 * 4 * nq,
                                   "fp64": 8.0 * (ir_steps + 1) * D ** 3 * 4 * nq}}
     print(f"[bench] refined closed form: {ir_step_ms:.1f} ms per call, {ir_steps} steps, residual {ir_res:.1e}",

@@ -319,9 +319,11 @@ int tups_batch_set_product_forms(tups_batch* batch, int32_t allow_three);
  * there through tups_batch_upload) and a context for the largest box (Nf =
  * nf_max, Nl = max nl_m, full IBZ; upload LAMBDA there; its M tensors are
  * built at creation), both on one stream.  tups_uneven_parts hands them out
- * (owned by the iteration: do not destroy them).  tups_uneven_iteration runs
- * BSE (one persistent launch, SRC/parquet_BSE.f90:62-132 per q) -> crossing ->
- * G = F - G (SRC/loop.f90:219-222) with the crossing defined as the reference's
+ * (owned by the iteration: do not destroy them; the context's crossing forms
+ * only the rows (l1, l2, Q) with l1, l2 < Nl of Q, the ones the boxes keep).
+ * tups_uneven_iteration runs BSE (one persistent launch,
+ * SRC/parquet_BSE.f90:62-132 per q) -> crossing -> G = F - G
+ * (SRC/loop.f90:219-222) with the crossing defined as the reference's
  * calc_PhiR + calculate_PAE on the largest box of the vertices embedded in it
  * (form factors beyond Nl_q zero, frequencies outside q's box clamped to its
  * edge, as pickNu does) and restricted to each q's box; on uniform boxes this

@@ -21,7 +21,8 @@
 //  3. Re C' = (P+ + P-)/2 and Im C' = (P+ - P-)/(2j) (mod m); the Chinese
 //     remainder theorem gives C' = sum_m u_m W_m (mod M), W_m = M/m ((M/m)^-1
 //     mod m), M = prod m; the factors 1/2 and 1/(2j) are folded into the
-//     weights, so only the residues of P+ + P- and P+ - P- are formed.  ta + tb <= log2 M - log2(8D) - 1 keeps |C'| < M/4,
+//     weights, so only the residues of P+ + P- and P+ - P- are formed.
+//     ta + tb <= log2 M - log2(8D) - 1 keeps |C'| < M/4,
 //     so C' is the representative nearest zero.  The sum is held as an exact
 //     high part (W rounded to 41 significant bits, a multiple of 2^L) plus a
 //     low part, so C' = (S_hi - q M_hi) + (S_lo - q M_lo), q = rint(S / M),
@@ -775,7 +776,7 @@ static int crt_big_tb(int D) {
 
 // tile variants: 0 = the 64 x 64 kernel, else crt_gemm_big<tb> (32 tb x 32 tb tiles)
 struct SplitVariant {
-  int tb;
+  int tb;  // 0: 64 x 64 workgroup tile; 5, 6: crt_gemm_big<tb>
   const char* name;
 };
 static const SplitVariant kSplitVariants[] = {{0, "crt_64x64"}, {5, "crt_160x160"}, {6, "crt_192x192"}};
@@ -790,7 +791,7 @@ static int split_tb(const tups_ctx* c) {
 }
 
 static size_t crt_padded(int D, int tb) {
-  const size_t L = tb ? std::lcm(32 * tb, 64) : 64;
+  const size_t L = tb > 0 ? std::lcm(32 * tb, 64) : 64;
   return ((size_t)D + L - 1) / L * L;
 }
 
@@ -877,7 +878,7 @@ int launch_bse_split(tups_ctx* c) {
   a.Kp = (int)Kp;
   a.nf = s.nf;
   a.nl = s.nl;
-  a.tiles_n = (int)(Dp / (tb ? 32 * tb : CRT_TILE));
+  a.tiles_n = (int)(Dp / (tb > 0 ? 32 * tb : CRT_TILE));
   a.alpha = s.alpha;
   const double cr[4] = {0.5, 0.5, -0.25, 0.25};
   for (int r = 0; r < 4; ++r) a.cf[r] = cr[r] * c->f_damping;

@@ -321,6 +321,7 @@ int tups_destroy(tups_ctx* c) {
   f(c->d_flag);
   f(c->pae_y);
   f(c->pae_ym);
+  f(c->pae_rows);
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   for (int i = 0; i < 2; ++i)

@@ -455,6 +455,7 @@ struct PaeMmArgs {
   const double2* phir[4];
   const int* signs;
   double2* Y;  // [kt = 4 pair + 2 routine + target][qw][j = l3 + Nl l4 + Nl^2 R][n = nu1 + Nf nu2]
+  const int* rows;  // GEMM row -> (l1, l2, Q) row index l1 + Nl l2 + Nl^2 m, or null for all rows in order
   int nf, nl, nw, nibz, nr, nz, kk, rr, tiles_r, tiles_n;
   double inv_nx, nc_aux;
 };
@@ -525,7 +526,8 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
   const int sr = tid & 63, sk = tid >> 6;
   const int gr = row0 + sr;
   const bool rok = gr < a.rr;
-  const int rl = rok ? gr % nl2 : 0, rm = rok ? gr / nl2 : 0;
+  const int grow = rok ? (a.rows ? a.rows[gr] : gr) : 0;
+  const int rl = grow % nl2, rm = grow / nl2;
   const int gn = col0 + sr;
   const bool nok = gn < (int)nn;
   double2 ra[4], rb[4];
@@ -596,8 +598,9 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
       const int nu1 = n % nf, nu2 = n / nf;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int r = row0 + wr + i * 16 + lk + 4 * e;
-        if (r >= a.rr) continue;
+        const int rg = row0 + wr + i * 16 + lk + 4 * e;
+        if (rg >= a.rr) continue;
+        const int r = a.rows ? a.rows[rg] : rg;
         const int l = r % nl2, m = r / nl2;
         const int l1 = l % nl, l2 = l / nl;
         const size_t q = (size_t)m * a.nw + w0;
@@ -634,7 +637,10 @@ static int pae_gemm_path(tups_ctx* c) {
   a.nr = s.nr;
   a.nz = s.nz;
   a.kk = s.nl * s.nl * s.nr;
-  a.rr = s.nl * s.nl * s.nibz;
+  // the rows the caller keeps (an uneven-box iteration drops the form factors
+  // beyond each momentum's Nl), else every (l1, l2, Q)
+  a.rows = c->pae_rows;
+  a.rr = c->pae_rows ? c->pae_nrows : s.nl * s.nl * s.nibz;
   a.tiles_r = (a.rr + 63) / 64;
   a.tiles_n = (s.nf * s.nf + 63) / 64;
   a.inv_nx = 1.0 / s.nx;

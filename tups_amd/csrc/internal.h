@@ -140,6 +140,9 @@ struct tups_ctx {
   int bse_variant = -1;          // kernel variant chosen by the autotuner (-1: default)
   bool three_mult = false;       // the autotuner may pick the three-multiplication product forms
   int pae_variant = -1;
+  int* pae_rows = nullptr;       // Nl > 1 crossing: the (l1, l2, Q) rows to form (null: all); F's other rows
+                                 // are not written (set only by the uneven-box iteration, which reads no other)
+  int pae_nrows = 0;
   int split_moduli = 12;         // split ladder: CRT moduli (8..16)
   int split_qbatch = 0;          // split ladder: q points per pass (0: default)
   int split_auto_q = 0;          // split ladder: the default, fixed at first use until set_split

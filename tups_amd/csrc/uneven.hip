@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 struct tups_uneven {
   tups_ctx* ctx = nullptr;     // the largest box, full IBZ
@@ -214,6 +215,20 @@ extern "C" int tups_uneven_create(int32_t nx, int32_t nf_max, const int32_t* nf_
     u->batch->stream = u->ctx->stream;
     u->batch->own_stream = false;
     rc = tups_build_m(u->ctx);
+  }
+  if (!rc && nl_max > 1) {
+    // the crossing GEMM forms only the rows (l1, l2, m) that the extraction keeps
+    std::vector<int> rows;
+    for (int m = 0; m < nibz; ++m)
+      for (int l2 = 0; l2 < nl_m[m]; ++l2)
+        for (int l1 = 0; l1 < nl_m[m]; ++l1) rows.push_back(l1 + nl_max * l2 + nl_max * nl_max * m);
+    if (hipMalloc(&u->ctx->pae_rows, rows.size() * sizeof(int)) != hipSuccess ||
+        hipMemcpy(u->ctx->pae_rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("uneven-box iteration: crossing row table upload failed");
+      rc = TUPS_EHIP;
+    } else {
+      u->ctx->pae_nrows = (int)rows.size();
+    }
   }
   if (rc) {
     tups_uneven_destroy(u);
