@@ -310,6 +310,73 @@ __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
   }
 }
 
+// The residues of one modulus (INT32 sums of the + and - products of a 32 x 32
+// wave tile, 2 x 2 blocks) join the CRT sums, held per element as exact high
+// and rounded low parts; the accumulators restart at zero.
+__device__ __forceinline__ void crt_join(const CrtArgs& a, int mod, v4i (&accp)[2][2], v4i (&accm)[2][2],
+                                         double (&shr)[16], double (&slr)[16], double (&shi)[16],
+                                         double (&sli)[16]) {
+  const double m = a.m[mod], im = a.inv_m[mod];
+  const double wrh = a.wrh[mod], wrl = a.wrl[mod], wih = a.wih[mod], wil = a.wil[mod];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int x = 8 * i + 4 * j + e;
+        const int p = accp[i][j][e], n = accm[i][j][e];
+        const double ur = sym_mod((double)(p + n), m, im);
+        const double ui = sym_mod((double)(p - n), m, im);
+        shr[x] = fma(ur, wrh, shr[x]);
+        slr[x] = fma(ur, wrl, slr[x]);
+        shi[x] = fma(ui, wih, shi[x]);
+        sli[x] = fma(ui, wil, sli[x]);
+      }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) accp[i][j] = accm[i][j] = v4i{0, 0, 0, 0};
+}
+
+// Epilogue of a 32 x 32 wave tile at (r0, c0) of matrix mat.
+// C' = (S_hi - q M_hi) + (S_lo - q M_lo); T = C' 2^(e_i - ta + f_j - tb);
+// Phi = c f T + (1 - f)(F - Gamma).  Element e of lane l of a 16 x 16 INT32
+// block sits at row 4 (l / 16) + e, column l % 16.
+__device__ __forceinline__ void crt_tile_epilogue(const CrtArgs& a, int mat, int r0, int c0, int li, int lk,
+                                                  const double (&shr)[16], const double (&slr)[16],
+                                                  const double (&shi)[16], const double (&sli)[16]) {
+  const int D = a.D;
+  const size_t DD = (size_t)D * D;
+  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  const double2* Gm = a.gam[r] + DD * q;
+  const double2* Fm = a.fv[r] + DD * q;
+  double2* O = a.out[r] + DD * q;
+  const double cf = a.cf[r];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int gj = c0 + j * 16 + li;
+      if (gj >= D) continue;
+      const int fj = a.fscale[(size_t)mat * a.Dp + gj];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int gi = r0 + i * 16 + 4 * lk + e;
+        if (gi >= D) continue;
+        const int x = 8 * i + 4 * j + e;
+        const double qr = rint((shr[x] + slr[x]) * a.inv_mm);
+        const double qi = rint((shi[x] + sli[x]) * a.inv_mm);
+        const double cr = fma(-qr, a.mhi, shr[x]) + fma(-qr, a.mlo, slr[x]);
+        const double ci = fma(-qi, a.mhi, shi[x]) + fma(-qi, a.mlo, sli[x]);
+        const double s = ldexp(cf, a.escale[(size_t)mat * a.Dp + gi] - a.ta + fj - a.tb);
+        const size_t idx = (size_t)gj * D + gi;
+        const double2 fv = Fm[idx], gv = Gm[idx];
+        O[idx] = make_double2(fma(s, cr, a.omf * (fv.x - gv.x)), fma(s, ci, a.omf * (fv.y - gv.y)));
+      }
+    }
+}
+
 // One workgroup per 64 x 64 tile of one (q, r); four waves, each 32 x 32 =
 // 2 x 2 blocks of v_mfma_i32_16x16x64_i8.  The loop runs over (modulus, K chunk)
 // steps through a ring of four LDS stages (A+, A-, B+, B- chunks of 64 rows x
@@ -384,28 +451,7 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
         accm[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(am[i], bm[j], accm[i][j], 0, 0, 0);
       }
     if (++kstep == nk) {
-      // residues of this modulus -> CRT sums
-      const double m = a.m[mod], im = a.inv_m[mod];
-      const double wrh = a.wrh[mod], wrl = a.wrl[mod], wih = a.wih[mod], wil = a.wil[mod];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int x = 8 * i + 4 * j + e;
-            const int p = accp[i][j][e], n = accm[i][j][e];
-            const double ur = sym_mod((double)(p + n), m, im);
-            const double ui = sym_mod((double)(p - n), m, im);
-            shr[x] = fma(ur, wrh, shr[x]);
-            slr[x] = fma(ur, wrl, slr[x]);
-            shi[x] = fma(ui, wih, shi[x]);
-            sli[x] = fma(ui, wil, sli[x]);
-          }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) accp[i][j] = accm[i][j] = v4i{0, 0, 0, 0};
+      crt_join(a, mod, accp, accm, shr, slr, shi, sli);
       kstep = 0;
       ++mod;
     }
@@ -431,38 +477,153 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
     __syncthreads();
   }
 
-  // C' = (S_hi - q M_hi) + (S_lo - q M_lo); T = C' 2^(e_i - ta + f_j - tb);
-  // Phi = c f T + (1 - f)(F - Gamma).  Element e of lane l of a 16 x 16 INT32
-  // block sits at row 4 (l / 16) + e, column l % 16.
-  const int D = a.D;
-  const size_t DD = (size_t)D * D;
-  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
-  const double2* Gm = a.gam[r] + DD * q;
-  const double2* Fm = a.fv[r] + DD * q;
-  double2* O = a.out[r] + DD * q;
-  const double cf = a.cf[r];
+  crt_tile_epilogue(a, mat, row0 + wr, col0 + wc, li, lk, shr, slr, shi, sli);
+}
+
+// ---- 64 x 64 tile, stages filled by direct global -> LDS loads ----------------------
+//
+// Same tile, waves, swizzle and CRT sums as crt_gemm; the (modulus, K chunk)
+// steps go through a ring of NS LDS stages (16 KiB each: A+, A-, B+, B- of 64
+// rows x 64 bytes) filled by global_load_lds_dwordx4, so no registers stage
+// the data and NS - 1 steps are in flight while one is computed.  The 16
+// bytes of lane l land at its wave's base (M0) + 16 l, i.e. at row tid / 4,
+// slot tid % 4 of the piece, so the lane loads the 16-byte column that the
+// XOR swizzle puts in that slot.  One barrier per step: each thread first
+// waits for its own loads of the step (vmcnt counts them in issue order), the
+// barrier then makes every wave's loads visible, and the stage refilled after
+// the barrier is the one computed in the step before, which every wave has
+// finished reading (its MFMAs consumed the fragments before the barrier).
+// The loads are issued from asm, so the compiler, which cannot tell the
+// ring's stages apart across loop iterations, adds no vmcnt(0) before the
+// fragment reads; its own waits for other loads only grow more conservative.
+typedef __attribute__((address_space(3))) int8_t lds_i8_t;
+
+// M0 carries the LDS base; nothing else in these kernels reads M0
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void dma16(const int8_t* g, unsigned lds_base) {
+  asm volatile(
+      "s_mov_b32 m0, %1\n\t"
+      "global_load_lds_dwordx4 %0, off"
+      :
+      : "v"(g), "s"(lds_base)
+      : "m0");
+}
+#pragma clang diagnostic pop
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 16, "vmcnt immediate");
+  __builtin_amdgcn_s_waitcnt(0x0F70 | N);  // vmcnt(N); expcnt, lgkmcnt not waited
+}
+
+// a workgroup barrier that orders the compiler's LDS accesses but adds no wait
+// for outstanding loads (a fence would wait for the LDS loads in flight)
+__device__ __forceinline__ void stage_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+typedef int8_t CrtStage[4][CRT_TILE][64];
+
+template <int NS>
+__global__ __launch_bounds__(256, 2) void crt_gemm_dma(CrtArgs a) {
+  static_assert(NS >= 3 && NS <= 5, "stages");
+  // one LDS object per stage, so the compiler sees which stage a read touches
+  __shared__ __attribute__((aligned(16))) CrtStage S0;
+  __shared__ __attribute__((aligned(16))) CrtStage S1;
+  __shared__ __attribute__((aligned(16))) CrtStage S2;
+  __shared__ __attribute__((aligned(16))) CrtStage S3;
+  __shared__ __attribute__((aligned(16))) CrtStage S4;
+  const int tiles = a.tiles_n * a.tiles_n;
+  const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int mat = bid / tiles;
+  const int t = bid - mat * tiles;
+  const int row0 = (t / a.tiles_n) * CRT_TILE, col0 = (t % a.tiles_n) * CRT_TILE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nk = a.Kp / 64;
+  const int nsteps = a.nmod * nk;
+  const int drow = tid >> 2;
+  const int dch = (tid & 3) ^ ((drow >> 2) & 3);  // global 16-byte column landing in slot tid & 3
+  auto issue = [&](int step, CrtStage& S) {
+    const int md = step / nk, k0 = (step - md * nk) * 64;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int which = v >> 1, sign = v & 1;
+      const int8_t* g = (which ? a.B : a.A) + ws_off(a, mat, md, sign, (which ? col0 : row0) + drow, k0 + 16 * dch);
+      const unsigned base = (unsigned)(uintptr_t)(lds_i8_t*)&S[v][wave * 16][0];
+      dma16(g, __builtin_amdgcn_readfirstlane(base));
+    }
+  };
+  double shr[16], slr[16], shi[16], sli[16];  // CRT sums, element (i, j, e) -> 8 i + 4 j + e
+#pragma unroll
+  for (int x = 0; x < 16; ++x) shr[x] = slr[x] = shi[x] = sli[x] = 0.0;
+  v4i accp[2][2], accm[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int gj = col0 + wc + j * 16 + li;
-      if (gj >= D) continue;
-      const int fj = a.fscale[(size_t)mat * a.Dp + gj];
+    for (int j = 0; j < 2; ++j) accp[i][j] = accm[i][j] = v4i{0, 0, 0, 0};
+  int kstep = 0, mod = 0;
+  auto compute = [&](const CrtStage& S) {
+    auto frag = [&](int v, int row) -> v4i {
+      return *reinterpret_cast<const v4i*>(&S[v][row][16 * (lk ^ ((row >> 2) & 3))]);
+    };
+    v4i ap[2], am[2], bp[2], bm[2];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int gi = row0 + wr + i * 16 + 4 * lk + e;
-        if (gi >= D) continue;
-        const int x = 8 * i + 4 * j + e;
-        const double qr = rint((shr[x] + slr[x]) * a.inv_mm);
-        const double qi = rint((shi[x] + sli[x]) * a.inv_mm);
-        const double cr = fma(-qr, a.mhi, shr[x]) + fma(-qr, a.mlo, slr[x]);
-        const double ci = fma(-qi, a.mhi, shi[x]) + fma(-qi, a.mlo, sli[x]);
-        const double s = ldexp(cf, a.escale[(size_t)mat * a.Dp + gi] - a.ta + fj - a.tb);
-        const size_t idx = (size_t)gj * D + gi;
-        const double2 fv = Fm[idx], gv = Gm[idx];
-        O[idx] = make_double2(fma(s, cr, a.omf * (fv.x - gv.x)), fma(s, ci, a.omf * (fv.y - gv.y)));
-      }
+    for (int i = 0; i < 2; ++i) {
+      ap[i] = frag(0, wr + i * 16 + li);
+      am[i] = frag(1, wr + i * 16 + li);
+      bp[i] = frag(2, wc + i * 16 + li);
+      bm[i] = frag(3, wc + i * 16 + li);
     }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        accp[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ap[i], bp[j], accp[i][j], 0, 0, 0);
+        accm[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(am[i], bm[j], accm[i][j], 0, 0, 0);
+      }
+    if (++kstep == nk) {
+      crt_join(a, mod, accp, accm, shr, slr, shi, sli);
+      kstep = 0;
+      ++mod;
+    }
+  };
+  auto stage = [&](auto ic) -> CrtStage& {
+    constexpr int i = decltype(ic)::value % NS;
+    if constexpr (i == 0) return S0;
+    else if constexpr (i == 1) return S1;
+    else if constexpr (i == 2) return S2;
+    else if constexpr (i == 3) return S3;
+    else return S4;
+  };
+  // one step s held in stage I (= s mod NS)
+  auto step = [&](auto ic, int s) {
+    constexpr int I = decltype(ic)::value;
+    if (s >= nsteps) return;
+    if (s + NS - 2 < nsteps)
+      wait_vmcnt<4 * (NS - 2)>();  // the 4 (NS - 2) loads after this step's may still fly
+    else
+      wait_vmcnt<0>();
+    stage_barrier();
+    if (s + NS - 1 < nsteps) issue(s + NS - 1, stage(std::integral_constant<int, I + NS - 1>{}));
+    compute(stage(ic));
+  };
+  issue(0, S0);
+  if (1 < nsteps) issue(1, S1);
+  if (NS > 3 && 2 < nsteps) issue(2, S2);
+  if (NS > 4 && 3 < nsteps) issue(3, S3);
+  for (int s0 = 0; s0 < nsteps; s0 += NS) {
+    step(std::integral_constant<int, 0>{}, s0);
+    step(std::integral_constant<int, 1>{}, s0 + 1);
+    step(std::integral_constant<int, 2>{}, s0 + 2);
+    if constexpr (NS > 3) step(std::integral_constant<int, 3>{}, s0 + 3);
+    if constexpr (NS > 4) step(std::integral_constant<int, 4>{}, s0 + 4);
+  }
+  crt_tile_epilogue(a, mat, row0 + wr, col0 + wc, li, lk, shr, slr, shi, sli);
 }
 
 // ---- large-tile CRT GEMM --------------------------------------------------------------
@@ -777,9 +938,12 @@ static int crt_big_tb(int D) {
 // tile variants: 0 = the 64 x 64 kernel, else crt_gemm_big<tb> (32 tb x 32 tb tiles)
 struct SplitVariant {
   int tb;  // 0: 64 x 64 workgroup tile; 5, 6: crt_gemm_big<tb>
+  int ns;  // 64 x 64 tile: 0 register-staged (crt_gemm), else crt_gemm_dma<ns>
   const char* name;
 };
-static const SplitVariant kSplitVariants[] = {{0, "crt_64x64"}, {5, "crt_160x160"}, {6, "crt_192x192"}};
+static const SplitVariant kSplitVariants[] = {{0, 0, "crt_64x64"},     {5, 0, "crt_160x160"},
+                                              {6, 0, "crt_192x192"},   {0, 4, "crt_64x64_dma4"},
+                                              {0, 5, "crt_64x64_dma5"}};
 constexpr int kNumSplitVariants = sizeof(kSplitVariants) / sizeof(kSplitVariants[0]);
 
 int split_variant_count() { return kNumSplitVariants; }
@@ -788,6 +952,11 @@ const char* split_variant_name(int v) { return v >= 0 && v < kNumSplitVariants ?
 static int split_tb(const tups_ctx* c) {
   const int v = c->split_variant;
   return v >= 0 && v < kNumSplitVariants ? kSplitVariants[v].tb : crt_big_tb(c->sz.nz);
+}
+
+static int split_ns(const tups_ctx* c) {
+  const int v = c->split_variant;
+  return v >= 0 && v < kNumSplitVariants ? kSplitVariants[v].ns : 0;
 }
 
 static size_t crt_padded(int D, int tb) {
@@ -911,7 +1080,12 @@ int launch_bse_split(tups_ctx* c) {
     const long long nblk = (long long)a.tiles_n * a.tiles_n * mats;
     TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "split grid too large");
     if (int rc = timer_begin(c)) return rc;
-    void (*gemm)(CrtArgs) = tb == 6 ? crt_gemm_big<6> : (tb == 5 ? crt_gemm_big<5> : crt_gemm);
+    const int ns = split_ns(c);
+    void (*gemm)(CrtArgs) = tb == 6   ? crt_gemm_big<6>
+                            : tb == 5 ? crt_gemm_big<5>
+                            : ns == 4 ? crt_gemm_dma<4>
+                            : ns == 5 ? crt_gemm_dma<5>
+                                      : crt_gemm;
     hipLaunchKernelGGL(gemm, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_SPLIT_GEMM)) return rc;

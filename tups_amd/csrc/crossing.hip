@@ -460,12 +460,18 @@ struct PaeMmArgs {
   double inv_nx, nc_aux;
 };
 
-// Y for all (kt, qw, j, n): one thread per n for a fixed (j, qw); grid ((Nf^2 + 255) / 256, kk, nw)
+// Y for all (kt, qw, j, n): one thread per n for a fixed (j, qw).  The gathers of
+// neighbouring n at neighbouring qw share cache lines of PhiR (the frequency
+// maps shift nu1 against qw), so qw varies fastest over the workgroups of one
+// XCD.  grid: ((Nf^2 + 255) / 256) * kk * nw
 __global__ __launch_bounds__(256) void pae_ybuild(PaeMmArgs a) {
   const int nf = a.nf, nl = a.nl, nw = a.nw, nz = a.nz;
-  const int n = blockIdx.x * 256 + threadIdx.x;
+  const int nbn = (nf * nf + 255) / 256;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int w0 = logical % nw, rest = logical / nw;
+  const int j = rest / nbn, qw = w0 + 1;
+  const int n = (rest - j * nbn) * 256 + threadIdx.x;
   if (n >= nf * nf) return;
-  const int j = blockIdx.y, w0 = blockIdx.z, qw = w0 + 1;
   const int l3 = j % nl, l4 = (j / nl) % nl, R = j / (nl * nl);
   const int nu1 = n % nf + 1, nu2 = n / nf + 1;
   const double sgn = (double)(a.signs[l3] * a.signs[l4]);
@@ -506,7 +512,7 @@ __global__ __launch_bounds__(256) void pae_ybuild(PaeMmArgs a) {
 // One workgroup per 64 x 64 output tile of one (qw, pair, target); four waves of
 // 2 x 2 blocks of v_mfma_f64_16x16x4f64 (four real products per complex block);
 // the inner dimension walks 16 at a time with the next chunk prefetched into
-// registers.  grid: tiles_r * tiles_n * nw * 4 (remapped per XCD).
+// registers.  grid: tiles_r * tiles_n * nw * 4 (remapped per XCD, row tile fastest).
 constexpr int MMK = 16;
 __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
   __shared__ double As[2][MMK][64];
@@ -516,7 +522,9 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
   const int tile = logical % tiles;
   const int z = logical / tiles;  // = (w0 * 2 + pair) * 2 + target
   const int t = z & 1, p = (z >> 1) & 1, w0 = z >> 2;
-  const int row0 = (tile / a.tiles_n) * 64, col0 = (tile % a.tiles_n) * 64;
+  // row tiles vary fastest: the tiles_r workgroups that share one column tile of
+  // Y run together on one XCD, so Y comes from HBM once and the re-reads hit L2
+  const int row0 = (tile % a.tiles_r) * 64, col0 = (tile / a.tiles_r) * 64;
   const int nl2 = a.nl * a.nl, kk = a.kk, ktot = 2 * kk;
   const size_t nn = (size_t)a.nf * a.nf;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -647,8 +655,8 @@ static int pae_gemm_path(tups_ctx* c) {
   const double f0 = c->tab.ff[0];
   a.nc_aux = std::fabs(1.0 / f0 * 1.0 / f0);
   if (int rc = timer_begin(c)) return rc;
-  hipLaunchKernelGGL(pae_ybuild, dim3((unsigned)((s.nf * s.nf + 255) / 256), (unsigned)a.kk, (unsigned)s.nw),
-                     dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(pae_ybuild, dim3((unsigned)((s.nf * s.nf + 255) / 256 * a.kk * s.nw)), dim3(256), 0,
+                     c->stream, a);
   TUPS_HIP(hipGetLastError());
   if (int rc = timer_end(c, TUPS_K_PAE_GATHER)) return rc;
   const long long nblk = (long long)a.tiles_r * a.tiles_n * s.nw * 4;
