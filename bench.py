@@ -292,6 +292,90 @@ def phir_leg(c):
             "workload": f"calc_PhiR Nx={c.nx} Nf={c.nf} Nl={c.nl} NR={c.nr}, Nq={c.nq}, 4 channels"}
 
 
+def dropin_leg():
+    """One parquet iteration through the Fortran drop-in's transfers (tups_dropin.f90),
+    emulated from Python on the same C entry points at Nx=8, Nf=64, Nl=5 (all 480
+    IBZ q, D=320): per q a dense bubble pair packed and uploaded (BSE's calc_chi),
+    G_r_LL and F_r_LL uploaded only when not resident, BSE -> download Phi,
+    calc_PhiR -> download PhiR, calculate_PAE -> download F, G = F - G on the
+    device -> download G; the host mirrors are pageable arrays filled in place,
+    as the Fortran module arrays are.  Wall clock per iteration with the vertices
+    resident and with them re-uploaded, beside the device time of the same
+    kernels without transfers."""
+    import numpy as np
+    from tups_amd import GAMMA, LAMBDA, PHIR, TupsContext
+    from tups_amd import F as FV
+
+    c = TupsContext(nx=8, nf=64, nl=5, beta=2.0, f_damping=0.5)
+    nz, nq = c.nz, c.nq
+    for r in range(4):
+        c.synth_vertex(GAMMA, r, 60 + r, 2.0, 0.3 / np.sqrt(nz))
+        c.synth_vertex(FV, r, 70 + r, 2.0, 0.3 / np.sqrt(nz))
+        c.upload(LAMBDA, r, np.full((c.nf, c.nf, c.nw), (2.0, -2.0, 4.0, 0.0)[r], dtype=np.complex128))
+    g_host = [c.download(GAMMA, r) for r in range(4)]
+    f_host = [c.download(FV, r) for r in range(4)]
+    phir_host = [np.zeros(c.shape(PHIR), dtype=np.complex128, order="F") for _ in range(4)]
+    rng = np.random.default_rng(11)
+    chi_dense = np.zeros((nz, nz, 1), dtype=np.complex128, order="F")
+    blocks = 0.1 * (rng.standard_normal((c.nl, c.nl, c.nf)) + 1j * rng.standard_normal((c.nl, c.nl, c.nf)))
+    for nu in range(c.nf):
+        idx = np.arange(c.nl) * c.nf + nu
+        chi_dense[np.ix_(idx, idx, [0])] = blocks[:, :, nu][:, :, None]
+    c.calc_M()
+    vbytes = 16.0 * nz * nz * nq
+
+    def iteration(resident):
+        up = down = 0.0
+        c.sync()
+        t0 = time.perf_counter()
+        for q in range(nq):
+            c.upload_chi_dense(0, chi_dense, q)
+            c.upload_chi_dense(1, chi_dense, q)
+        up += 2 * 16.0 * c.nl * c.nl * c.nf * nq
+        if not resident:
+            for r in range(4):
+                c.upload(GAMMA, r, g_host[r])
+                c.upload(FV, r, f_host[r])
+            up += 8 * vbytes
+        c.BSE(0)
+        for r in range(4):
+            c.download(GAMMA, r, out=g_host[r])
+        c.calc_PhiR(0)
+        for r in range(4):
+            c.download(PHIR, r, out=phir_host[r])
+        c.calculate_PAE()
+        for r in range(4):
+            c.download(FV, r, out=f_host[r])
+        c.gamma_update()
+        for r in range(4):
+            c.download(GAMMA, r, out=g_host[r])
+        c.sync()
+        down += 12 * vbytes + 4 * 16.0 * phir_host[0].size
+        return (time.perf_counter() - t0) * 1e3, up, down
+
+    iteration(False)
+    t_res, up_res, down_res = min(iteration(True) for _ in range(2))
+    t_re, up_re, down_re = iteration(False)
+    c.sync()
+    c.record(0)
+    c.BSE(0)
+    c.calc_PhiR(0)
+    c.calculate_PAE()
+    c.gamma_update()
+    c.record(1)
+    c.sync()
+    gpu_ms = c.elapsed_ms(0, 1)
+    c.close()
+    print(f"[bench] drop-in iteration: {t_res:.0f} ms resident, {t_re:.0f} ms re-uploading, device {gpu_ms:.1f} ms",
+          file=sys.stderr, flush=True)
+    return {"ms_resident": round(t_res, 1), "ms_reupload": round(t_re, 1), "gpu_kernels_ms": round(gpu_ms, 2),
+            "pcie_bytes_resident": {"host_to_device": up_res, "device_to_host": down_res},
+            "pcie_bytes_reupload": {"host_to_device": up_re, "device_to_host": down_re},
+            "effective_gbs_resident": round((up_res + down_res) / (t_res / 1e3) / 1e9, 1),
+            "workload": f"drop-in iteration Nx=8 Nf=64 Nl=5, Nq={nq} (all IBZ q), D={nz}: BSE (FP64), calc_PhiR, "
+                        "calculate_PAE, G = F - G with the transfers of tups_dropin.f90 from pageable host arrays"}
+
+
 def main():
     import numpy as np
 
@@ -643,6 +727,7 @@ def main():
     prms_run = reference_prms_run()
 
     large = None if args.no_large else large_share(args.moduli)
+    dropin = dropin_leg()
 
     cpu = None
     if not args.no_cpu:
@@ -678,6 +763,7 @@ def main():
         "closed_form": closed,
         "closed_form_refined": refined,
         "uneven_iteration": uneven,
+        "dropin_iteration": dropin,
         "roofline": roofline,
         "crossing_roofline": crossing,
         "crossing_nl5": crossing5,

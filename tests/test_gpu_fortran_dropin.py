@@ -2,7 +2,9 @@
 
 tups_fortran_driver runs BSE -> calculate_expSqR/expqSR -> calc_PhiR ->
 calc_M_* -> calculate_PAE -> G = F - G (SRC/loop.f90:78-139, 219-222) through
-ISO_C_BINDING; every array it hands back is compared with the oracle.
+ISO_C_BINDING, then BSE on the resident vertices (the drop-in skips their
+upload) and BSE after the host halves F and declares it; every array it hands
+back is compared with the oracle.
 """
 import subprocess
 import sys
@@ -42,7 +44,7 @@ def test_fortran_dropin_iteration(tmp_path, nx, nf, nl, f):
     assert r.returncode == 0, r.stdout + r.stderr
     out = np.fromfile(fout, dtype=np.complex128)
     sizes = [lat.nz ** 2 * lat.nq] * 4 + [lat.nz ** 2 * lat.nw * lat.nr] * 4 \
-        + [lat.nl ** 4 * lat.nq * lat.nr] * 4 + [lat.nz ** 2 * lat.nq] * 8
+        + [lat.nl ** 4 * lat.nq * lat.nr] * 4 + [lat.nz ** 2 * lat.nq] * 16
     assert out.size == sum(sizes)
     parts, o = [], 0
     for s in sizes:
@@ -54,11 +56,16 @@ def test_fortran_dropin_iteration(tmp_path, nx, nf, nl, f):
     mt = P.calc_m_tensors(lat)
     f_ref = P.calculate_pae(lat, phi, lam, phir, mt)
     g_ref = B.gamma_update(f_ref, phi)
+    # second BSE on the resident G = F - Phi and F; third with the host's F / 2
+    phi2 = B.bse_ladder(lat, g_ref, f_ref, chi, beta, f)
+    phi3 = B.bse_ladder(lat, phi2, [0.5 * x for x in f_ref], chi, beta, f)
     for r_ in range(4):
         assert rel_fro(parts[r_], phi[r_].ravel(order="F")) < 1e-12
         assert rel_fro(parts[4 + r_], phir[r_].ravel(order="F")) < 1e-12
         assert rel_fro(parts[12 + r_], f_ref[r_].ravel(order="F")) < 1e-12
         assert rel_fro(parts[16 + r_], g_ref[r_].ravel(order="F")) < 1e-12
+        assert rel_fro(parts[20 + r_], phi2[r_].ravel(order="F")) < 1e-12
+        assert rel_fro(parts[24 + r_], phi3[r_].ravel(order="F")) < 1e-12
     # M in the reference's per-q layout M(l1,l2,l3,l4,q,R)
     for i, name in enumerate(P.TARGETS):
         per_q = mt[name][:, :, :, :, np.arange(lat.nq) // lat.nw, :]

@@ -57,7 +57,8 @@ struct CrtArgs {
   int8_t* A;                    // [mat][mod][sign][Kp / 64][Dp rows][64]: 64-byte K chunks
   int8_t* B;                    // [mat][mod][sign][Kp / 64][Dp cols][64]
   int* escale;                  // [mat][Dp] row exponents of A
-  double* rmax;                 // [mat][8][Dp] partial row maxima of A
+  double* rmax;                 // [mat][Dp] row maxima of A
+  double* rmaxp;                // [mat][Dp / 16][Dp] partial row maxima of A per 16-column block
   int* fscale;                  // [mat][Dp] column exponents of B
   int* crt;                     // large-tile GEMM: per-modulus (Re, Im) residue bytes of each tile
   int D, Dp, Kp, nf, nl;
@@ -86,6 +87,13 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int nblk) {
   const int per = nblk >> 3, rem = nblk & 7;
   return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
+
+// LDS slot of the 16-byte column c of a 64-byte row: c ^ crt_swz(row).  A
+// fragment read has lanes on 16 consecutive rows at one column; with this XOR
+// any 8 consecutive rows touch 32 distinct 4-byte banks and any 16 consecutive
+// rows 64, so the 16-byte reads are free of bank conflicts whether the LDS
+// serves them 8 or 16 lanes at a time.
+__device__ __forceinline__ int crt_swz(int row) { return ((row >> 1) & 3) ^ ((row >> 3) & 1); }
 
 // symmetric residue of an integer-valued y (|y| < 2^50): exact, |r| <= (m - 1) / 2
 __device__ __forceinline__ double sym_mod(double y, double m, double inv_m) {
@@ -123,36 +131,19 @@ __device__ __forceinline__ void gauss_residues(const CrtArgs& a, int mod, double
   rm = (int)sym_mod(ym, m, im) & 0xff;
 }
 
-// A = [Gamma | F] rows: partial row maxima over 8 blocks of the inner
-// dimension (lane = row; the four waves split a block).
-// grid: ((Dp / 64) * 8, 4 * nqb); out: rmax[mat][8][Dp]
-constexpr int CRT_KB = 8;
-__global__ __launch_bounds__(256) void crt_rowmax_a(CrtArgs a) {
-  __shared__ double red[4][64];
-  const int mat = blockIdx.y;  // r * nqb + qb
-  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
-  const int D = a.D;
-  const size_t DD = (size_t)D * D;
-  const double2* G = a.gam[r] + DD * q;
-  const double2* F = a.fv[r] + DD * q;
-  const int rb = blockIdx.x / CRT_KB, kb = blockIdx.x - rb * CRT_KB;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int row = rb * 64 + lane;
-  const int kw = (2 * D + 4 * CRT_KB - 1) / (4 * CRT_KB);  // inner indices per wave
-  const int k0 = (kb * 4 + wave) * kw, k1 = min(2 * D, k0 + kw);
+// Row maxima of A = [Gamma | F] from the partial maxima crt_prep_b leaves per
+// 16-column block (it reads every element of Gamma and F for B).  grid:
+// ((Dp + 255) / 256, 4 * nqb); one thread per row.
+constexpr int PREP_B_COLS = 16;  // columns per crt_prep_b workgroup
+__global__ __launch_bounds__(256) void crt_rowmax_fold(CrtArgs a) {
+  const int mat = blockIdx.y;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= a.Dp) return;
+  const int nb = a.Dp / PREP_B_COLS;
+  const double* p = a.rmaxp + (size_t)mat * nb * a.Dp + row;
   double mx = 0.0;
-  if (row < D) {
-#pragma unroll 4
-    for (int k = k0; k < k1; ++k) {
-      const double2 v = ld_stream(k < D ? G + (size_t)k * D + row : F + (size_t)(k - D) * D + row);
-      mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
-    }
-  }
-  red[wave][lane] = mx;
-  __syncthreads();
-  if (wave == 0)
-    a.rmax[((size_t)mat * CRT_KB + kb) * a.Dp + row] =
-        fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+  for (int b = 0; b < nb; ++b) mx = fmax(mx, p[(size_t)b * a.Dp]);
+  a.rmax[(size_t)mat * a.Dp + row] = mx;
 }
 
 // A = [Gamma | F] residues: one workgroup per 64 rows x 64 inner indices; lane
@@ -175,9 +166,7 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
   const int row = rb * 64 + lane;
   const bool rok = row < D;
   const int k0 = kc * 64 + wave * 16;
-  double mx = 0.0;
-#pragma unroll
-  for (int kb = 0; kb < CRT_KB; ++kb) mx = fmax(mx, a.rmax[((size_t)mat * CRT_KB + kb) * a.Dp + row]);
+  const double mx = a.rmax[(size_t)mat * a.Dp + row];
   int e = 0;
   if (mx > 0.0) frexp(mx, &e);
   if (kc == 0 && wave == 0) a.escale[(size_t)mat * a.Dp + row] = e;
@@ -214,45 +203,85 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
   }
 }
 
-// B = [alpha chi F ; alpha chi Gamma]: each wave does CPW columns in turn.
-// With CL the q's bubble blocks are first staged in LDS with nu fastest
-// ([l1][l3][nu]), so the mixing reads them conflict-free instead of every
-// column re-reading them from the cache.  Per column the wave forms its 2D
-// mixed values with lanes along nu (coalesced reads of X) into LDS, then lane
+// B = [alpha chi F ; alpha chi Gamma] residues, and the row maxima of
+// A = [Gamma | F] on the way (every element of Gamma and F passes here).  A
+// workgroup owns 16 adjacent columns; wave w does columns 16 blockIdx.x +
+// 4 w .. + 3 in turn, independently of the other waves.  With CL the q's
+// bubble blocks are first staged in LDS with nu fastest ([l1][l3][nu]), so the
+// mixing reads them conflict-free.  Per column the wave forms its 2D mixed
+// values with lanes along nu (coalesced reads of F and Gamma) into LDS, keeps
+// the running maxima of |Gamma| and |F| per row in its own LDS row, then lane
 // l converts inner indices 4 l + 256 n .. + 3 and stores one dword per
-// (modulus, sign).  grid: (Dp / (4 CPW), 4 * nqb); dynamic LDS
-// 4 Kp 16 bytes (+ nl^2 nf 16 with CL).
-constexpr int PREP_B_CPW = 4;
+// (modulus, sign).  At the end the four waves' row maxima are folded into the
+// block's partial rmaxp[mat][blockIdx.x][row].  grid: (Dp / 16, 4 * nqb);
+// dynamic LDS 4 Kp 16 + 4 Dp 8 (+ nl^2 nf 16 with CL).
+constexpr int PREP_B_CPW = PREP_B_COLS / 4;
 template <int NL, bool CL>
 __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
-  extern __shared__ double2 colv[];  // [wave][Kp], then with CL the bubble blocks [l1][l3][nu]
+  extern __shared__ double2 colv[];  // [4][Kp] mixed columns, [4][Dp] row maxima, with CL [l1][l3][nu]
   const int mat = blockIdx.y;
   const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
-  const int D = a.D, nf = a.nf;
+  const int D = a.D, nf = a.nf, Kp = a.Kp, Dp = a.Dp;
   constexpr int nl = NL;
   const size_t DD = (size_t)D * D;
   const double2* chig = a.chi[r >> 1] + (size_t)nf * nl * nl * q;  // [nu][l3][l1]
-  double2* chil = colv + 4 * (size_t)a.Kp;
+  double* rmx = reinterpret_cast<double*>(colv + 4 * (size_t)Kp);   // [4][Dp]
+  double2* chil = reinterpret_cast<double2*>(rmx + 4 * (size_t)Dp);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int u = tid; u < 4 * Dp; u += 256) rmx[u] = 0.0;
   if (CL) {
-    for (int u = threadIdx.x; u < nl * nl * nf; u += 256) {
+    for (int u = tid; u < nl * nl * nf; u += 256) {
       const int nu = u % nf, l = u / nf;  // l = l1 + nl l3
       chil[u] = chig[(size_t)nu * nl * nl + l];
     }
-    __syncthreads();
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double2* v = colv + (size_t)wave * a.Kp;
+  __syncthreads();
+  double2* v = colv + (size_t)wave * Kp;
+  double* rw = rmx + (size_t)wave * Dp;
+  const double2* Fq = a.fv[r] + DD * q;
+  const double2* Gq = a.gam[r] + DD * q;
+  // loads from a clamped address, unconditionally (the caller zeroes what lies
+  // outside), so the registers are written by the loads alone
+  auto load_x = [&](int col, int nu, double2 (&xf)[NL], double2 (&xg)[NL]) {
+    const size_t o = (size_t)(col < D ? col : 0) * D + (nu < nf ? nu : 0);
+#pragma unroll
+    for (int l3 = 0; l3 < NL; ++l3) {
+      xf[l3] = Fq[o + l3 * nf];
+      xg[l3] = Gq[o + l3 * nf];
+    }
+  };
+  const int col0 = blockIdx.x * PREP_B_COLS + wave * PREP_B_CPW;
+  // the first 64 nu of the next column are loaded before the current column's
+  // conversion, so their latency hides behind it
+  double2 pf[NL], pg[NL];
+  load_x(col0, lane, pf, pg);
+  int fkeep = 0;  // lane cc keeps column cc's exponent; stored after the loop, so no store in it
+                  // holds back the prefetch (a store's data register is waited for before reuse)
   for (int cc = 0; cc < PREP_B_CPW; ++cc) {
-    const int col = (blockIdx.x * 4 + wave) * PREP_B_CPW + cc;
-    const bool cok = col < D;
+    const int col = col0 + cc;
     // B((part, l1, nu), col) = alpha sum_l3 chi_nu(l1, l3) X((l3, nu), col), X = F (part 0), Gamma (part 1)
     double mx = 0.0;
-    for (int nu = lane; nu < nf; nu += 64) {
+    for (int nu0 = 0; nu0 < nf; nu0 += 64) {
+      const int nu = nu0 + lane;
       double2 xf[NL], xg[NL];
+      if (nu0 == 0) {
+#pragma unroll
+        for (int l3 = 0; l3 < NL; ++l3) {
+          xf[l3] = pf[l3];
+          xg[l3] = pg[l3];
+        }
+      } else {
+        load_x(col, nu, xf, xg);
+      }
+      if (nu >= nf) continue;
+      if (col >= D) {
+#pragma unroll
+        for (int l3 = 0; l3 < NL; ++l3) xf[l3] = xg[l3] = make_double2(0.0, 0.0);
+      }
 #pragma unroll
       for (int l3 = 0; l3 < NL; ++l3) {
-        xf[l3] = cok ? a.fv[r][DD * q + (size_t)col * D + l3 * nf + nu] : make_double2(0.0, 0.0);
-        xg[l3] = cok ? a.gam[r][DD * q + (size_t)col * D + l3 * nf + nu] : make_double2(0.0, 0.0);
+        const double am = fmax(fmax(fabs(xf[l3].x), fabs(xf[l3].y)), fmax(fabs(xg[l3].x), fabs(xg[l3].y)));
+        rw[l3 * nf + nu] = fmax(rw[l3 * nf + nu], am);
       }
 #pragma unroll
       for (int l1 = 0; l1 < NL; ++l1) {
@@ -272,17 +301,19 @@ __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
         mx = fmax(mx, fmax(fmax(fabs(mf.x), fabs(mf.y)), fmax(fabs(mg.x), fabs(mg.y))));
       }
     }
-    for (int k = 2 * D + lane; k < a.Kp; k += 64) v[k] = make_double2(0.0, 0.0);
+    for (int k = 2 * D + lane; k < Kp; k += 64) v[k] = make_double2(0.0, 0.0);
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
     int f = 0;
     if (mx > 0.0) frexp(mx, &f);
-    if (lane == 0) a.fscale[(size_t)mat * a.Dp + col] = f;
     const double sc = ldexp(1.0, a.tb - f);
-    // the wave's column in LDS is complete (each wave reads only its own)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (cc + 1 < PREP_B_CPW) load_x(col + 1, lane, pf, pg);
+    // the wave's column in LDS is complete (each wave reads only its own; the
+    // LDS executes one wave's accesses in order, so only the compiler needs
+    // holding back)
+    asm volatile("" ::: "memory");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (int k0 = 4 * lane; k0 < a.Kp; k0 += 256) {
+    asm volatile("" ::: "memory");
+    for (int k0 = 4 * lane; k0 < Kp; k0 += 256) {
       double r1[4], r0[4], i1[4], i0[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -303,11 +334,17 @@ __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
         *reinterpret_cast<int*>(a.B + ws_off(a, mat, mod, 1, col, k0)) = wm;
       }
     }
+    if (lane == cc) fkeep = f;
     // the next column overwrites v only after every lane has read it
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    asm volatile("" ::: "memory");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    asm volatile("" ::: "memory");
   }
+  if (lane < PREP_B_CPW) a.fscale[(size_t)mat * Dp + col0 + lane] = fkeep;
+  __syncthreads();
+  for (int row = tid; row < Dp; row += 256)
+    a.rmaxp[((size_t)mat * (Dp / PREP_B_COLS) + blockIdx.x) * Dp + row] =
+        fmax(fmax(rmx[row], rmx[Dp + row]), fmax(rmx[2 * Dp + row], rmx[3 * Dp + row]));
 }
 
 // The residues of one modulus (INT32 sums of the + and - products of a 32 x 32
@@ -380,7 +417,7 @@ __device__ __forceinline__ void crt_tile_epilogue(const CrtArgs& a, int mat, int
 // One workgroup per 64 x 64 tile of one (q, r); four waves, each 32 x 32 =
 // 2 x 2 blocks of v_mfma_i32_16x16x64_i8.  The loop runs over (modulus, K chunk)
 // steps through a ring of four LDS stages (A+, A-, B+, B- chunks of 64 rows x
-// 64 bytes, 16 KiB; the 16-byte columns of each row XOR-swizzled by row / 4 so
+// 64 bytes, 16 KiB; the 16-byte columns of each row XOR-swizzled by crt_swz so
 // the fragment reads are free of bank conflicts).  Each loop iteration computes
 // two steps from two stages, then stores the two steps that arrived in
 // registers during it into the two free stages and issues the loads of the two
@@ -405,7 +442,7 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
   // staging: thread tid moves 16 bytes (column tid & 3) of row tid >> 2 of each
   // piece v = 2 * which + sign (which 0: A rows, 1: B columns)
   const int srow = tid >> 2, sch = tid & 3;
-  const int sput = 16 * (sch ^ ((srow >> 2) & 3));
+  const int sput = 16 * (sch ^ crt_swz(srow));
   auto gptr = [&](int step, int v) -> const v4i* {
     const int md = step / nk, k0 = (step - md * nk) * 64;
     const int which = v >> 1, sign = v & 1;
@@ -422,7 +459,7 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
     for (int v = 0; v < 4; ++v) *reinterpret_cast<v4i*>(&S[st][v][srow][sput]) = r[v];
   };
   auto frag = [&](int st, int v, int row) -> v4i {
-    return *reinterpret_cast<const v4i*>(&S[st][v][row][16 * (lk ^ ((row >> 2) & 3))]);
+    return *reinterpret_cast<const v4i*>(&S[st][v][row][16 * (lk ^ crt_swz(row))]);
   };
   double shr[16], slr[16], shi[16], sli[16];  // CRT sums, element (i, j, e) -> 8 i + 4 j + e
 #pragma unroll
@@ -501,12 +538,14 @@ typedef __attribute__((address_space(3))) int8_t lds_i8_t;
 // M0 carries the LDS base; nothing else in these kernels reads M0
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
-__device__ __forceinline__ void dma16(const int8_t* g, unsigned lds_base) {
+// 16 bytes from gbase + voff (uniform 64-bit base, per-lane 32-bit offset) to
+// LDS at lds_base + 16 lane
+__device__ __forceinline__ void dma16(const int8_t* gbase, unsigned voff, unsigned lds_base) {
   asm volatile(
-      "s_mov_b32 m0, %1\n\t"
-      "global_load_lds_dwordx4 %0, off"
+      "s_mov_b32 m0, %2\n\t"
+      "global_load_lds_dwordx4 %0, %1"
       :
-      : "v"(g), "s"(lds_base)
+      : "v"(voff), "s"(gbase), "s"(lds_base)
       : "m0");
 }
 #pragma clang diagnostic pop
@@ -547,16 +586,23 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_dma(CrtArgs a) {
   const int nk = a.Kp / 64;
   const int nsteps = a.nmod * nk;
   const int drow = tid >> 2;
-  const int dch = (tid & 3) ^ ((drow >> 2) & 3);  // global 16-byte column landing in slot tid & 3
+  const int dch = (tid & 3) ^ crt_swz(drow);  // global 16-byte column landing in slot tid & 3
+  // the byte offsets of ws_off split into a per-lane part (row drow, column
+  // dch of a 64-row block) and a uniform part, so a step's four loads need no
+  // vector address arithmetic
+  const unsigned voff = (unsigned)(drow * 64 + 16 * dch);
+  const size_t plane = (size_t)a.Dp * a.Kp;
+  const int8_t* baseA = a.A + (size_t)mat * a.nmod * 2 * plane + (size_t)row0 * 64;
+  const int8_t* baseB = a.B + (size_t)mat * a.nmod * 2 * plane + (size_t)col0 * 64;
   auto issue = [&](int step, CrtStage& S) {
-    const int md = step / nk, k0 = (step - md * nk) * 64;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int which = v >> 1, sign = v & 1;
-      const int8_t* g = (which ? a.B : a.A) + ws_off(a, mat, md, sign, (which ? col0 : row0) + drow, k0 + 16 * dch);
-      const unsigned base = (unsigned)(uintptr_t)(lds_i8_t*)&S[v][wave * 16][0];
-      dma16(g, __builtin_amdgcn_readfirstlane(base));
-    }
+    const int md = step / nk, kc = step - md * nk;
+    const size_t off = (size_t)md * 2 * plane + (size_t)kc * a.Dp * 64;
+    // pieces A+, A-, B+, B- are 4 KiB apart in the stage
+    const unsigned lds0 = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(lds_i8_t*)&S[0][wave * 16][0]);
+    dma16(baseA + off, voff, lds0);
+    dma16(baseA + off + plane, voff, lds0 + 4096);
+    dma16(baseB + off, voff, lds0 + 8192);
+    dma16(baseB + off + plane, voff, lds0 + 12288);
   };
   double shr[16], slr[16], shi[16], sli[16];  // CRT sums, element (i, j, e) -> 8 i + 4 j + e
 #pragma unroll
@@ -569,7 +615,7 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_dma(CrtArgs a) {
   int kstep = 0, mod = 0;
   auto compute = [&](const CrtStage& S) {
     auto frag = [&](int v, int row) -> v4i {
-      return *reinterpret_cast<const v4i*>(&S[v][row][16 * (lk ^ ((row >> 2) & 3))]);
+      return *reinterpret_cast<const v4i*>(&S[v][row][16 * (lk ^ crt_swz(row))]);
     };
     v4i ap[2], am[2], bp[2], bm[2];
 #pragma unroll
@@ -634,7 +680,7 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_dma(CrtArgs a) {
 // 4 per 4, which leaves it bound by LDS bandwidth).  The steps run over
 // (modulus, sign, 64-byte K chunk): a step stages the T rows of A_sign and the
 // T columns of B_sign (2 x T x 64 bytes, 16-byte columns XOR-swizzled by
-// row / 4) into one of two LDS stages, from registers loaded three steps
+// crt_swz) into one of two LDS stages, from registers loaded three steps
 // earlier.  The INT32 products of the + sign are reduced to residues and kept
 // packed as bytes; after the - sign each element's (Re, Im) residue pair goes
 // to a global scratch (two bytes per element and modulus) instead of CRT sums
@@ -678,7 +724,7 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_big(CrtArgs a) {
     const int which = u / (4 * T), row = (u >> 2) % T, ch = u & 3;
     const size_t off = ws_off(a, mat, 0, 0, (which ? col0 : row0) + row, 16 * ch);
     pbase[v] = (which ? a.B : a.A) + off;
-    lofs[v] = (which * T + row) * 64 + 16 * (ch ^ ((row >> 2) & 3));
+    lofs[v] = (which * T + row) * 64 + 16 * (ch ^ crt_swz(row));
   }
   const size_t plane = (size_t)a.Dp * a.Kp;
   auto step_off = [&](int step) __attribute__((always_inline)) -> size_t {
@@ -696,7 +742,7 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_big(CrtArgs a) {
     for (int v = 0; v < VPT; ++v) *reinterpret_cast<v4i*>(&S[st][0][0][0] + lofs[v]) = r[v];
   };
   auto frag = [&](int st, int which, int row) __attribute__((always_inline)) -> v4i {
-    return *reinterpret_cast<const v4i*>(&S[st][which][row][16 * (lk ^ ((row >> 2) & 3))]);
+    return *reinterpret_cast<const v4i*>(&S[st][which][row][16 * (lk ^ crt_swz(row))]);
   };
 
   v4i acc[TB][TB];
@@ -954,9 +1000,11 @@ static int split_tb(const tups_ctx* c) {
   return v >= 0 && v < kNumSplitVariants ? kSplitVariants[v].tb : crt_big_tb(c->sz.nz);
 }
 
+// default for the 64 x 64 tile: four DMA-filled stages (measured 13% faster
+// than the register-staged crt_gemm at D = 320)
 static int split_ns(const tups_ctx* c) {
   const int v = c->split_variant;
-  return v >= 0 && v < kNumSplitVariants ? kSplitVariants[v].ns : 0;
+  return v >= 0 && v < kNumSplitVariants ? kSplitVariants[v].ns : 4;
 }
 
 static size_t crt_padded(int D, int tb) {
@@ -972,9 +1020,10 @@ static size_t crt_scratch_per_mat(int D, int nmod, int tb) {
 }
 
 // prep_b for Nl, with the bubble blocks staged in LDS when they fit beside the
-// four columns (160 KiB per workgroup); *lds receives the dynamic LDS bytes
-static void (*prep_b_kernel(int nl, size_t Kp, int nf, size_t* lds))(CrtArgs) {
-  const size_t cols = 4 * Kp * sizeof(double2), chi = (size_t)nl * nl * nf * sizeof(double2);
+// four columns and the row maxima (160 KiB per workgroup); *lds receives the
+// dynamic LDS bytes
+static void (*prep_b_kernel(int nl, size_t Kp, size_t Dp, int nf, size_t* lds))(CrtArgs) {
+  const size_t cols = 4 * Kp * sizeof(double2) + 4 * Dp * sizeof(double), chi = (size_t)nl * nl * nf * sizeof(double2);
   const bool cl = cols + chi <= 160 * 1024;
   *lds = cols + (cl ? chi : 0);
   if (nl == 1) return cl ? crt_prep_b<1, true> : crt_prep_b<1, false>;
@@ -993,8 +1042,8 @@ int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
   const size_t Dp = crt_padded(s.nz, tb);
   const size_t Kp = (2 * (size_t)s.nz + 63) / 64 * 64;
   // per q point: 4 channels x (A and B) x 2 signs x nmod residue planes + exponents + GEMM scratch
-  const size_t per_q = 4 * (4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int) + CRT_KB * Dp * sizeof(double) +
-                            crt_scratch_per_mat(s.nz, nmod, tb) + 16);
+  const size_t per_q = 4 * (4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int) +
+                            (Dp / PREP_B_COLS + 1) * Dp * sizeof(double) + crt_scratch_per_mat(s.nz, nmod, tb) + 16);
   // default pass: about 4 GiB of residues, at most an eighth of the free memory
   // (fewer, larger passes measured 3-5% faster than 1 GiB ones); fixed at
   // first use, so a later call (possibly under stream capture) finds the same
@@ -1018,7 +1067,7 @@ int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
     c->split_ws_bytes = need;
   }
   size_t lds_b = 0;
-  void (*prep_b)(CrtArgs) = prep_b_kernel(s.nl, Kp, s.nf, &lds_b);
+  void (*prep_b)(CrtArgs) = prep_b_kernel(s.nl, Kp, Dp, s.nf, &lds_b);
   if (lds_b > 65536)
     TUPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(prep_b), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds_b));
@@ -1054,7 +1103,7 @@ int launch_bse_split(tups_ctx* c) {
   a.omf = 1.0 - c->f_damping;
   crt_constants(a, nmod, s.nz);
   size_t lds_b = 0;
-  void (*prep_b)(CrtArgs) = prep_b_kernel(s.nl, Kp, s.nf, &lds_b);
+  void (*prep_b)(CrtArgs) = prep_b_kernel(s.nl, Kp, Dp, s.nf, &lds_b);
   for (int q0 = 0; q0 < s.nq; q0 += qbatch) {
     a.q0 = q0;
     a.nqb = std::min(qbatch, s.nq - q0);
@@ -1063,18 +1112,18 @@ int launch_bse_split(tups_ctx* c) {
     a.A = base;
     a.B = base + mats * 2 * nmod * Dp * Kp;
     a.rmax = reinterpret_cast<double*>(a.B + mats * 2 * nmod * Dp * Kp);  // 8-byte aligned: Kp % 64 == 0
-    a.escale = reinterpret_cast<int*>(a.rmax + mats * CRT_KB * Dp);
+    a.rmaxp = a.rmax + mats * Dp;
+    a.escale = reinterpret_cast<int*>(a.rmaxp + mats * (Dp / PREP_B_COLS) * Dp);
     a.fscale = a.escale + mats * Dp;
     a.crt = reinterpret_cast<int*>((reinterpret_cast<uintptr_t>(a.fscale + mats * Dp) + 15) / 16 * 16);
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL(crt_rowmax_a, dim3((unsigned)(Dp / 64 * CRT_KB), (unsigned)mats), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(prep_b, dim3((unsigned)(Dp / PREP_B_COLS), (unsigned)mats), dim3(256), lds_b, c->stream, a);
+    TUPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(crt_rowmax_fold, dim3((unsigned)((Dp + 255) / 256), (unsigned)mats), dim3(256), 0, c->stream,
                        a);
     TUPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(crt_prep_a, dim3((unsigned)((Dp / 64) * (Kp / 64)), (unsigned)mats), dim3(256), 0,
                        c->stream, a);
-    TUPS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(prep_b, dim3((unsigned)(Dp / (4 * PREP_B_CPW)), (unsigned)mats), dim3(256), lds_b, c->stream,
-                       a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_SPLIT_PREP)) return rc;
     const long long nblk = (long long)a.tiles_n * a.tiles_n * mats;

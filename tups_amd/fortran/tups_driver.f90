@@ -2,11 +2,14 @@
 !> execute_loop (SRC/loop.f90:78-139, 219-222):
 !>   BSE -> calculate_expSqR/expqSR -> calc_PhiR -> calc_M_* -> calculate_PAE
 !>   -> G = F - G
+!> then BSE again on the resident vertices (no upload), and once more after the
+!> host halves F_r_LL and declares it with tups_host_modified.
 !> Usage: tups_fortran_driver <input.bin> <output.bin>
 !> input (stream): int32 nx, nf, nl, nr; real(8) beta, f_damping;
 !>   complex(8) G_d,G_m,G_s,G_t, F_d..F_t (Nz,Nz,Nq), chi_ph, chi_pp (Nz,Nz,Nq),
 !>   L_d..L_t (Nf,Nf,Nf/2)
-!> output (stream): Phi_r (4), PhiR_r (4), M tensors (4, per q), F_r (4), G_r (4)
+!> output (stream): Phi_r (4), PhiR_r (4), M tensors (4, per q), F_r (4), G_r (4),
+!>   Phi_r of the second BSE (4), Phi_r of the third BSE (4)
 program tups_fortran_driver
   use, intrinsic :: iso_c_binding
   use tups_gpu_iface
@@ -54,6 +57,15 @@ program tups_fortran_driver
   call calculate_PAE()
   write(u) F_d_LL, F_m_LL, F_s_LL, F_t_LL
   call update_vertices()
+  write(u) G_d_LL, G_m_LL, G_s_LL, G_t_LL
+  call BSE(1)
+  write(u) G_d_LL, G_m_LL, G_s_LL, G_t_LL
+  F_d_LL = 0.5d0 * F_d_LL
+  F_m_LL = 0.5d0 * F_m_LL
+  F_s_LL = 0.5d0 * F_s_LL
+  F_t_LL = 0.5d0 * F_t_LL
+  call tups_host_modified(.false., .true.)
+  call BSE(2)
   write(u) G_d_LL, G_m_LL, G_s_LL, G_t_LL
   close(u)
   call tups_gpu_finalize()

@@ -13,8 +13,14 @@
 !>
 !> Residency: the vertex arrays stay in HBM between calls; the host arrays are
 !> kept as mirrors (G_r_LL after BSE holds Phi for the host-side SDE,
-!> SRC/parquet_SDE.f90:523-566; F_r_LL after calculate_PAE).  BSE uploads G_r_LL
-!> and F_r_LL because the host owns them between iterations.
+!> SRC/parquet_SDE.f90:523-566; F_r_LL after calculate_PAE; G_r_LL after
+!> update_vertices).  BSE uploads G_r_LL and F_r_LL only when the device copy
+!> may differ from the host array: before the first BSE, and after the host
+!> declared a change with tups_host_modified.  In execute_loop
+!> (SRC/loop.f90:78-222) the host writes these arrays only in the G = F - G
+!> step, which update_vertices does on the device; a host that keeps that step
+!> (or changes the arrays elsewhere) calls tups_host_modified(.true., .false.)
+!> after it, or sets tups_trust_residency = .false. to upload on every BSE.
 module tups_dropin
   use, intrinsic :: iso_c_binding
   use tups_gpu_iface
@@ -26,6 +32,11 @@ module tups_dropin
   logical, save :: lambda_on_device = .false.
   ! arithmetic of BSE(ite): TUPS_BSE_LADDER_FP64 (default) or TUPS_BSE_LADDER_SPLIT
   integer(c_int32_t), save :: tups_bse_mode = TUPS_BSE_LADDER_FP64
+  ! .false.: BSE uploads G_r_LL and F_r_LL on every call
+  logical, save :: tups_trust_residency = .true.
+  ! the device copy of G_r_LL (its Gamma slot) / F_r_LL equals the host array
+  logical, save :: g_on_device = .false.
+  logical, save :: f_on_device = .false.
 
 contains
 
@@ -52,7 +63,17 @@ contains
     end if
     m_built = .false.
     lambda_on_device = .false.
+    g_on_device = .false.
+    f_on_device = .false.
   end subroutine tups_gpu_init
+
+  !> The host changed G_r_LL (gamma) and/or F_r_LL (f) since the library last
+  !> saw them: the next BSE uploads them.
+  subroutine tups_host_modified(gamma, f)
+    logical, intent(in) :: gamma, f
+    if (gamma) g_on_device = .false.
+    if (f) f_on_device = .false.
+  end subroutine tups_host_modified
 
   subroutine tups_gpu_finalize()
     if (c_associated(tups_h)) call tups_check(tups_destroy(tups_h), 'tups_destroy')
@@ -75,19 +96,25 @@ contains
       call tups_check(tups_upload_chi_dense(tups_h, 1_c_int32_t, int(idx_q - 1, c_int32_t), 1_c_int32_t, &
                                             chi_pp_LL), 'upload chi_pp')
     end do
-    call put(TUPS_ARR_GAMMA, TUPS_CH_D, G_d_LL)
-    call put(TUPS_ARR_GAMMA, TUPS_CH_M, G_m_LL)
-    call put(TUPS_ARR_GAMMA, TUPS_CH_S, G_s_LL)
-    call put(TUPS_ARR_GAMMA, TUPS_CH_T, G_t_LL)
-    call put(TUPS_ARR_F, TUPS_CH_D, F_d_LL)
-    call put(TUPS_ARR_F, TUPS_CH_M, F_m_LL)
-    call put(TUPS_ARR_F, TUPS_CH_S, F_s_LL)
-    call put(TUPS_ARR_F, TUPS_CH_T, F_t_LL)
+    if (.not. (g_on_device .and. tups_trust_residency)) then
+      call put(TUPS_ARR_GAMMA, TUPS_CH_D, G_d_LL)
+      call put(TUPS_ARR_GAMMA, TUPS_CH_M, G_m_LL)
+      call put(TUPS_ARR_GAMMA, TUPS_CH_S, G_s_LL)
+      call put(TUPS_ARR_GAMMA, TUPS_CH_T, G_t_LL)
+    end if
+    if (.not. (f_on_device .and. tups_trust_residency)) then
+      call put(TUPS_ARR_F, TUPS_CH_D, F_d_LL)
+      call put(TUPS_ARR_F, TUPS_CH_M, F_m_LL)
+      call put(TUPS_ARR_F, TUPS_CH_S, F_s_LL)
+      call put(TUPS_ARR_F, TUPS_CH_T, F_t_LL)
+      f_on_device = .true.
+    end if
     call tups_check(tups_bse(tups_h, tups_bse_mode), 'BSE')
     call get(TUPS_ARR_GAMMA, TUPS_CH_D, G_d_LL)
     call get(TUPS_ARR_GAMMA, TUPS_CH_M, G_m_LL)
     call get(TUPS_ARR_GAMMA, TUPS_CH_S, G_s_LL)
     call get(TUPS_ARR_GAMMA, TUPS_CH_T, G_t_LL)
+    g_on_device = .true.  ! the Gamma slot holds Phi, as G_r_LL now does
     if (ite < 0) stop 1
   end subroutine BSE
 
@@ -185,6 +212,7 @@ contains
     call get(TUPS_ARR_F, TUPS_CH_M, F_m_LL)
     call get(TUPS_ARR_F, TUPS_CH_S, F_s_LL)
     call get(TUPS_ARR_F, TUPS_CH_T, F_t_LL)
+    f_on_device = .true.
   end subroutine calculate_PAE
 
   !> G_r = F_r - G_r on the device, SRC/loop.f90:219-222
@@ -194,6 +222,7 @@ contains
     call get(TUPS_ARR_GAMMA, TUPS_CH_M, G_m_LL)
     call get(TUPS_ARR_GAMMA, TUPS_CH_S, G_s_LL)
     call get(TUPS_ARR_GAMMA, TUPS_CH_T, G_t_LL)
+    g_on_device = .true.
   end subroutine update_vertices
 
   subroutine put(which, ch, a)

@@ -156,7 +156,9 @@ def main(argv=None):
     ap.add_argument("--nf", type=int, default=64)
     ap.add_argument("--nl", type=int, default=1)
     ap.add_argument("--moduli", type=int, default=12)
-    ap.add_argument("--variant", default="", help="batch: tile variant by name (default: autotune)")
+    ap.add_argument("--variant", default="",
+                    help="batch: tile variant by name (default: autotune); split: GEMM tile by name (default: the "
+                         "library's)")
     args = ap.parse_args(argv)
     if args.workload == "sus":
         # calc_sus on Nx=8, Nf=64, Nl=5 (all 480 IBZ q): random F and Phi, Sigma = 0
@@ -256,6 +258,9 @@ def main(argv=None):
             print(f"closed_ir: {args.steps} calls, refinement (steps, residual) {c.solve_info()}")
         elif args.workload == "split":
             c.set_split(args.moduli)
+            if args.variant:
+                from .runtime import K_SPLIT_GEMM
+                c.set_variant(dict((n, v) for v, n in c.variants(K_SPLIT_GEMM))[args.variant], K_SPLIT_GEMM)
             for _ in range(args.steps):
                 c.BSE(0, mode=BSE_LADDER_SPLIT)
             ops = 2.0 * args.moduli * 2.0 * c.nz ** 2 * (2 * c.nz) * 4 * c.nq

@@ -309,8 +309,13 @@ class TupsContext:
             raise ValueError(f"shape {a.shape} != {self.shape(which)}")
         _check(self.lib.tups_upload(self._h, which, channel, a.ctypes.data_as(_D)), "tups_upload")
 
-    def download(self, which: int, channel: int) -> np.ndarray:
-        out = np.zeros(self.shape(which), dtype=np.complex128, order="F")
+    def download(self, which: int, channel: int, out: np.ndarray = None) -> np.ndarray:
+        """Device array -> host; into `out` (complex128, Fortran order, the array's
+        shape) when given, as the drop-in fills the host mirrors in place."""
+        if out is None:
+            out = np.zeros(self.shape(which), dtype=np.complex128, order="F")
+        elif out.dtype != np.complex128 or out.shape != self.shape(which) or not out.flags.f_contiguous:
+            raise ValueError("out must be a Fortran-ordered complex128 array of the array's shape")
         _check(self.lib.tups_download(self._h, which, channel, out.ctypes.data_as(_D)), "tups_download")
         return out
 
