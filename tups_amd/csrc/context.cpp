@@ -255,8 +255,11 @@ int tups_create(const tups_dims* d, tups_ctx** out) {
   }
   for (int p = 0; p < 2 && rc == 0; ++p) rc = dalloc(&c->chi[p], array_elems(c, TUPS_CHI));
   if (!rc && s.full_ibz && s.nl == 1) rc = dalloc(&c->pae_y, (size_t)8 * s.nf * s.nf * s.nw * s.nr);
-  if (!rc && s.full_ibz && s.nl > 1)
-    rc = dalloc(&c->pae_ym, (size_t)8 * s.nf * s.nf * s.nw * s.nl * s.nl * s.nr);
+  if (!rc && s.full_ibz && s.nl > 1) {
+    // columns in 8 x 8 blocks of (nu1, nu2), padded to whole blocks
+    const size_t nblk = (size_t)(s.nf + 7) / 8;
+    rc = dalloc(&c->pae_ym, (size_t)8 * 64 * nblk * nblk * s.nw * s.nl * s.nl * s.nr);
+  }
   if (!rc) {
     he = hipMalloc((void**)&c->d_flag, sizeof(int));
     if (he != hipSuccess) rc = TUPS_ENOMEM;

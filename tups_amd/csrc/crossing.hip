@@ -446,6 +446,11 @@ int autotune_pae(tups_ctx* c, int reps, int* chosen, float* best_ms) {
 // (Nl^2 NR each) concatenated, each (qw, pair, target) is one complex GEMM
 //   OUT = [M_k0 | M_k1] [Y_k0t ; Y_k1t]
 // whose epilogue adds Phi and Lambda Nc (l1 = l2 = 1 block) and writes F once.
+// The columns are ordered in 8 x 8 blocks of (nu1, nu2), one block per 64-column
+// tile: a routine reaches the entries of a band in (nu1, nu2) (|nu2 - nu1| or
+// nu1 + nu2 against qw bounded), so for about a fifth of the (tile, routine)
+// pairs its Y block is zero throughout, and the tile skips that half of the
+// inner dimension.
 
 struct PaeMmArgs {
   const double2* phi[4];
@@ -454,11 +459,38 @@ struct PaeMmArgs {
   const double2* M[4];
   const double2* phir[4];
   const int* signs;
-  double2* Y;  // [kt = 4 pair + 2 routine + target][qw][j = l3 + Nl l4 + Nl^2 R][n = nu1 + Nf nu2]
+  double2* Y;  // [kt = 4 pair + 2 routine + target][qw][j = l3 + Nl l4 + Nl^2 R][n]: n = 64 block + c,
+               // block = bx + ceil(Nf / 8) by, nu1 = 8 bx + c % 8, nu2 = 8 by + c / 8
   const int* rows;  // GEMM row -> (l1, l2, Q) row index l1 + Nl l2 + Nl^2 m, or null for all rows in order
-  int nf, nl, nw, nibz, nr, nz, kk, rr, tiles_r, tiles_n;
+  int nf, nl, nw, nibz, nr, nz, kk, rr, tiles_r, tiles_n, nbf;  // nbf = ceil(Nf / 8), tiles_n = nbf^2
   double inv_nx, nc_aux;
 };
+
+// 0-based (nu1, nu2) of blocked column n; false for the padding of the last blocks
+__device__ __forceinline__ bool pae_col(const PaeMmArgs& a, int n, int& nu1, int& nu2) {
+  const int blk = n >> 6, c = n & 63;
+  nu1 = 8 * (blk % a.nbf) + (c & 7);
+  nu2 = 8 * (blk / a.nbf) + (c >> 3);
+  return nu1 < a.nf && nu2 < a.nf;
+}
+
+// does routine rt of target pair p reach entry (nu1, nu2) (1-based) at qw?  The
+// frequency conditions of the terms ybuild forms (weight nonzero)
+__device__ __forceinline__ bool pae_reaches(int p, int rt, int nu1, int nu2, int qw, int nf, int nw) {
+  if (p == rt) {  // phbar->ph (p 0, rt 0) and phbar->pp (p 1, rt 1): |nu2 - nu1| within the bosonic box
+    const int dn = nu2 - nu1;
+    return (dn >= 0 && dn + 1 <= nw) || (dn < 0 && 1 - dn <= nw);
+  }
+  int w2, w2t;
+  if (p == 0) {  // pp->ph
+    w2 = nu1 + nu2 + qw - nf - 1;
+    w2t = nf + 3 - qw - nu1 - nu2;
+  } else {  // ph->pp
+    w2 = qw - nu1 - nu2 + nf + 1;
+    w2t = nu1 + nu2 - qw - nf + 1;
+  }
+  return (w2 >= 1 && w2 <= nw) || (w2t >= 2 && w2t <= nw);
+}
 
 // Y for all (kt, qw, j, n): one thread per n for a fixed (j, qw).  The gathers of
 // neighbouring n at neighbouring qw share cache lines of PhiR (the frequency
@@ -466,14 +498,25 @@ struct PaeMmArgs {
 // XCD.  grid: ((Nf^2 + 255) / 256) * kk * nw
 __global__ __launch_bounds__(256) void pae_ybuild(PaeMmArgs a) {
   const int nf = a.nf, nl = a.nl, nw = a.nw, nz = a.nz;
-  const int nbn = (nf * nf + 255) / 256;
+  const int ncol = 64 * a.tiles_n;
+  const int nbn = ncol / 256 + (ncol % 256 ? 1 : 0);
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
   const int w0 = logical % nw, rest = logical / nw;
   const int j = rest / nbn, qw = w0 + 1;
   const int n = (rest - j * nbn) * 256 + threadIdx.x;
-  if (n >= nf * nf) return;
+  if (n >= ncol) return;
+  const size_t nn = (size_t)ncol;
+  auto at = [&](int kt) { return ((size_t)(kt * nw + w0) * a.kk + j) * nn + n; };
+  int nu1, nu2;
+  if (!pae_col(a, n, nu1, nu2)) {
+    // padding columns: finite zeros (their products land in columns the epilogue drops)
+#pragma unroll
+    for (int kt = 0; kt < 8; ++kt) a.Y[at(kt)] = make_double2(0.0, 0.0);
+    return;
+  }
+  ++nu1;
+  ++nu2;
   const int l3 = j % nl, l4 = (j / nl) % nl, R = j / (nl * nl);
-  const int nu1 = n % nf + 1, nu2 = n / nf + 1;
   const double sgn = (double)(a.signs[l3] * a.signs[l4]);
   auto comb = [&](int s0, int s1, const Term& t, double c0, double c1) {
     const size_t i = (((size_t)R * nw + (t.w2 - 1)) * nz + (size_t)(l4 * nf + t.nub - 1)) * nz +
@@ -497,8 +540,6 @@ __global__ __launch_bounds__(256) void pae_ybuild(PaeMmArgs a) {
                             nf + 1 - nu1, nu1 - qw - w2t + 2);
   const Term t3 = make_term(pb_plain, pb_tr, dn + 1, nu1, clampnu(qw - nu1 - dn + nf, nf), 1 - dn,
                             nf + 1 - nu1, clampnu(nu1 - qw - (1 - dn) + 2, nf));
-  const size_t nn = (size_t)nf * nf;
-  auto at = [&](int kt) { return ((size_t)(kt * nw + w0) * a.kk + j) * nn + n; };
   a.Y[at(0)] = comb(0, 1, t0, -0.5, -1.5);  // phbar->ph -> d
   a.Y[at(1)] = comb(0, 1, t0, -0.5, 0.5);   //           -> m
   a.Y[at(2)] = comb(2, 3, t1, 0.5, 1.5);    // pp->ph    -> d
@@ -525,9 +566,21 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
   // row tiles vary fastest: the tiles_r workgroups that share one column tile of
   // Y run together on one XCD, so Y comes from HBM once and the re-reads hit L2
   const int row0 = (tile % a.tiles_r) * 64, col0 = (tile / a.tiles_r) * 64;
-  const int nl2 = a.nl * a.nl, kk = a.kk, ktot = 2 * kk;
-  const size_t nn = (size_t)a.nf * a.nf;
+  const int nl2 = a.nl * a.nl, kk = a.kk;
+  const int kkp = (kk + MMK - 1) / MMK * MMK;  // each routine's inner range padded to whole chunks
+  const size_t nn = (size_t)64 * a.tiles_n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the routines that reach some entry of this tile's 8 x 8 frequency block
+  // (lane = column; every wave forms the same ballot)
+  bool reach0 = false, reach1 = false;
+  {
+    int nu1, nu2;
+    if (pae_col(a, col0 + lane, nu1, nu2)) {
+      reach0 = pae_reaches(p, 0, nu1 + 1, nu2 + 1, w0 + 1, a.nf, a.nw);
+      reach1 = pae_reaches(p, 1, nu1 + 1, nu2 + 1, w0 + 1, a.nf, a.nw);
+    }
+  }
+  const int vbeg = __any(reach0) ? 0 : kkp, vend = __any(reach1) ? 2 * kkp : kkp;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
   // staging: A element (row tid & 63, k (tid >> 6) + 4 i), B element (k (tid >> 6) + 4 i, column tid & 63)
@@ -537,19 +590,19 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
   const int grow = rok ? (a.rows ? a.rows[gr] : gr) : 0;
   const int rl = grow % nl2, rm = grow / nl2;
   const int gn = col0 + sr;
-  const bool nok = gn < (int)nn;
   double2 ra[4], rb[4];
-  auto load = [&](int k0) {
+  // virtual inner index v: routine v / kkp, its index jj = v % kkp (< kk, else padding)
+  auto load = [&](int v0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int k = k0 + sk + 4 * i;
-      const bool kok = k < ktot;
-      const int rt = k >= kk ? 1 : 0, jj = k - rt * kk;
+      const int v = v0 + sk + 4 * i;
+      const int rt = v >= kkp ? 1 : 0, jj = v - rt * kkp;
+      const bool kok = jj < kk;
       const int jl = jj % nl2, jr = jj / nl2;
       const size_t mi = (size_t)rl + (size_t)nl2 * jl + (size_t)nl2 * nl2 * (rm + (size_t)a.nibz * jr);
       ra[i] = (rok && kok) ? a.M[2 * p + rt][mi] : make_double2(0.0, 0.0);
       const int kt = 4 * p + 2 * rt + t;
-      rb[i] = (nok && kok) ? a.Y[((size_t)(kt * a.nw + w0) * kk + jj) * nn + gn] : make_double2(0.0, 0.0);
+      rb[i] = kok ? a.Y[((size_t)(kt * a.nw + w0) * kk + jj) * nn + gn] : make_double2(0.0, 0.0);
     }
   };
   auto store = [&]() {
@@ -566,11 +619,11 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc_re[i][j] = acc_im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  load(0);
-  for (int k0 = 0; k0 < ktot; k0 += MMK) {
+  if (vbeg < vend) load(vbeg);
+  for (int k0 = vbeg; k0 < vend; k0 += MMK) {
     store();
     __syncthreads();
-    if (k0 + MMK < ktot) load(k0 + MMK);
+    if (k0 + MMK < vend) load(k0 + MMK);
 #pragma unroll
     for (int ks = 0; ks < MMK / 4; ++ks) {
       const int kr = ks * 4 + lk;
@@ -601,9 +654,8 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int n = col0 + wc + j * 16 + li;
-      if (n >= (int)nn) continue;
-      const int nu1 = n % nf, nu2 = n / nf;
+      int nu1, nu2;
+      if (!pae_col(a, col0 + wc + j * 16 + li, nu1, nu2)) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int rg = row0 + wr + i * 16 + lk + 4 * e;
@@ -650,12 +702,13 @@ static int pae_gemm_path(tups_ctx* c) {
   a.rows = c->pae_rows;
   a.rr = c->pae_rows ? c->pae_nrows : s.nl * s.nl * s.nibz;
   a.tiles_r = (a.rr + 63) / 64;
-  a.tiles_n = (s.nf * s.nf + 63) / 64;
+  a.nbf = (s.nf + 7) / 8;
+  a.tiles_n = a.nbf * a.nbf;
   a.inv_nx = 1.0 / s.nx;
   const double f0 = c->tab.ff[0];
   a.nc_aux = std::fabs(1.0 / f0 * 1.0 / f0);
   if (int rc = timer_begin(c)) return rc;
-  hipLaunchKernelGGL(pae_ybuild, dim3((unsigned)((s.nf * s.nf + 255) / 256 * a.kk * s.nw)), dim3(256), 0,
+  hipLaunchKernelGGL(pae_ybuild, dim3((unsigned)((64 * a.tiles_n + 255) / 256 * a.kk * s.nw)), dim3(256), 0,
                      c->stream, a);
   TUPS_HIP(hipGetLastError());
   if (int rc = timer_end(c, TUPS_K_PAE_GATHER)) return rc;

@@ -133,7 +133,7 @@ struct tups_ctx {
   int* d_flag = nullptr;
   double2* d_sigma = nullptr;   // calc_chi: Sigma(Nx Ny Nf)
   double2* d_sigh = nullptr;    // calc_chi: Sigma_H(Nx, Ny)
-  double2* pae_ym = nullptr;    // Nl > 1 crossing: gathered PhiR, 8 x (Nl^2 NR, Nf^2) per qw
+  double2* pae_ym = nullptr;    // Nl > 1 crossing: gathered PhiR, 8 x (Nl^2 NR, 64 ceil(Nf / 8)^2) per qw
   double2* pae_y = nullptr;     // Nl = 1 crossing: re-indexed PhiR, 8 x (Nf,Nf,Nw,NR)
 
 
