@@ -567,7 +567,7 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
   // Y run together on one XCD, so Y comes from HBM once and the re-reads hit L2
   const int row0 = (tile % a.tiles_r) * 64, col0 = (tile / a.tiles_r) * 64;
   const int nl2 = a.nl * a.nl, kk = a.kk;
-  const int kkp = (kk + MMK - 1) / MMK * MMK;  // each routine's inner range padded to whole chunks
+  const int kkp = (kk + 3) / 4 * 4;  // each routine's inner range padded to whole MFMA k-steps
   const size_t nn = (size_t)64 * a.tiles_n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // the routines that reach some entry of this tile's 8 x 8 frequency block
@@ -591,7 +591,8 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
   const int rl = grow % nl2, rm = grow / nl2;
   const int gn = col0 + sr;
   double2 ra[4], rb[4];
-  // virtual inner index v: routine v / kkp, its index jj = v % kkp (< kk, else padding)
+  // virtual inner index v: routine v / kkp, its index jj = v % kkp (< kk, else padding); a
+  // chunk may straddle the two routines
   auto load = [&](int v0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -626,6 +627,7 @@ __global__ __launch_bounds__(256, 2) void pae_mm(PaeMmArgs a) {
     if (k0 + MMK < vend) load(k0 + MMK);
 #pragma unroll
     for (int ks = 0; ks < MMK / 4; ++ks) {
+      if (k0 + 4 * ks >= vend) break;  // the last chunk may end early
       const int kr = ks * 4 + lk;
       double ar[2], ai[2], br[2], bi[2];
 #pragma unroll
