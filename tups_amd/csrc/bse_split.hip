@@ -209,26 +209,27 @@ __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
 // 4 w .. + 3 in turn, independently of the other waves.  With CL the q's
 // bubble blocks are first staged in LDS with nu fastest ([l1][l3][nu]), so the
 // mixing reads them conflict-free.  Per column the wave forms its 2D mixed
-// values with lanes along nu (coalesced reads of F and Gamma) into LDS, keeps
-// the running maxima of |Gamma| and |F| per row in its own LDS row, then lane
-// l converts inner indices 4 l + 256 n .. + 3 and stores one dword per
-// (modulus, sign).  At the end the four waves' row maxima are folded into the
-// block's partial rmaxp[mat][blockIdx.x][row].  grid: (Dp / 16, 4 * nqb);
-// dynamic LDS 4 Kp 16 + 4 Dp 8 (+ nl^2 nf 16 with CL).
+// values with lanes along nu (coalesced reads of F and Gamma) into LDS, raises
+// the workgroup's running maxima of |Gamma| and |F| per row (an LDS atomic max
+// on the bits of the non-negative doubles, which order as unsigned integers),
+// then lane l converts inner indices 4 l + 256 n .. + 3 and stores one dword
+// per (modulus, sign).  At the end the row maxima are the block's partial
+// rmaxp[mat][blockIdx.x][row].  grid: (Dp / 16, 4 * nqb); dynamic LDS
+// 4 Kp 16 + Dp 8 (+ nl^2 nf 16 with CL).
 constexpr int PREP_B_CPW = PREP_B_COLS / 4;
 template <int NL, bool CL>
 __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
-  extern __shared__ double2 colv[];  // [4][Kp] mixed columns, [4][Dp] row maxima, with CL [l1][l3][nu]
+  extern __shared__ double2 colv[];  // [4][Kp] mixed columns, [Dp] row maxima (bits), with CL [l1][l3][nu]
   const int mat = blockIdx.y;
   const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
   const int D = a.D, nf = a.nf, Kp = a.Kp, Dp = a.Dp;
   constexpr int nl = NL;
   const size_t DD = (size_t)D * D;
   const double2* chig = a.chi[r >> 1] + (size_t)nf * nl * nl * q;  // [nu][l3][l1]
-  double* rmx = reinterpret_cast<double*>(colv + 4 * (size_t)Kp);   // [4][Dp]
-  double2* chil = reinterpret_cast<double2*>(rmx + 4 * (size_t)Dp);
+  unsigned long long* rmx = reinterpret_cast<unsigned long long*>(colv + 4 * (size_t)Kp);  // [Dp]
+  double2* chil = reinterpret_cast<double2*>(rmx + (size_t)Dp);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int u = tid; u < 4 * Dp; u += 256) rmx[u] = 0.0;
+  for (int u = tid; u < Dp; u += 256) rmx[u] = 0ull;
   if (CL) {
     for (int u = tid; u < nl * nl * nf; u += 256) {
       const int nu = u % nf, l = u / nf;  // l = l1 + nl l3
@@ -237,7 +238,6 @@ __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
   }
   __syncthreads();
   double2* v = colv + (size_t)wave * Kp;
-  double* rw = rmx + (size_t)wave * Dp;
   const double2* Fq = a.fv[r] + DD * q;
   const double2* Gq = a.gam[r] + DD * q;
   // loads from a clamped address, unconditionally (the caller zeroes what lies
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
 #pragma unroll
       for (int l3 = 0; l3 < NL; ++l3) {
         const double am = fmax(fmax(fabs(xf[l3].x), fabs(xf[l3].y)), fmax(fabs(xg[l3].x), fabs(xg[l3].y)));
-        rw[l3 * nf + nu] = fmax(rw[l3 * nf + nu], am);
+        atomicMax(&rmx[l3 * nf + nu], (unsigned long long)__double_as_longlong(am));
       }
 #pragma unroll
       for (int l1 = 0; l1 < NL; ++l1) {
@@ -343,8 +343,7 @@ __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
   if (lane < PREP_B_CPW) a.fscale[(size_t)mat * Dp + col0 + lane] = fkeep;
   __syncthreads();
   for (int row = tid; row < Dp; row += 256)
-    a.rmaxp[((size_t)mat * (Dp / PREP_B_COLS) + blockIdx.x) * Dp + row] =
-        fmax(fmax(rmx[row], rmx[Dp + row]), fmax(rmx[2 * Dp + row], rmx[3 * Dp + row]));
+    a.rmaxp[((size_t)mat * (Dp / PREP_B_COLS) + blockIdx.x) * Dp + row] = __longlong_as_double((long long)rmx[row]);
 }
 
 // The residues of one modulus (INT32 sums of the + and - products of a 32 x 32
@@ -1023,7 +1022,7 @@ static size_t crt_scratch_per_mat(int D, int nmod, int tb) {
 // four columns and the row maxima (160 KiB per workgroup); *lds receives the
 // dynamic LDS bytes
 static void (*prep_b_kernel(int nl, size_t Kp, size_t Dp, int nf, size_t* lds))(CrtArgs) {
-  const size_t cols = 4 * Kp * sizeof(double2) + 4 * Dp * sizeof(double), chi = (size_t)nl * nl * nf * sizeof(double2);
+  const size_t cols = 4 * Kp * sizeof(double2) + Dp * sizeof(double), chi = (size_t)nl * nl * nf * sizeof(double2);
   const bool cl = cols + chi <= 160 * 1024;
   *lds = cols + (cl ? chi : 0);
   if (nl == 1) return cl ? crt_prep_b<1, true> : crt_prep_b<1, false>;
