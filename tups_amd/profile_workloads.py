@@ -156,6 +156,7 @@ def main(argv=None):
     ap.add_argument("--nf", type=int, default=64)
     ap.add_argument("--nl", type=int, default=1)
     ap.add_argument("--moduli", type=int, default=12)
+    ap.add_argument("--qbatch", type=int, default=0, help="split: q points per pass (default: the library's)")
     ap.add_argument("--variant", default="",
                     help="batch: tile variant by name (default: autotune); split: GEMM tile by name (default: the "
                          "library's)")
@@ -257,7 +258,7 @@ def main(argv=None):
                 c.BSE(0, mode=BSE_CLOSED_IR)
             print(f"closed_ir: {args.steps} calls, refinement (steps, residual) {c.solve_info()}")
         elif args.workload == "split":
-            c.set_split(args.moduli)
+            c.set_split(args.moduli, args.qbatch)
             if args.variant:
                 from .runtime import K_SPLIT_GEMM
                 c.set_variant(dict((n, v) for v, n in c.variants(K_SPLIT_GEMM))[args.variant], K_SPLIT_GEMM)
