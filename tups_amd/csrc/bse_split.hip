@@ -37,7 +37,6 @@
 #include <cstdint>
 #include <numeric>
 #include <type_traits>
-#include <utility>
 
 namespace tups {
 
@@ -566,21 +565,9 @@ __device__ __forceinline__ void stage_barrier() {
 
 typedef int8_t CrtStage[4][CRT_TILE][64];
 
-// f(integral_constant<int, I>) for I in the sequence, in order
-template <typename F, int... I>
-__device__ __forceinline__ void static_for(F&& f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-
-//
-// PF: the fragments of step s + 1 are read from LDS into a second register set
-// before the MFMAs of step s issue, so the LDS latency of one step overlaps
-// the matrix work of the other.  Stage s + 1 must then be complete one step
-// earlier: the barrier of iteration s follows the thread's wait for its loads
-// of step s + 1, and NS - 2 steps stay in flight.
-template <int NS, bool PF>
+template <int NS>
 __global__ __launch_bounds__(256, 2) void crt_gemm_dma(CrtArgs a) {
-  static_assert(NS >= 3 && NS <= 5 && (!PF || NS >= 4), "stages");
+  static_assert(NS >= 3 && NS <= 5, "stages");
   // one LDS object per stage, so the compiler sees which stage a read touches
   __shared__ __attribute__((aligned(16))) CrtStage S0;
   __shared__ __attribute__((aligned(16))) CrtStage S1;
@@ -670,76 +657,16 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_dma(CrtArgs a) {
     if (s + NS - 1 < nsteps) issue(s + NS - 1, stage(std::integral_constant<int, I + NS - 1>{}));
     compute(stage(ic));
   };
-  // PF: fragment sets f0 (even steps) and f1 (odd steps)
-  v4i f0[8], f1[8];
-  auto readfrag = [&](const CrtStage& S, v4i(&f)[8]) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int ra = wr + i * 16 + li, rb = wc + i * 16 + li;
-      f[i] = *reinterpret_cast<const v4i*>(&S[0][ra][16 * (lk ^ crt_swz(ra))]);
-      f[2 + i] = *reinterpret_cast<const v4i*>(&S[1][ra][16 * (lk ^ crt_swz(ra))]);
-      f[4 + i] = *reinterpret_cast<const v4i*>(&S[2][rb][16 * (lk ^ crt_swz(rb))]);
-      f[6 + i] = *reinterpret_cast<const v4i*>(&S[3][rb][16 * (lk ^ crt_swz(rb))]);
-    }
-  };
-  auto mma = [&](const v4i(&f)[8]) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        accp[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f[i], f[4 + j], accp[i][j], 0, 0, 0);
-        accm[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f[2 + i], f[6 + j], accm[i][j], 0, 0, 0);
-      }
-    if (++kstep == nk) {
-      crt_join(a, mod, accp, accm, shr, slr, shi, sli);
-      kstep = 0;
-      ++mod;
-    }
-  };
-  auto step_pf = [&](auto ic, int s) {
-    constexpr int I = decltype(ic)::value;
-    if (s >= nsteps) return;
-    if (s + 1 < nsteps) {
-      // this thread's loads after those of step s + 1 are steps s + 2 .. s + NS - 2
-      if (s + NS - 2 < nsteps)
-        wait_vmcnt<4 * (NS - 3)>();
-      else
-        wait_vmcnt<0>();
-      stage_barrier();
-      if (s + NS - 1 < nsteps) issue(s + NS - 1, stage(std::integral_constant<int, I + NS - 1>{}));
-      if constexpr (I & 1)
-        readfrag(stage(std::integral_constant<int, I + 1>{}), f0);
-      else
-        readfrag(stage(std::integral_constant<int, I + 1>{}), f1);
-    }
-    if constexpr (I & 1)
-      mma(f1);
-    else
-      mma(f0);
-  };
   issue(0, S0);
   if (1 < nsteps) issue(1, S1);
   if (NS > 3 && 2 < nsteps) issue(2, S2);
   if (NS > 4 && 3 < nsteps) issue(3, S3);
-  if constexpr (PF) {
-    if (NS - 2 < nsteps)
-      wait_vmcnt<4 * (NS - 2)>();
-    else
-      wait_vmcnt<0>();
-    stage_barrier();
-    readfrag(S0, f0);
-    // unrolled by a multiple of NS and of 2, so stage and fragment set are static
-    constexpr int U = NS % 2 ? 2 * NS : NS;
-    for (int s0 = 0; s0 < nsteps; s0 += U)
-      static_for([&](auto ic) { step_pf(ic, s0 + decltype(ic)::value); }, std::make_integer_sequence<int, U>{});
-  } else {
-    for (int s0 = 0; s0 < nsteps; s0 += NS) {
-      step(std::integral_constant<int, 0>{}, s0);
-      step(std::integral_constant<int, 1>{}, s0 + 1);
-      step(std::integral_constant<int, 2>{}, s0 + 2);
-      if constexpr (NS > 3) step(std::integral_constant<int, 3>{}, s0 + 3);
-      if constexpr (NS > 4) step(std::integral_constant<int, 4>{}, s0 + 4);
-    }
+  for (int s0 = 0; s0 < nsteps; s0 += NS) {
+    step(std::integral_constant<int, 0>{}, s0);
+    step(std::integral_constant<int, 1>{}, s0 + 1);
+    step(std::integral_constant<int, 2>{}, s0 + 2);
+    if constexpr (NS > 3) step(std::integral_constant<int, 3>{}, s0 + 3);
+    if constexpr (NS > 4) step(std::integral_constant<int, 4>{}, s0 + 4);
   }
   crt_tile_epilogue(a, mat, row0 + wr, col0 + wc, li, lk, shr, slr, shi, sli);
 }
@@ -1056,13 +983,12 @@ static int crt_big_tb(int D) {
 // tile variants: 0 = the 64 x 64 kernel, else crt_gemm_big<tb> (32 tb x 32 tb tiles)
 struct SplitVariant {
   int tb;  // 0: 64 x 64 workgroup tile; 5, 6: crt_gemm_big<tb>
-  int ns;  // 64 x 64 tile: 0 register-staged (crt_gemm), else crt_gemm_dma<|ns|>, with fragment prefetch if < 0
+  int ns;  // 64 x 64 tile: 0 register-staged (crt_gemm), else crt_gemm_dma<ns>
   const char* name;
 };
-static const SplitVariant kSplitVariants[] = {{0, 0, "crt_64x64"},        {5, 0, "crt_160x160"},
-                                              {6, 0, "crt_192x192"},      {0, 4, "crt_64x64_dma4"},
-                                              {0, 5, "crt_64x64_dma5"},   {0, -4, "crt_64x64_dma4_pf"},
-                                              {0, -5, "crt_64x64_dma5_pf"}};
+static const SplitVariant kSplitVariants[] = {{0, 0, "crt_64x64"},     {5, 0, "crt_160x160"},
+                                              {6, 0, "crt_192x192"},   {0, 4, "crt_64x64_dma4"},
+                                              {0, 5, "crt_64x64_dma5"}};
 constexpr int kNumSplitVariants = sizeof(kSplitVariants) / sizeof(kSplitVariants[0]);
 
 int split_variant_count() { return kNumSplitVariants; }
@@ -1203,13 +1129,11 @@ int launch_bse_split(tups_ctx* c) {
     TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "split grid too large");
     if (int rc = timer_begin(c)) return rc;
     const int ns = split_ns(c);
-    void (*gemm)(CrtArgs) = tb == 6    ? crt_gemm_big<6>
-                            : tb == 5  ? crt_gemm_big<5>
-                            : ns == 4  ? crt_gemm_dma<4, false>
-                            : ns == 5  ? crt_gemm_dma<5, false>
-                            : ns == -4 ? crt_gemm_dma<4, true>
-                            : ns == -5 ? crt_gemm_dma<5, true>
-                                       : crt_gemm;
+    void (*gemm)(CrtArgs) = tb == 6   ? crt_gemm_big<6>
+                            : tb == 5 ? crt_gemm_big<5>
+                            : ns == 4 ? crt_gemm_dma<4>
+                            : ns == 5 ? crt_gemm_dma<5>
+                                      : crt_gemm;
     hipLaunchKernelGGL(gemm, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_SPLIT_GEMM)) return rc;
