@@ -617,6 +617,30 @@ def main():
     print(f"[bench] batching: {bt_ms:.2f} ms per launch", file=sys.stderr, flush=True)
     bt.close()
 
+    # split form of the same uneven batch: error against its FP64 step from the same inputs, then the rate
+    bs = batch_context()
+    bs.BSE()
+    bs.sync()
+    ref = [np.concatenate([x.ravel() for x in bs.download(GAMMA, r)]) for r in range(4)]
+    bs.close()
+    bs = batch_context()
+    bs.BSE_split(args.moduli)
+    bs.sync()
+    got = [np.concatenate([x.ravel() for x in bs.download(GAMMA, r)]) for r in range(4)]
+    berr = [float(np.linalg.norm(got[r] - ref[r]) / np.linalg.norm(ref[r])) for r in range(4)]
+    for _ in range(args.warmup):
+        bs.BSE_split(args.moduli)
+    bs.sync()
+    bs_ms = sum(bs.BSE_split(args.moduli, timed=True) for _ in range(args.steps)) / args.steps
+    batching_split = {"value": round(bs.nq / (bs_ms / 1e3), 2), "unit": "q-points/s", "ms_per_call": round(bs_ms, 3),
+                      "vs_fp64_batch": round(bt_ms / bs_ms, 3), "moduli": args.moduli,
+                      "error_vs_fp64": {"max": max(berr), "per_channel": berr},
+                      "kernel": "crt_prep_b + crt_rowmax_fold + crt_prep_a + crt_gemm_dma<4> / crt_gemm_big<6>, "
+                                "per (Nf_q, Nl_q) class",
+                      "workload": "the batching workload above in the split form"}
+    print(f"[bench] batching split: {bs_ms:.2f} ms per call, error {max(berr):.2e}", file=sys.stderr, flush=True)
+    bs.close()
+
     # uneven-box iteration (BSE + crossing + G = F - G) replayed as one HIP graph
     from tups_amd.profile_workloads import uneven_context
     un = uneven_context()
@@ -770,6 +794,7 @@ def main():
         "phir_nl1": phir1,
         "phir_nl5": phir5,
         "batching": batching,
+        "batching_split": batching_split,
         "large_share": large,
         "sde": sde,
         "sus": sus,

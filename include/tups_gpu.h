@@ -303,6 +303,12 @@ int tups_batch_query(const tups_batch* batch, int64_t* vertex_elems, int64_t* ch
 int tups_batch_upload(tups_batch* batch, int32_t which, int32_t channel, const double* host);
 int tups_batch_download(tups_batch* batch, int32_t which, int32_t channel, double* host);
 int tups_batch_bse(tups_batch* batch, float* ms);
+/* Split (INT8 CRT, `moduli` in 8..16) form of tups_batch_bse: the same update
+ * of every (q, channel) (SRC/parquet_BSE.f90:62-132), the q of one (Nf_q, Nl_q)
+ * class run together as passes of the context's split kernels
+ * (tups_bse(ctx, TUPS_BSE_LADDER_SPLIT)); 12 moduli keep it within about 1e-10
+ * of the FP64 form.  Allocates its residue workspace on first use. */
+int tups_batch_bse_split(tups_batch* batch, int32_t moduli, float* ms);
 int tups_batch_sync(tups_batch* batch);
 /* Project autotuner for the batch: times every tile variant (tile choice per
  * Nl, XCD-striped or plain item order) on the resident batch without changing

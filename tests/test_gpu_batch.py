@@ -176,3 +176,31 @@ def test_batch_rejects_bad_shapes():
     with TupsBatch(4, [8, 4], [5, 1]) as b:
         with pytest.raises(ValueError):
             b.upload_packed(0, 0, np.zeros(3, dtype=np.complex128))
+
+
+@pytest.mark.parametrize(
+    "nx,nf,nl",
+    [(4, [16, 8, 6, 12, 64, 2, 10, 4, 8, 16], [1, 5, 9, 5, 1, 5, 9, 1, 5, 1]),  # classes with one and two q
+     (8, [64, 40, 16, 64, 30, 64], [9, 5, 1, 5, 9, 5])],                         # D 576 (192 tile) .. 16
+)
+def test_batch_split_parity(nx, nf, nl):
+    """tups_batch_bse_split: every q of every (Nf_q, Nl_q) class within the split
+    bar (relative Frobenius 1e-10 at 12 moduli) of the oracle's FP64 update."""
+    from tups_amd import CHI, F, GAMMA, TupsBatch
+
+    _say(f"batch split nx={nx} nf={nf} nl={nl}")
+    rng = np.random.default_rng(sum(nf) + 7)
+    gam, fv, chi = _inputs(rng, nf, nl)
+    exp = _oracle(nx, nf, nl, gam, fv, chi)
+    with TupsBatch(nx, nf, nl, beta=BETA, f_damping=FD) as b:
+        for r in range(4):
+            b.upload(GAMMA, r, gam[r])
+            b.upload(F, r, fv[r])
+        for p in range(2):
+            b.upload(CHI, p, chi[p])
+        b.BSE_split(12)
+        b.sync()
+        got = [b.download(GAMMA, r) for r in range(4)]
+    for r in range(4):
+        for q in range(len(nf)):
+            assert rel_fro(got[r][q], exp[r][q]) < 1e-10, (r, q, nf[q], nl[q])

@@ -566,6 +566,8 @@ extern "C" int tups_batch_destroy(tups_batch* b) {
   f(b->d_nf);
   f(b->d_nl);
   f(b->d_tickets);
+  f(b->d_qlist);
+  f(b->split_ws);
   if (b->stream && b->own_stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return TUPS_OK;

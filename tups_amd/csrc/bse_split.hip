@@ -33,6 +33,7 @@
 // relative Frobenius error, 16 moduli reach FP64 rounding.
 #include "internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <numeric>
@@ -61,6 +62,11 @@ struct CrtArgs {
   double* rmaxp;                // [mat][Dp / 16][Dp] partial row maxima of A per 16-column block
   int* fscale;                  // [mat][Dp] column exponents of B
   int* crt;                     // large-tile GEMM: per-modulus (Re, Im) residue bytes of each tile
+  // batch (null for a context): the q of this (Nf, Nl) class, and the element
+  // offsets of every q's vertex and bubble blocks
+  const int* qlist;
+  const long long* voff;
+  const long long* coff;
   int D, Dp, Kp, nf, nl;
   int q0, nqb, tiles_n;
   int nmod, ta, tb;
@@ -74,6 +80,22 @@ struct CrtArgs {
   unsigned nm24[CRT_MAX];       // (-m) mod 2^24
   double mhi, mlo, inv_mm;
 };
+
+// channel r of workspace matrix mat and the element offsets of its q's vertex
+// block (vo) and bubble blocks (co): q after q for a context, from the offset
+// tables for a batch class
+__device__ __forceinline__ void crt_mat(const CrtArgs& a, int mat, int& r, size_t& vo, size_t& co) {
+  r = mat / a.nqb;
+  const int ql = a.q0 + (mat - r * a.nqb);
+  if (a.qlist) {
+    const int q = a.qlist[ql];
+    vo = (size_t)a.voff[q];
+    co = (size_t)a.coff[q];
+  } else {
+    vo = (size_t)a.D * a.D * ql;
+    co = (size_t)a.nf * a.nl * a.nl * ql;
+  }
+}
 
 // byte offset of inner index k of row (column) rc: a 64-row tile of one K chunk
 // is 4 KiB contiguous
@@ -155,11 +177,12 @@ constexpr int PREP_A_ROW = 80;
 __global__ __launch_bounds__(256) void crt_prep_a(CrtArgs a) {
   __shared__ __attribute__((aligned(16))) int8_t T[2][2][64 * PREP_A_ROW];  // [buffer][sign][row][80]
   const int mat = blockIdx.y;
-  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  int r;
+  size_t vo, co;
+  crt_mat(a, mat, r, vo, co);
   const int D = a.D;
-  const size_t DD = (size_t)D * D;
-  const double2* G = a.gam[r] + DD * q;
-  const double2* F = a.fv[r] + DD * q;
+  const double2* G = a.gam[r] + vo;
+  const double2* F = a.fv[r] + vo;
   const int nkc = a.Kp / 64;
   const int rb = blockIdx.x / nkc, kc = blockIdx.x - rb * nkc;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -221,11 +244,12 @@ template <int NL, bool CL>
 __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
   extern __shared__ double2 colv[];  // [4][Kp] mixed columns, [Dp] row maxima (bits), with CL [l1][l3][nu]
   const int mat = blockIdx.y;
-  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+  int r;
+  size_t vo, co;
+  crt_mat(a, mat, r, vo, co);
   const int D = a.D, nf = a.nf, Kp = a.Kp, Dp = a.Dp;
   constexpr int nl = NL;
-  const size_t DD = (size_t)D * D;
-  const double2* chig = a.chi[r >> 1] + (size_t)nf * nl * nl * q;  // [nu][l3][l1]
+  const double2* chig = a.chi[r >> 1] + co;  // [nu][l3][l1]
   unsigned long long* rmx = reinterpret_cast<unsigned long long*>(colv + 4 * (size_t)Kp);  // [Dp]
   double2* chil = reinterpret_cast<double2*>(rmx + (size_t)Dp);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -238,8 +262,8 @@ __global__ __launch_bounds__(256) void crt_prep_b(CrtArgs a) {
   }
   __syncthreads();
   double2* v = colv + (size_t)wave * Kp;
-  const double2* Fq = a.fv[r] + DD * q;
-  const double2* Gq = a.gam[r] + DD * q;
+  const double2* Fq = a.fv[r] + vo;
+  const double2* Gq = a.gam[r] + vo;
   // loads from a clamped address, unconditionally (the caller zeroes what lies
   // outside), so the registers are written by the loads alone
   auto load_x = [&](int col, int nu, double2 (&xf)[NL], double2 (&xg)[NL]) {
@@ -383,11 +407,12 @@ __device__ __forceinline__ void crt_tile_epilogue(const CrtArgs& a, int mat, int
                                                   const double (&shr)[16], const double (&slr)[16],
                                                   const double (&shi)[16], const double (&sli)[16]) {
   const int D = a.D;
-  const size_t DD = (size_t)D * D;
-  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
-  const double2* Gm = a.gam[r] + DD * q;
-  const double2* Fm = a.fv[r] + DD * q;
-  double2* O = a.out[r] + DD * q;
+  int r;
+  size_t vo, co;
+  crt_mat(a, mat, r, vo, co);
+  const double2* Gm = a.gam[r] + vo;
+  const double2* Fm = a.fv[r] + vo;
+  double2* O = a.out[r] + vo;
   const double cf = a.cf[r];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -836,11 +861,12 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_big(CrtArgs a) {
   // CRT over the scratch, two elements (one dword) at a time; element e of
   // block (i, j) sits at row 4 (lane / 16) + e, column lane % 16
   const int D = a.D;
-  const size_t DD = (size_t)D * D;
-  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
-  const double2* Gm = a.gam[r] + DD * q;
-  const double2* Fm = a.fv[r] + DD * q;
-  double2* O = a.out[r] + DD * q;
+  int r;
+  size_t vo, co;
+  crt_mat(a, mat, r, vo, co);
+  const double2* Gm = a.gam[r] + vo;
+  const double2* Fm = a.fv[r] + vo;
+  double2* O = a.out[r] + vo;
   const double cf = a.cf[r];
   // groups of TB dwords (2 TB elements); the scratch is read four moduli at a
   // time, all 4 TB loads of a chunk issued before any is used (a chunk past
@@ -1030,6 +1056,27 @@ static void (*prep_b_kernel(int nl, size_t Kp, size_t Dp, int nf, size_t* lds))(
   return cl ? crt_prep_b<9, true> : crt_prep_b<9, false>;
 }
 
+// workspace bytes of one matrix: residues of A and B, exponents, row maxima and
+// their partials, large-tile scratch
+static size_t crt_bytes_per_mat(int D, int nmod, int tb) {
+  const size_t Dp = crt_padded(D, tb);
+  const size_t Kp = (2 * (size_t)D + 63) / 64 * 64;
+  return 4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int) + (Dp / PREP_B_COLS + 1) * Dp * sizeof(double) +
+         crt_scratch_per_mat(D, nmod, tb) + 16;
+}
+
+// a pass of `mats` matrices laid out in the workspace at base
+static void crt_workspace(CrtArgs& a, void* base_v, size_t mats, int nmod, size_t Dp, size_t Kp) {
+  int8_t* base = static_cast<int8_t*>(base_v);
+  a.A = base;
+  a.B = base + mats * 2 * nmod * Dp * Kp;
+  a.rmax = reinterpret_cast<double*>(a.B + mats * 2 * nmod * Dp * Kp);  // 8-byte aligned: Kp % 64 == 0
+  a.rmaxp = a.rmax + mats * Dp;
+  a.escale = reinterpret_cast<int*>(a.rmaxp + mats * (Dp / PREP_B_COLS) * Dp);
+  a.fscale = a.escale + mats * Dp;
+  a.crt = reinterpret_cast<int*>((reinterpret_cast<uintptr_t>(a.fscale + mats * Dp) + 15) / 16 * 16);
+}
+
 // q points per pass and workspace of the split ladder; allocates (outside any
 // stream capture) when the workspace is too small
 int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
@@ -1041,8 +1088,7 @@ int bse_split_reserve(tups_ctx* c, int* qbatch_out) {
   const size_t Dp = crt_padded(s.nz, tb);
   const size_t Kp = (2 * (size_t)s.nz + 63) / 64 * 64;
   // per q point: 4 channels x (A and B) x 2 signs x nmod residue planes + exponents + GEMM scratch
-  const size_t per_q = 4 * (4 * (size_t)nmod * Dp * Kp + 2 * Dp * sizeof(int) +
-                            (Dp / PREP_B_COLS + 1) * Dp * sizeof(double) + crt_scratch_per_mat(s.nz, nmod, tb) + 16);
+  const size_t per_q = 4 * crt_bytes_per_mat(s.nz, nmod, tb);
   // default pass: about 4 GiB of residues, at most an eighth of the free memory
   // (fewer, larger passes measured 3-5% faster than 1 GiB ones); fixed at
   // first use, so a later call (possibly under stream capture) finds the same
@@ -1107,14 +1153,7 @@ int launch_bse_split(tups_ctx* c) {
     a.q0 = q0;
     a.nqb = std::min(qbatch, s.nq - q0);
     const size_t mats = 4 * (size_t)a.nqb;
-    int8_t* base = static_cast<int8_t*>(c->split_ws);
-    a.A = base;
-    a.B = base + mats * 2 * nmod * Dp * Kp;
-    a.rmax = reinterpret_cast<double*>(a.B + mats * 2 * nmod * Dp * Kp);  // 8-byte aligned: Kp % 64 == 0
-    a.rmaxp = a.rmax + mats * Dp;
-    a.escale = reinterpret_cast<int*>(a.rmaxp + mats * (Dp / PREP_B_COLS) * Dp);
-    a.fscale = a.escale + mats * Dp;
-    a.crt = reinterpret_cast<int*>((reinterpret_cast<uintptr_t>(a.fscale + mats * Dp) + 15) / 16 * 16);
+    crt_workspace(a, c->split_ws, mats, nmod, Dp, Kp);
     if (int rc = timer_begin(c)) return rc;
     hipLaunchKernelGGL(prep_b, dim3((unsigned)(Dp / PREP_B_COLS), (unsigned)mats), dim3(256), lds_b, c->stream, a);
     TUPS_HIP(hipGetLastError());
@@ -1140,6 +1179,141 @@ int launch_bse_split(tups_ctx* c) {
   }
   return TUPS_OK;
 }
+
+// ---- split form of a batch -------------------------------------------------------------
+//
+// The q of one (Nf, Nl) class share D, so each class runs as passes of the
+// uniform kernels above; the kernels find a q's blocks through the batch's
+// offset tables (CrtArgs::qlist, voff, coff).  Classes go largest D first,
+// each with the GEMM tile its D pads to best (the 192 tile within 7%, else the
+// 64 tile with four DMA stages).
+static int batch_split_classes(tups_batch* b) {
+  if (b->d_qlist) return TUPS_OK;
+  std::vector<std::pair<int, int>> keys;  // (nf, nl)
+  for (int q = 0; q < b->nq; ++q) {
+    const std::pair<int, int> k{b->nf[q], b->nl[q]};
+    if (std::find(keys.begin(), keys.end(), k) == keys.end()) keys.push_back(k);
+  }
+  std::sort(keys.begin(), keys.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) {
+    return x.first * x.second > y.first * y.second;
+  });
+  b->qlist.clear();
+  b->class_start.clear();
+  for (const auto& k : keys) {
+    b->class_start.push_back((int)b->qlist.size());
+    for (int q = 0; q < b->nq; ++q)
+      if (b->nf[q] == k.first && b->nl[q] == k.second) b->qlist.push_back(q);
+  }
+  b->class_start.push_back((int)b->qlist.size());
+  TUPS_HIP(hipMalloc((void**)&b->d_qlist, b->qlist.size() * sizeof(int)));
+  TUPS_HIP(hipMemcpy(b->d_qlist, b->qlist.data(), b->qlist.size() * sizeof(int), hipMemcpyHostToDevice));
+  return TUPS_OK;
+}
+
+int batch_split_launch(tups_batch* b, int nmod) {
+  TUPS_CHECK(nmod >= 8 && nmod <= CRT_MAX, TUPS_EINVAL, "split moduli must be in 8..16");
+  if (int rc = batch_split_classes(b)) return rc;
+  const int ncls = (int)b->class_start.size() - 1;
+  // passes of about 4 GiB of residues; the workspace holds the largest
+  size_t need = 0;
+  std::vector<int> qbatch(ncls);
+  for (int k = 0; k < ncls; ++k) {
+    const int q = b->qlist[b->class_start[k]], D = b->nf[q] * b->nl[q];
+    TUPS_CHECK(D <= 1024, TUPS_EINVAL, "split path needs D <= 1024 (exact INT32 sums)");
+    const size_t per_q = 4 * crt_bytes_per_mat(D, nmod, crt_big_tb(D));
+    const int cnt = b->class_start[k + 1] - b->class_start[k];
+    qbatch[k] = (int)std::max<size_t>(1, std::min<size_t>(cnt, ((size_t)4 << 30) / per_q));
+    need = std::max(need, per_q * qbatch[k]);
+  }
+  if (b->split_ws_bytes < need) {
+    if (b->split_ws) TUPS_HIP(hipFree(b->split_ws));
+    b->split_ws = nullptr;
+    b->split_ws_bytes = 0;
+    TUPS_HIP(hipMalloc(&b->split_ws, need));
+    b->split_ws_bytes = need;
+  }
+  for (int k = 0; k < ncls; ++k) {
+    const int q = b->qlist[b->class_start[k]];
+    const int nf = b->nf[q], nl = b->nl[q], D = nf * nl;
+    const int tb = crt_big_tb(D);
+    const size_t Dp = crt_padded(D, tb);
+    const size_t Kp = (2 * (size_t)D + 63) / 64 * 64;
+    CrtArgs a{};
+    for (int r = 0; r < 4; ++r) {
+      a.gam[r] = b->gamma[r];
+      a.fv[r] = b->fvert[r];
+      a.out[r] = b->spare[r];
+    }
+    a.chi[0] = b->chi[0];
+    a.chi[1] = b->chi[1];
+    a.qlist = b->d_qlist + b->class_start[k];
+    a.voff = b->d_voff;
+    a.coff = b->d_coff;
+    a.D = D;
+    a.Dp = (int)Dp;
+    a.Kp = (int)Kp;
+    a.nf = nf;
+    a.nl = nl;
+    a.tiles_n = (int)(Dp / (tb > 0 ? 32 * tb : CRT_TILE));
+    a.alpha = b->alpha;
+    const double cr[4] = {0.5, 0.5, -0.25, 0.25};
+    for (int r = 0; r < 4; ++r) a.cf[r] = cr[r] * b->f_damping;
+    a.omf = 1.0 - b->f_damping;
+    crt_constants(a, nmod, D);
+    size_t lds_b = 0;
+    void (*prep_b)(CrtArgs) = prep_b_kernel(nl, Kp, Dp, nf, &lds_b);
+    if (lds_b > 65536)
+      TUPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(prep_b),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    void (*gemm)(CrtArgs) = tb == 6 ? crt_gemm_big<6> : crt_gemm_dma<4>;
+    const int cnt = b->class_start[k + 1] - b->class_start[k];
+    for (int q0 = 0; q0 < cnt; q0 += qbatch[k]) {
+      a.q0 = q0;
+      a.nqb = std::min(qbatch[k], cnt - q0);
+      const size_t mats = 4 * (size_t)a.nqb;
+      crt_workspace(a, b->split_ws, mats, nmod, Dp, Kp);
+      hipLaunchKernelGGL(prep_b, dim3((unsigned)(Dp / PREP_B_COLS), (unsigned)mats), dim3(256), lds_b, b->stream,
+                         a);
+      TUPS_HIP(hipGetLastError());
+      hipLaunchKernelGGL(crt_rowmax_fold, dim3((unsigned)((Dp + 255) / 256), (unsigned)mats), dim3(256), 0,
+                         b->stream, a);
+      TUPS_HIP(hipGetLastError());
+      hipLaunchKernelGGL(crt_prep_a, dim3((unsigned)((Dp / 64) * (Kp / 64)), (unsigned)mats), dim3(256), 0,
+                         b->stream, a);
+      TUPS_HIP(hipGetLastError());
+      const long long nblk = (long long)a.tiles_n * a.tiles_n * mats;
+      TUPS_CHECK(nblk < (1LL << 31), TUPS_EINVAL, "split grid too large");
+      hipLaunchKernelGGL(gemm, dim3((unsigned)nblk), dim3(256), 0, b->stream, a);
+      TUPS_HIP(hipGetLastError());
+    }
+  }
+  return TUPS_OK;
+}
+
+}  // namespace tups
+
+extern "C" int tups_batch_bse_split(tups_batch* b, int32_t moduli, float* ms) {
+  using namespace tups;
+  TUPS_CHECK(b, TUPS_EINVAL, "null batch");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms) {
+    TUPS_HIP(hipEventCreate(&e0));
+    TUPS_HIP(hipEventCreate(&e1));
+    TUPS_HIP(hipEventRecord(e0, b->stream));
+  }
+  if (int rc = batch_split_launch(b, moduli)) return rc;
+  if (ms) {
+    TUPS_HIP(hipEventRecord(e1, b->stream));
+    TUPS_HIP(hipEventSynchronize(e1));
+    TUPS_HIP(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  for (int r = 0; r < 4; ++r) std::swap(b->gamma[r], b->spare[r]);
+  return TUPS_OK;
+}
+
+namespace tups {
 
 // Times one split step (preparation + GEMM, outputs to the ping-pong buffer) per
 // tile variant on the resident inputs after a warm-up, keeps the fastest.

@@ -271,4 +271,10 @@ struct tups_batch {
   int len = 0;     // entries per XCD list, padding included
   int variant = 0;
   bool three_mult = false;            // the autotuner may pick the three-multiplication product forms
+  // split form (tups_batch_bse_split): q grouped by (Nf, Nl) class, largest D first
+  std::vector<int> qlist;             // q of class 0, then class 1, ...
+  std::vector<int> class_start;       // class k holds qlist[class_start[k] .. class_start[k + 1])
+  int* d_qlist = nullptr;
+  void* split_ws = nullptr;
+  size_t split_ws_bytes = 0;
 };

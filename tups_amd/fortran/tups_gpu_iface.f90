@@ -212,6 +212,14 @@ module tups_gpu_iface
       integer(c_int) :: rc
     end function tups_batch_bse
 
+    function tups_batch_bse_split(batch, moduli, ms) bind(C, name="tups_batch_bse_split") result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: moduli
+      type(c_ptr), value :: ms
+      integer(c_int) :: rc
+    end function tups_batch_bse_split
+
     function tups_batch_sync(batch) bind(C, name="tups_batch_sync") result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: batch

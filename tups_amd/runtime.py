@@ -105,6 +105,7 @@ SIGNATURES = {
     "tups_batch_upload": ([_V, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_batch_download": ([_V, C.c_int32, C.c_int32, _D], C.c_int),
     "tups_batch_bse": ([_V, _P(C.c_float)], C.c_int),
+    "tups_batch_bse_split": ([_V, C.c_int32, _P(C.c_float)], C.c_int),
     "tups_batch_sync": ([_V], C.c_int),
     "tups_batch_autotune": ([_V, _I, _P(C.c_float)], C.c_int),
     "tups_batch_set_variant": ([_V, C.c_int32], C.c_int),
@@ -649,6 +650,13 @@ class TupsBatch:
         """The damped ladder of every (q, channel) in one persistent launch."""
         ms = C.c_float()
         _check(self.lib.tups_batch_bse(self._h, C.byref(ms) if timed else None), "tups_batch_bse")
+        return float(ms.value) if timed else None
+
+    def BSE_split(self, moduli: int = 12, timed: bool = False):
+        """The same update in the split (INT8 CRT) form, class by class of (Nf_q, Nl_q)."""
+        ms = C.c_float()
+        _check(self.lib.tups_batch_bse_split(self._h, moduli, C.byref(ms) if timed else None),
+               "tups_batch_bse_split")
         return float(ms.value) if timed else None
 
     def sync(self):
