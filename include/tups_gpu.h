@@ -369,9 +369,14 @@ int tups_autotune(tups_ctx* ctx, int32_t kernel, int32_t* variant, float* ms);
  * product variants only.  tups_set_variant can still force any variant. */
 int tups_set_product_forms(tups_ctx* ctx, int32_t allow_three);
 const char* tups_variant_name(int32_t kernel, int32_t variant);
-/* forces a variant id (must match the context's Nl); -1 restores the default */
+/* forces a variant id (must match the context's Nl); -1 restores the default.
+ * TUPS_K_BSE_CLOSED and TUPS_K_BSE_IR choose the elimination of the closed
+ * forms: 0 "gj_panel16" (16-column panels), 1 "gj_block64" (64-column blocks,
+ * each passing the later columns through memory once; the default from
+ * D = 128 on) */
 int tups_set_variant(tups_ctx* ctx, int32_t kernel, int32_t variant);
-/* number of variant ids of `kernel` (ids 0 .. n-1; each serves one Nl) */
+/* number of variant ids of `kernel` (ids 0 .. n-1; a ladder or crossing
+ * variant serves one Nl, a split-GEMM or elimination variant every Nl) */
 int tups_variant_count(int32_t kernel);
 
 /* --- stream control and timing */

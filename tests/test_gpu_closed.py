@@ -29,21 +29,24 @@ def _say(*a):
     print(*a, flush=True, file=sys.stderr)
 
 
+@pytest.mark.parametrize("variant", [0, 1])  # panels of 16, blocks of 64
 @pytest.mark.parametrize(
     "nx,nf,nl,nq",
     [(4, 16, 1, 0),     # Nz=16: one panel
      (4, 8, 5, 0),      # Nz=40: partial last panel (8 columns)
      (4, 6, 5, 3),      # Nz=30: panel of 14, odd tile edges
-     (4, 8, 9, 2),      # Nz=72: two update tiles
-     (8, 64, 5, 2)],    # Nz=320: the batched-BSE shape
+     (4, 8, 9, 2),      # Nz=72: two update tiles, a block of 64 and one of 8
+     (8, 64, 5, 2)],    # Nz=320: the batched-BSE shape, five blocks
 )
-def test_closed_form_parity(nx, nf, nl, nq):
+def test_closed_form_parity(nx, nf, nl, nq, variant):
     from tups_amd import BSE_CLOSED_LU, CHI, F, GAMMA
+    from tups_amd.runtime import K_BSE_CLOSED
 
-    _say(f"closed parity nx={nx} nf={nf} nl={nl} nq={nq}")
+    _say(f"closed parity nx={nx} nf={nf} nl={nl} nq={nq} variant={variant}")
     rng = np.random.default_rng(nx * 31 + nf + nl)
     lat = Lattice(nx, nf, nl)
     with _ctx(nx=nx, nf=nf, nl=nl, nq=nq, beta=2.0, f_damping=0.5) as c:
+        c.set_variant(variant, K_BSE_CLOSED)
         gam = random_vertices(rng, c.nz, c.nq, diag=2.0, scale=0.3)
         chic = [random_chi_compact(rng, nl, nf, c.nq, 3.0) for _ in range(2)]
         for r in range(4):
@@ -100,21 +103,25 @@ def test_closed_form_reports_singular_system():
             c.BSE(0, mode=BSE_CLOSED_LU)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize(
     "nx,nf,nl,nq",
     [(4, 16, 1, 0), (4, 8, 5, 0), (4, 6, 5, 3), (4, 8, 9, 2), (8, 64, 5, 2)],
 )
-def test_refined_closed_form_parity(nx, nf, nl, nq):
+def test_refined_closed_form_parity(nx, nf, nl, nq, variant):
     """FP32 LU (Gauss-Jordan inverse) + FP64 iterative refinement: the residual
     |Gamma - A X|_F / |Gamma|_F reaches 1e-12 for every (q, r) and X matches the
     oracle's FP64 LAPACK solve to cond(A) times that; the FP32 inverse alone
     (no refinement step allowed) does not reach the tolerance and is reported."""
     from tups_amd import BSE_CLOSED_IR, CHI, F, GAMMA, TupsError
 
-    _say(f"refined closed parity nx={nx} nf={nf} nl={nl} nq={nq}")
+    from tups_amd.runtime import K_BSE_IR
+
+    _say(f"refined closed parity nx={nx} nf={nf} nl={nl} nq={nq} variant={variant}")
     rng = np.random.default_rng(nx * 31 + nf + nl)
     lat = Lattice(nx, nf, nl)
     with _ctx(nx=nx, nf=nf, nl=nl, nq=nq, beta=2.0, f_damping=0.5) as c:
+        c.set_variant(variant, K_BSE_IR)
         gam = random_vertices(rng, c.nz, c.nq, diag=2.0, scale=0.3)
         chic = [random_chi_compact(rng, nl, nf, c.nq, 3.0) for _ in range(2)]
         for p in range(2):

@@ -43,8 +43,9 @@ namespace tups {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int CL_NB = 16;  // panel width
+constexpr int CL_NB = 16;  // panel width (closed_solve), sub-panel width (closed_solve_b64)
 constexpr int CL_T = 64;   // update tile (rows and columns)
+constexpr int CL_BB = 64;  // block width of closed_solve_b64
 
 __device__ __forceinline__ double2 mkc(double x, double y) { return make_double2(x, y); }
 __device__ __forceinline__ float2 mkc(float x, float y) { return make_float2(x, y); }
@@ -103,6 +104,8 @@ struct ClosedArgs {
   double2* out[4];
   const double2* chi[2];
   void* W;      // [mat][2D columns][D rows], column-major: double2 (direct) or float2 (inverse)
+  void* S;      // closed_solve_b64: [mat][CL_BB columns][D rows], the block column being factored
+  void* T;      // closed_solve_b64: [mat][3][CL_BB][CL_BB], L11^-1, U11^-1 and their product
   int* info;    // first failing pivot (1-based column) over all matrices, 0 if none
   int D, nf, nl, q0, nqb;
   double kappa[4];
@@ -165,8 +168,9 @@ __global__ __launch_bounds__(256) void ir_build32(ClosedArgs a) {
 }
 
 // Z(r, c) -= sum_t W(r, k0 + t) W(k0 + t, c) for r in [r0, r1), c in [c0, c1),
-// t < nb, by 64 x 64 tiles; four waves of 2 x 2 blocks of 16 x 16 x 4 MFMAs.
-// A fragments: lane l holds A[l % 16][l / 16]; B: B[l / 16][l % 16].
+// t < nb (nb <= CL_BB), by 64 x 64 tiles held in registers while the inner
+// index passes through LDS CL_NB at a time; four waves of 2 x 2 blocks of
+// 16 x 16 x 4 MFMAs.  A fragments: lane l holds A[l % 16][l / 16]; B: B[l / 16][l % 16].
 template <class R>
 __device__ void closed_update(typename Prec<R>::C* W, int D, int r0, int r1, int c0, int c1, int k0, int nb,
                               typename Prec<R>::C (*As)[CL_T + 1], typename Prec<R>::C (*Bs)[CL_T + 1]) {
@@ -179,13 +183,6 @@ __device__ void closed_update(typename Prec<R>::C* W, int D, int r0, int r1, int
   const C zero = mkc((R)0, (R)0);
   for (int tr = r0; tr < r1; tr += CL_T)
     for (int tc = c0; tc < c1; tc += CL_T) {
-      for (int u = tid; u < CL_NB * CL_T; u += 256) {
-        const int row = u & (CL_T - 1), ka = u >> 6;   // A: rows contiguous
-        As[ka][row] = (tr + row < r1 && ka < nb) ? W[(size_t)(k0 + ka) * D + tr + row] : zero;
-        const int kb = u % CL_NB, col = u / CL_NB;     // B: k contiguous in a column
-        Bs[kb][col] = (tc + col < c1 && kb < nb) ? W[(size_t)(tc + col) * D + k0 + kb] : zero;
-      }
-      __syncthreads();
       V ar[2][2], ai[2][2];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -198,6 +195,75 @@ __device__ void closed_update(typename Prec<R>::C* W, int D, int r0, int r1, int
             ar[i][j][e] = z.x;
             ai[i][j][e] = z.y;
           }
+      for (int kc = 0; kc < nb; kc += CL_NB) {
+        for (int u = tid; u < CL_NB * CL_T; u += 256) {
+          const int row = u & (CL_T - 1), ka = u >> 6;  // A: rows contiguous
+          As[ka][row] = (tr + row < r1 && kc + ka < nb) ? W[(size_t)(k0 + kc + ka) * D + tr + row] : zero;
+          const int kb = u % CL_NB, col = u / CL_NB;    // B: k contiguous in a column
+          Bs[kb][col] = (tc + col < c1 && kc + kb < nb) ? W[(size_t)(tc + col) * D + k0 + kc + kb] : zero;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < CL_NB; kk += 4) {
+          C fa[2], fb[2];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) fa[i] = As[kk + lk][wr + i * 16 + li];
+#pragma unroll
+          for (int j = 0; j < 2; ++j) fb[j] = Bs[kk + lk][wc + j * 16 + li];
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              // Z - A B: re -= ar br - ai bi, im -= ar bi + ai br
+              ar[i][j] = P::fma_sub(fa[i].x, fb[j].x, ar[i][j]);
+              ar[i][j] = P::fma_add(fa[i].y, fb[j].y, ar[i][j]);
+              ai[i][j] = P::fma_sub(fa[i].x, fb[j].y, ai[i][j]);
+              ai[i][j] = P::fma_sub(fa[i].y, fb[j].x, ai[i][j]);
+            }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
+            if (row < r1 && col < c1) W[(size_t)col * D + row] = mkc(ar[i][j][e], ai[i][j][e]);
+          }
+    }
+  __syncthreads();  // the updated Z is visible to the whole workgroup
+}
+
+// Z(k0 + i, c) <- sum_t T(i, t) Z(k0 + t, c) for i, t < nb (nb <= CL_BB) and every
+// column c in [c0, c1): T column-major with leading dimension CL_BB.  Each 64 x
+// 64 tile of Z is read whole (through LDS, CL_NB rows at a time) before its
+// result is written back, so the product runs in place.
+template <class R>
+__device__ void closed_apply_T(typename Prec<R>::C* W, const typename Prec<R>::C* T, int D, int k0, int nb, int c0,
+                               int c1, typename Prec<R>::C (*As)[CL_T + 1], typename Prec<R>::C (*Bs)[CL_T + 1]) {
+  using P = Prec<R>;
+  using C = typename P::C;
+  using V = typename P::V;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const C zero = mkc((R)0, (R)0);
+  for (int tc = c0; tc < c1; tc += CL_T) {
+    V ar[2][2], ai[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) ar[i][j] = ai[i][j] = V{};
+    for (int kc = 0; kc < nb; kc += CL_NB) {
+      for (int u = tid; u < CL_NB * CL_T; u += 256) {
+        const int row = u & (CL_T - 1), ka = u >> 6;
+        As[ka][row] = (row < nb && kc + ka < nb) ? T[(size_t)(kc + ka) * CL_BB + row] : zero;
+        const int kb = u % CL_NB, col = u / CL_NB;
+        Bs[kb][col] = (tc + col < c1 && kc + kb < nb) ? W[(size_t)(tc + col) * D + k0 + kc + kb] : zero;
+      }
+      __syncthreads();
 #pragma unroll
       for (int kk = 0; kk < CL_NB; kk += 4) {
         C fa[2], fb[2];
@@ -209,24 +275,26 @@ __device__ void closed_update(typename Prec<R>::C* W, int D, int r0, int r1, int
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
-            // Z - A B: re -= ar br - ai bi, im -= ar bi + ai br
-            ar[i][j] = P::fma_sub(fa[i].x, fb[j].x, ar[i][j]);
-            ar[i][j] = P::fma_add(fa[i].y, fb[j].y, ar[i][j]);
-            ai[i][j] = P::fma_sub(fa[i].x, fb[j].y, ai[i][j]);
-            ai[i][j] = P::fma_sub(fa[i].y, fb[j].x, ai[i][j]);
+            // + A B: re += ar br - ai bi, im += ar bi + ai br
+            ar[i][j] = P::fma_add(fa[i].x, fb[j].x, ar[i][j]);
+            ar[i][j] = P::fma_sub(fa[i].y, fb[j].y, ar[i][j]);
+            ai[i][j] = P::fma_add(fa[i].x, fb[j].y, ai[i][j]);
+            ai[i][j] = P::fma_add(fa[i].y, fb[j].x, ai[i][j]);
           }
       }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
-            if (row < r1 && col < c1) W[(size_t)col * D + row] = mkc(ar[i][j][e], ai[i][j][e]);
-          }
       __syncthreads();
     }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
+          if (row < nb && col < c1) W[(size_t)col * D + k0 + row] = mkc(ar[i][j][e], ai[i][j][e]);
+        }
+  }
+  __syncthreads();
 }
 
 // Z(k0 + i, c) <- sum_t M[i][t] Z(k0 + t, c) for i, t < nb and every column c in
@@ -360,6 +428,174 @@ __global__ __launch_bounds__(256, 2) void closed_solve(ClosedArgs a) {
     panel_rows_apply<R>(W, D, k0, nb, j0, D2, Ui);
     __syncthreads();
     // 5. rows above the panel
+    closed_update<R>(W, D, 0, k0, j0, D2, k0, nb, As, Bs);
+  }
+  if (info && tid == 0) atomicCAS(a.info, 0, info);
+  if constexpr (PHI) {
+    // Phi = X - Gamma
+    const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
+    const double2* G = a.gam[r] + (size_t)D * D * q;
+    double2* O = a.out[r] + (size_t)D * D * q;
+    for (size_t u = tid; u < (size_t)D * D; u += 256) O[u] = csub(W[(size_t)D * D + u], G[u]);
+  }
+}
+
+// Block Gauss-Jordan with CL_BB-wide blocks, one workgroup per (q, r) matrix.
+// For the block column k0 .. k0 + nb:
+//  A. its rows k0 .. D - 1 are copied to S and factored there (LU with partial
+//     pivoting, CL_NB-wide sub-panels: column-by-column within a sub-panel,
+//     then the sub-panel's rank-CL_NB update of the rest of the block); each
+//     row swap is applied to W from column k0 on, so W keeps the permuted
+//     original block column;
+//  B. T = A11^-1 = U11^-1 L11^-1 from the factors (nb x nb);
+//  C. the block rows of the later columns become X = T Z1, in place;
+//  D. every other row r: Z(r, .) -= W(r, k0 .. k0 + nb) X (rank-nb MFMA updates).
+// Each block thus passes the later columns through memory once, where the
+// CL_NB-wide form (closed_solve) passes them once per CL_NB columns.
+template <class R, bool PHI>
+__global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
+  using C = typename Prec<R>::C;
+  __shared__ C As[CL_NB][CL_T + 1];
+  __shared__ C Bs[CL_NB][CL_T + 1];
+  __shared__ C Ls[CL_NB][CL_NB];
+  __shared__ C Li[CL_NB][CL_NB];
+  __shared__ R red_v[256];
+  __shared__ int red_i[256];
+  const int mat = blockIdx.x;
+  const int D = a.D, D2 = 2 * D;
+  C* W = static_cast<C*>(a.W) + (size_t)mat * 2 * D * D;
+  C* S = static_cast<C*>(a.S) + (size_t)mat * CL_BB * D;
+  C* Linv = static_cast<C*>(a.T) + (size_t)mat * 3 * CL_BB * CL_BB;  // column-major, leading dimension CL_BB
+  C* Uinv = Linv + CL_BB * CL_BB;
+  C* T = Uinv + CL_BB * CL_BB;
+  const int tid = threadIdx.x;
+  const C one = mkc((R)1, (R)0), zero = mkc((R)0, (R)0);
+  int info = 0;
+  for (int k0 = 0; k0 < D; k0 += CL_BB) {
+    const int nb = min(CL_BB, D - k0);
+    // S(row, col) for col in [k0, k0 + nb): Sk[col * D + row]
+    C* Sk = S - (size_t)k0 * D;
+    // A. copy, then factor
+    for (int u = tid; u < nb * (D - k0); u += 256) {
+      const int col = k0 + u / (D - k0), row = k0 + u % (D - k0);
+      Sk[(size_t)col * D + row] = W[(size_t)col * D + row];
+    }
+    __syncthreads();
+    for (int s0 = k0; s0 < k0 + nb; s0 += CL_NB) {
+      const int sb = min(CL_NB, k0 + nb - s0);
+      for (int c = s0; c < s0 + sb; ++c) {
+        R best = -1;
+        int bi = c;
+        for (int row = c + tid; row < D; row += 256) {
+          const C v = Sk[(size_t)c * D + row];
+          const R m = fabs(v.x) + fabs(v.y);
+          if (m > best) {
+            best = m;
+            bi = row;
+          }
+        }
+        red_v[tid] = best;
+        red_i[tid] = bi;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+          if (tid < st) {
+            const R o = red_v[tid + st];
+            const int oi = red_i[tid + st];
+            if (o > red_v[tid] || (o == red_v[tid] && oi < red_i[tid])) {
+              red_v[tid] = o;
+              red_i[tid] = oi;
+            }
+          }
+          __syncthreads();
+        }
+        const int p = red_i[0];
+        const R pmax = red_v[0];
+        __syncthreads();
+        if (!(pmax > 0)) {  // zero or NaN pivot column: the system is singular
+          if (info == 0) info = c + 1;
+          continue;
+        }
+        if (p != c) {
+          for (int col = k0 + tid; col < D2; col += 256) {
+            const C x = W[(size_t)col * D + c];
+            W[(size_t)col * D + c] = W[(size_t)col * D + p];
+            W[(size_t)col * D + p] = x;
+          }
+          for (int col = k0 + tid; col < k0 + nb; col += 256) {
+            const C x = Sk[(size_t)col * D + c];
+            Sk[(size_t)col * D + c] = Sk[(size_t)col * D + p];
+            Sk[(size_t)col * D + p] = x;
+          }
+        }
+        __syncthreads();
+        const C inv = cinv(Sk[(size_t)c * D + c]);
+        for (int row = c + 1 + tid; row < D; row += 256) {
+          const C l = cmul(Sk[(size_t)c * D + row], inv);
+          Sk[(size_t)c * D + row] = l;
+          for (int col = c + 1; col < s0 + sb; ++col)
+            Sk[(size_t)col * D + row] = csub(Sk[(size_t)col * D + row], cmul(l, Sk[(size_t)col * D + c]));
+        }
+        __syncthreads();
+      }
+      if (s0 + sb < k0 + nb) {
+        // the rest of the block: U12 = L11^-1 S12 on the sub-panel rows, S22 -= L21 U12 below
+        for (int u = tid; u < CL_NB * CL_NB; u += 256) {
+          const int i = u % CL_NB, j = u / CL_NB;
+          Ls[i][j] = (i < sb && j < sb) ? Sk[(size_t)(s0 + j) * D + s0 + i] : zero;
+        }
+        __syncthreads();
+        if (tid < CL_NB) {
+          const int t = tid;
+          for (int i = 0; i < CL_NB; ++i) {
+            C x = i == t ? one : zero;
+            if (i < sb && t < sb)
+              for (int s2 = t; s2 < i; ++s2) x = csub(x, cmul(Ls[i][s2], Li[s2][t]));
+            Li[i][t] = x;
+          }
+        }
+        __syncthreads();
+        panel_rows_apply<R>(Sk, D, s0, sb, s0 + sb, k0 + nb, Li);
+        __syncthreads();
+        closed_update<R>(Sk, D, s0 + sb, D, s0 + sb, k0 + nb, s0, sb, As, Bs);
+      }
+    }
+    // B. inverses of the unit lower L11 and the upper U11 (identity-padded to
+    // CL_BB), thread t < CL_BB forming column t of each, then T = U11^-1 L11^-1
+    if (tid < CL_BB) {
+      const int t = tid;
+      for (int i = 0; i < CL_BB; ++i) {
+        C x = i == t ? one : zero;
+        if (i < nb && t < nb)
+          for (int s2 = t; s2 < i; ++s2) x = csub(x, cmul(Sk[(size_t)(k0 + s2) * D + k0 + i], Linv[s2 + CL_BB * t]));
+        Linv[i + CL_BB * t] = x;
+      }
+      for (int i = CL_BB - 1; i >= 0; --i) {
+        C x = i == t ? one : zero;
+        if (i < nb && t < nb) {
+          for (int s2 = i + 1; s2 <= t; ++s2)
+            x = csub(x, cmul(Sk[(size_t)(k0 + s2) * D + k0 + i], Uinv[s2 + CL_BB * t]));
+          x = cmul(x, cinv(Sk[(size_t)(k0 + i) * D + k0 + i]));
+        }
+        Uinv[i + CL_BB * t] = x;
+      }
+    }
+    __syncthreads();
+    for (int u = tid; u < CL_BB * CL_BB; u += 256) {
+      const int i = u % CL_BB, j = u / CL_BB;
+      R xr = 0, xi = 0;
+      for (int s2 = 0; s2 < CL_BB; ++s2) {
+        const C ui = Uinv[i + CL_BB * s2], lv = Linv[s2 + CL_BB * j];
+        xr = fma(ui.x, lv.x, fma(-ui.y, lv.y, xr));
+        xi = fma(ui.x, lv.y, fma(ui.y, lv.x, xi));
+      }
+      T[u] = mkc(xr, xi);
+    }
+    __syncthreads();
+    const int j0 = k0 + nb;
+    // C. block rows of the later columns: X = T Z1
+    closed_apply_T<R>(W, T, D, k0, nb, j0, D2, As, Bs);
+    // D. rows below and above the block
+    closed_update<R>(W, D, k0 + nb, D, j0, D2, k0, nb, As, Bs);
     closed_update<R>(W, D, 0, k0, j0, D2, k0, nb, As, Bs);
   }
   if (info && tid == 0) atomicCAS(a.info, 0, info);
@@ -588,13 +824,31 @@ static void closed_common(tups_ctx* c, ClosedArgs& a) {
   for (int r = 0; r < 4; ++r) a.kappa[r] = 2.0 * cr[r] * s.alpha;
 }
 
+// elimination variants: 0 = CL_NB-wide panels (closed_solve), 1 = CL_BB-wide
+// blocks (closed_solve_b64); -1 = the block form from D = 128 on
+static const char* const kClosedVariants[] = {"gj_panel16", "gj_block64"};
+int closed_variant_count() { return 2; }
+const char* closed_variant_name(int v) { return v >= 0 && v < 2 ? kClosedVariants[v] : ""; }
+static bool closed_blocked(const tups_ctx* c) {
+  return c->closed_variant == 1 || (c->closed_variant < 0 && c->sz.nz >= 128);
+}
+// scratch of closed_solve_b64 per matrix, in complex elements
+static size_t closed_b64_elems(size_t D) { return (size_t)CL_BB * D + 3 * (size_t)CL_BB * CL_BB; }
+
 int launch_bse_closed(tups_ctx* c) {
   const tups_sizes& s = c->sz;
   const size_t D = s.nz;
+  const bool blk = closed_blocked(c);
+  const size_t per_mat = (2 * D * D + (blk ? closed_b64_elems(D) : 0)) * sizeof(double2);
   int qbatch = 0;
-  if (int rc = closed_reserve(c, 4 * 2 * D * D * sizeof(double2), 0, &qbatch)) return rc;
+  if (int rc = closed_reserve(c, 4 * per_mat, 0, &qbatch)) return rc;
   ClosedArgs a{};
   closed_common(c, a);
+  if (blk) {
+    const size_t mats_max = 4 * (size_t)qbatch;
+    a.S = static_cast<double2*>(a.W) + mats_max * 2 * D * D;
+    a.T = static_cast<double2*>(a.S) + mats_max * CL_BB * D;
+  }
   hipLaunchKernelGGL(closed_clear, dim3(1), dim3(1), 0, c->stream, a.info);
   TUPS_HIP(hipGetLastError());
   for (int q0 = 0; q0 < s.nq; q0 += qbatch) {
@@ -604,7 +858,10 @@ int launch_bse_closed(tups_ctx* c) {
     if (int rc = timer_begin(c)) return rc;
     hipLaunchKernelGGL(closed_build, dim3((unsigned)(2 * D), mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
-    hipLaunchKernelGGL((closed_solve<double, true>), dim3(mats), dim3(256), 0, c->stream, a);
+    if (blk)
+      hipLaunchKernelGGL((closed_solve_b64<double, true>), dim3(mats), dim3(256), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((closed_solve<double, true>), dim3(mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (int rc = timer_end(c, TUPS_K_BSE_CLOSED)) return rc;
   }
@@ -622,8 +879,10 @@ int launch_bse_closed_ir(tups_ctx* c) {
   const size_t D = s.nz, DD = D * D;
   const int tiles_n = (int)((D + CL_T - 1) / CL_T);
   // per matrix: [A32 | I] (2 D^2 float2), R and chi X (D^2 double2 each), tile partials, res
+  const bool blk = closed_blocked(c);
   const size_t per_mat = 2 * DD * sizeof(float2) + 2 * DD * sizeof(double2) +
-                         (size_t)tiles_n * tiles_n * 2 * sizeof(double) + sizeof(double) + 64;
+                         (size_t)tiles_n * tiles_n * 2 * sizeof(double) + sizeof(double) + 64 +
+                         (blk ? closed_b64_elems(D) * sizeof(float2) : 0);
   int qbatch = 0;
   if (int rc = closed_reserve(c, 4 * per_mat, 256, &qbatch)) return rc;
   ClosedArgs a{};
@@ -653,6 +912,10 @@ int launch_bse_closed_ir(tups_ctx* c) {
   b.res = reinterpret_cast<double*>(base);
   base += mats_max * sizeof(double);
   b.rmax = reinterpret_cast<unsigned long long*>((reinterpret_cast<uintptr_t>(base) + 63) / 64 * 64);
+  if (blk) {  // FP32 scratch of the block elimination after the counter
+    a.S = reinterpret_cast<float2*>(reinterpret_cast<char*>(b.rmax) + 64);
+    a.T = static_cast<float2*>(a.S) + mats_max * CL_BB * D;
+  }
   hipLaunchKernelGGL(closed_clear, dim3(1), dim3(1), 0, c->stream, a.info);
   TUPS_HIP(hipGetLastError());
   int steps_max = 0;
@@ -665,7 +928,10 @@ int launch_bse_closed_ir(tups_ctx* c) {
     if (int rc = timer_begin(c)) return rc;
     hipLaunchKernelGGL(ir_build32, dim3((unsigned)(2 * D), mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
-    hipLaunchKernelGGL((closed_solve<float, false>), dim3(mats), dim3(256), 0, c->stream, a);
+    if (blk)
+      hipLaunchKernelGGL((closed_solve_b64<float, false>), dim3(mats), dim3(256), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((closed_solve<float, false>), dim3(mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     b.b = nullptr;  // X = A32^-1 Gamma
     hipLaunchKernelGGL(ir_gemm<0>, dim3(gblk), dim3(256), 0, c->stream, b);

@@ -571,6 +571,7 @@ int tups_autotune(tups_ctx* c, int32_t kernel, int32_t* variant, float* ms) {
 const char* tups_variant_name(int32_t kernel, int32_t variant) {
   if (kernel == TUPS_K_BSE_BATCH) return batch_variant_name(variant);
   if (kernel == TUPS_K_SPLIT_GEMM) return split_variant_name(variant);
+  if (kernel == TUPS_K_BSE_CLOSED || kernel == TUPS_K_BSE_IR) return closed_variant_name(variant);
   return kernel == TUPS_K_BSE_LADDER ? bse_variant_name(variant)
                                      : (kernel == TUPS_K_PAE ? pae_variant_name(variant) : "");
 }
@@ -588,6 +589,11 @@ int tups_set_variant(tups_ctx* c, int32_t kernel, int32_t variant) {
     c->split_variant = variant;
     return TUPS_OK;
   }
+  if (kernel == TUPS_K_BSE_CLOSED || kernel == TUPS_K_BSE_IR) {
+    TUPS_CHECK(variant >= -1 && variant < closed_variant_count(), TUPS_EINVAL, "no such elimination variant");
+    c->closed_variant = variant;
+    return TUPS_OK;
+  }
   TUPS_CHECK(kernel == TUPS_K_BSE_LADDER, TUPS_EINVAL, "kernel has no tile variants");
   TUPS_CHECK(variant == -1 || bse_variant_nl(variant) == c->sz.nl, TUPS_EINVAL,
              "variant does not exist for this Nl");
@@ -598,6 +604,7 @@ int tups_set_variant(tups_ctx* c, int32_t kernel, int32_t variant) {
 int tups_variant_count(int32_t kernel) {
   if (kernel == TUPS_K_BSE_BATCH) return batch_variant_count();
   if (kernel == TUPS_K_SPLIT_GEMM) return split_variant_count();
+  if (kernel == TUPS_K_BSE_CLOSED || kernel == TUPS_K_BSE_IR) return closed_variant_count();
   return kernel == TUPS_K_BSE_LADDER ? bse_variant_count() : (kernel == TUPS_K_PAE ? pae_variant_count() : 0);
 }
 

@@ -140,6 +140,7 @@ struct tups_ctx {
   int bse_variant = -1;          // kernel variant chosen by the autotuner (-1: default)
   bool three_mult = false;       // the autotuner may pick the three-multiplication product forms
   int pae_variant = -1;
+  int closed_variant = -1;       // closed-form elimination: 0 panels of 16, 1 blocks of 64 (-1: by D)
   int* pae_rows = nullptr;       // Nl > 1 crossing: the (l1, l2, Q) rows to form (null: all); F's other rows
                                  // are not written (set only by the uneven-box iteration, which reads no other)
   int pae_nrows = 0;
@@ -243,6 +244,8 @@ int upload_tables(tups_ctx* c);
 int timer_begin(tups_ctx* c);
 // device address of the scratch array of tups_conv_noalloc, once per device (conv.hip)
 int conv_noalloc_setup(int device);
+int closed_variant_count();
+const char* closed_variant_name(int v);
 int timer_end(tups_ctx* c, int kernel);
 // one launch of the batching form's ladder, variant v, into the spare buffers (bse.hip)
 int batch_launch(tups_batch* b, int v);

@@ -513,7 +513,7 @@ class TupsContext:
         out = []
         for v in range(self.lib.tups_variant_count(kernel)):
             name = self.lib.tups_variant_name(kernel, v).decode()
-            if kernel == K_SPLIT_GEMM:  # every tile serves every Nl
+            if kernel in (K_SPLIT_GEMM, K_BSE_CLOSED, K_BSE_IR):  # every variant serves every Nl
                 out.append((v, name))
                 continue
             if not name.startswith(f"nl{self.nl}_"):
