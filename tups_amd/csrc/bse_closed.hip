@@ -559,41 +559,108 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
         closed_update<R>(Sk, D, s0 + sb, D, s0 + sb, k0 + nb, s0, sb, As, Bs);
       }
     }
-    // B. inverses of the unit lower L11 and the upper U11 (identity-padded to
-    // CL_BB), thread t < CL_BB forming column t of each, then T = U11^-1 L11^-1
-    if (tid < CL_BB) {
-      const int t = tid;
-      for (int i = 0; i < CL_BB; ++i) {
-        C x = i == t ? one : zero;
-        if (i < nb && t < nb)
-          for (int s2 = t; s2 < i; ++s2) x = csub(x, cmul(Sk[(size_t)(k0 + s2) * D + k0 + i], Linv[s2 + CL_BB * t]));
-        Linv[i + CL_BB * t] = x;
+    // B. T = U11^-1 L11^-1 (identity-padded to CL_BB) by 16 x 16 blocks: the
+    // inverses of the four diagonal blocks of L11 and U11 from copies in LDS
+    // (thread = block column), then the off-diagonal blocks by their distance d
+    // from the diagonal, all blocks at one distance at once,
+    //   Linv_ab = -Linv_aa sum_{c=b}^{a-1} L_ac Linv_cb      (a > b)
+    //   Uinv_ab = -Uinv_aa sum_{c=a+1}^{b} U_ac Uinv_cb      (a < b),
+    // then T = Uinv Linv on MFMA.
+    {
+      constexpr int NBB = CL_BB / CL_NB;
+      // L11 (unit lower) and U11 (upper) entry (i, s), identity beyond nb
+      auto Lv = [&](int i, int s2) -> C {
+        if (i >= nb || s2 >= nb) return i == s2 ? one : zero;
+        return i > s2 ? Sk[(size_t)(k0 + s2) * D + k0 + i] : (i == s2 ? one : zero);
+      };
+      auto Uv = [&](int i, int s2) -> C {
+        if (i >= nb || s2 >= nb) return i == s2 ? one : zero;
+        return i <= s2 ? Sk[(size_t)(k0 + s2) * D + k0 + i] : zero;
+      };
+      C* Ld = &As[0][0];  // the NBB diagonal blocks of L11, [blk][col][row]
+      C* Ud = &Bs[0][0];  // and of U11
+      for (int u = tid; u < NBB * CL_NB * CL_NB; u += 256) {
+        const int blk = u / (CL_NB * CL_NB), e = u % (CL_NB * CL_NB), i = e % CL_NB, j = e / CL_NB;
+        Ld[u] = Lv(blk * CL_NB + i, blk * CL_NB + j);
+        Ud[u] = Uv(blk * CL_NB + i, blk * CL_NB + j);
       }
-      for (int i = CL_BB - 1; i >= 0; --i) {
-        C x = i == t ? one : zero;
-        if (i < nb && t < nb) {
-          for (int s2 = i + 1; s2 <= t; ++s2)
-            x = csub(x, cmul(Sk[(size_t)(k0 + s2) * D + k0 + i], Uinv[s2 + CL_BB * t]));
-          x = cmul(x, cinv(Sk[(size_t)(k0 + i) * D + k0 + i]));
+      __syncthreads();
+      if (tid < 2 * NBB * CL_NB) {
+        const bool up = tid >= NBB * CL_NB;
+        const int blk = (tid / CL_NB) % NBB, t = tid % CL_NB, off = blk * CL_NB;
+        const C* Md = (up ? Ud : Ld) + blk * CL_NB * CL_NB;
+        C* out = (up ? Uinv : Linv) + (size_t)(off + t) * CL_BB + off;  // column t of the diagonal block
+        if (!up) {
+          for (int i = 0; i < CL_NB; ++i) {
+            C v = i == t ? one : zero;
+            for (int s2 = t; s2 < i; ++s2) v = csub(v, cmul(Md[s2 * CL_NB + i], out[s2]));
+            out[i] = v;
+          }
+        } else {
+          for (int i = CL_NB - 1; i >= 0; --i) {
+            C v = i == t ? one : zero;
+            for (int s2 = i + 1; s2 <= t; ++s2) v = csub(v, cmul(Md[s2 * CL_NB + i], out[s2]));
+            out[i] = cmul(v, cinv(Md[i * CL_NB + i]));
+          }
         }
-        Uinv[i + CL_BB * t] = x;
       }
-    }
-    __syncthreads();
-    for (int u = tid; u < CL_BB * CL_BB; u += 256) {
-      const int i = u % CL_BB, j = u / CL_BB;
-      R xr = 0, xi = 0;
-      for (int s2 = 0; s2 < CL_BB; ++s2) {
-        const C ui = Uinv[i + CL_BB * s2], lv = Linv[s2 + CL_BB * j];
-        xr = fma(ui.x, lv.x, fma(-ui.y, lv.y, xr));
-        xi = fma(ui.x, lv.y, fma(ui.y, lv.x, xi));
+      // zero blocks: above the diagonal of Linv, below it of Uinv
+      for (int u = tid; u < CL_BB * CL_BB; u += 256) {
+        const int i = u % CL_BB, j = u / CL_BB;
+        if (i / CL_NB < j / CL_NB) Linv[u] = zero;
+        if (i / CL_NB > j / CL_NB) Uinv[u] = zero;
       }
-      T[u] = mkc(xr, xi);
+      __syncthreads();
+      C* M = T;  // products of one distance: [L pairs][U pairs] x 16 x 16
+      for (int d = 1; d < NBB; ++d) {
+        const int np = NBB - d;  // pairs per triangle
+        for (int u = tid; u < 2 * np * CL_NB * CL_NB; u += 256) {
+          const int pr = u / (CL_NB * CL_NB), e = u % (CL_NB * CL_NB), i = e % CL_NB, j = e / CL_NB;
+          R xr = 0, xi = 0;
+          if (pr < np) {  // L: a = d + pr, b = pr
+            const int a2 = d + pr, b2 = pr;
+            for (int c2 = b2; c2 < a2; ++c2)
+              for (int k = 0; k < CL_NB; ++k) {
+                const C l = Lv(a2 * CL_NB + i, c2 * CL_NB + k);
+                const C y = Linv[(size_t)(b2 * CL_NB + j) * CL_BB + c2 * CL_NB + k];
+                xr = fma(l.x, y.x, fma(-l.y, y.y, xr));
+                xi = fma(l.x, y.y, fma(l.y, y.x, xi));
+              }
+          } else {  // U: a = pr - np, b = a + d
+            const int a2 = pr - np, b2 = a2 + d;
+            for (int c2 = a2 + 1; c2 <= b2; ++c2)
+              for (int k = 0; k < CL_NB; ++k) {
+                const C uu = Uv(a2 * CL_NB + i, c2 * CL_NB + k);
+                const C y = Uinv[(size_t)(b2 * CL_NB + j) * CL_BB + c2 * CL_NB + k];
+                xr = fma(uu.x, y.x, fma(-uu.y, y.y, xr));
+                xi = fma(uu.x, y.y, fma(uu.y, y.x, xi));
+              }
+          }
+          M[u] = mkc(xr, xi);
+        }
+        __syncthreads();
+        for (int u = tid; u < 2 * np * CL_NB * CL_NB; u += 256) {
+          const int pr = u / (CL_NB * CL_NB), e = u % (CL_NB * CL_NB), i = e % CL_NB, j = e / CL_NB;
+          const bool upp = pr >= np;
+          const int a2 = upp ? pr - np : d + pr, b2 = upp ? a2 + d : pr;
+          const C* Dg = (upp ? Uinv : Linv) + (size_t)(a2 * CL_NB) * CL_BB + a2 * CL_NB;  // diagonal block a
+          R xr = 0, xi = 0;
+          for (int k = 0; k < CL_NB; ++k) {
+            const C g = Dg[(size_t)k * CL_BB + i];
+            const C m = M[pr * CL_NB * CL_NB + j * CL_NB + k];
+            xr = fma(g.x, m.x, fma(-g.y, m.y, xr));
+            xi = fma(g.x, m.y, fma(g.y, m.x, xi));
+          }
+          (upp ? Uinv : Linv)[(size_t)(b2 * CL_NB + j) * CL_BB + a2 * CL_NB + i] = mkc(-xr, -xi);
+        }
+        __syncthreads();
+      }
+      // T = Uinv Linv, in place in Linv
+      closed_apply_T<R>(Linv, Uinv, CL_BB, 0, CL_BB, 0, CL_BB, As, Bs);
     }
-    __syncthreads();
     const int j0 = k0 + nb;
     // C. block rows of the later columns: X = T Z1
-    closed_apply_T<R>(W, T, D, k0, nb, j0, D2, As, Bs);
+    closed_apply_T<R>(W, Linv, D, k0, nb, j0, D2, As, Bs);
     // D. rows below and above the block
     closed_update<R>(W, D, k0 + nb, D, j0, D2, k0, nb, As, Bs);
     closed_update<R>(W, D, 0, k0, j0, D2, k0, nb, As, Bs);
