@@ -195,14 +195,39 @@ __device__ void closed_update(typename Prec<R>::C* W, int D, int r0, int r1, int
             ar[i][j][e] = z.x;
             ai[i][j][e] = z.y;
           }
+      // staging: thread tid moves A(row tid % 64, k (tid / 64) + 4 v) and
+      // B(k tid % 16, column tid / 16 + 16 v), v < 4.  In FP32 the next chunk
+      // is loaded into registers while the MFMAs of this one run; in FP64 that
+      // costs spills in the panel kernel, and the chunk goes straight to LDS.
+      constexpr bool kPrefetch = sizeof(R) == 4;
+      C pa[4], pb[4];
+      auto fetch = [&](int kc) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = tid & (CL_T - 1), ka = (tid >> 6) + 4 * v;
+          pa[v] = (tr + row < r1 && kc + ka < nb) ? W[(size_t)(k0 + kc + ka) * D + tr + row] : zero;
+          const int kb = tid % CL_NB, col = (tid / CL_NB) + 16 * v;
+          pb[v] = (tc + col < c1 && kc + kb < nb) ? W[(size_t)(tc + col) * D + k0 + kc + kb] : zero;
+        }
+      };
+      if constexpr (kPrefetch) fetch(0);
       for (int kc = 0; kc < nb; kc += CL_NB) {
-        for (int u = tid; u < CL_NB * CL_T; u += 256) {
-          const int row = u & (CL_T - 1), ka = u >> 6;  // A: rows contiguous
-          As[ka][row] = (tr + row < r1 && kc + ka < nb) ? W[(size_t)(k0 + kc + ka) * D + tr + row] : zero;
-          const int kb = u % CL_NB, col = u / CL_NB;    // B: k contiguous in a column
-          Bs[kb][col] = (tc + col < c1 && kc + kb < nb) ? W[(size_t)(tc + col) * D + k0 + kc + kb] : zero;
+        if constexpr (kPrefetch) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            As[(tid >> 6) + 4 * v][tid & (CL_T - 1)] = pa[v];
+            Bs[tid % CL_NB][(tid / CL_NB) + 16 * v] = pb[v];
+          }
+        } else {
+          for (int u = tid; u < CL_NB * CL_T; u += 256) {
+            const int row = u & (CL_T - 1), ka = u >> 6;
+            As[ka][row] = (tr + row < r1 && kc + ka < nb) ? W[(size_t)(k0 + kc + ka) * D + tr + row] : zero;
+            const int kb = u % CL_NB, col = u / CL_NB;
+            Bs[kb][col] = (tc + col < c1 && kc + kb < nb) ? W[(size_t)(tc + col) * D + k0 + kc + kb] : zero;
+          }
         }
         __syncthreads();
+        if (kPrefetch && kc + CL_NB < nb) fetch(kc + CL_NB);
 #pragma unroll
         for (int kk = 0; kk < CL_NB; kk += 4) {
           C fa[2], fb[2];
@@ -256,14 +281,36 @@ __device__ void closed_apply_T(typename Prec<R>::C* W, const typename Prec<R>::C
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) ar[i][j] = ai[i][j] = V{};
+    // staging as in closed_update
+    constexpr bool kPrefetch = sizeof(R) == 4;
+    C pa[4], pb[4];
+    auto fetch = [&](int kc) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tid & (CL_T - 1), ka = (tid >> 6) + 4 * v;
+        pa[v] = (row < nb && kc + ka < nb) ? T[(size_t)(kc + ka) * CL_BB + row] : zero;
+        const int kb = tid % CL_NB, col = (tid / CL_NB) + 16 * v;
+        pb[v] = (tc + col < c1 && kc + kb < nb) ? W[(size_t)(tc + col) * D + k0 + kc + kb] : zero;
+      }
+    };
+    if constexpr (kPrefetch) fetch(0);
     for (int kc = 0; kc < nb; kc += CL_NB) {
-      for (int u = tid; u < CL_NB * CL_T; u += 256) {
-        const int row = u & (CL_T - 1), ka = u >> 6;
-        As[ka][row] = (row < nb && kc + ka < nb) ? T[(size_t)(kc + ka) * CL_BB + row] : zero;
-        const int kb = u % CL_NB, col = u / CL_NB;
-        Bs[kb][col] = (tc + col < c1 && kc + kb < nb) ? W[(size_t)(tc + col) * D + k0 + kc + kb] : zero;
+      if constexpr (kPrefetch) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          As[(tid >> 6) + 4 * v][tid & (CL_T - 1)] = pa[v];
+          Bs[tid % CL_NB][(tid / CL_NB) + 16 * v] = pb[v];
+        }
+      } else {
+        for (int u = tid; u < CL_NB * CL_T; u += 256) {
+          const int row = u & (CL_T - 1), ka = u >> 6;
+          As[ka][row] = (row < nb && kc + ka < nb) ? T[(size_t)(kc + ka) * CL_BB + row] : zero;
+          const int kb = u % CL_NB, col = u / CL_NB;
+          Bs[kb][col] = (tc + col < c1 && kc + kb < nb) ? W[(size_t)(tc + col) * D + k0 + kc + kb] : zero;
+        }
       }
       __syncthreads();
+      if (kPrefetch && kc + CL_NB < nb) fetch(kc + CL_NB);
 #pragma unroll
       for (int kk = 0; kk < CL_NB; kk += 4) {
         C fa[2], fb[2];
