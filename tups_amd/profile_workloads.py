@@ -7,6 +7,7 @@
     python -m tups_amd.profile_workloads calib    --steps 3   # G = F - G streaming kernel
     python -m tups_amd.profile_workloads batch    --steps 3   # uneven q batch, Nx=8, D 16..576
     python -m tups_amd.profile_workloads uneven   --steps 3   # uneven-box iteration with the crossing
+    python -m tups_amd.profile_workloads closed    --steps 1  # direct FP64 elimination, D=320
     python -m tups_amd.profile_workloads closed_ir --steps 1  # FP32 LU + FP64 refinement, D=320
 
 `calib` runs a streaming kernel whose bytes are known exactly (per element:
@@ -150,7 +151,7 @@ def batch_flops(b: TupsBatch) -> float:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("workload", choices=["bse", "split", "crossing", "phir", "calib", "batch", "gtau", "config1",
-                                         "conv", "sus", "closed_ir", "uneven"])
+                                         "conv", "sus", "closed", "closed_ir", "uneven"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--nx", type=int, default=16)
     ap.add_argument("--nf", type=int, default=64)
@@ -244,7 +245,7 @@ def main(argv=None):
         b.close()
         sys.stdout.flush()
         return
-    if args.workload in ("bse", "split", "calib", "closed_ir"):
+    if args.workload in ("bse", "split", "calib", "closed", "closed_ir"):
         c = TupsContext(nx=8, nf=64, nl=5, nq=1024, beta=2.0, f_damping=0.5)
         batched_bse_inputs(c)
         if args.workload == "bse":
@@ -252,6 +253,11 @@ def main(argv=None):
             for _ in range(args.steps):
                 c.BSE(0)
             print(f"bse: variant {name}, {args.steps} launches, flops/launch {16.0 * c.nz ** 3 * 4 * c.nq:.4e}")
+        elif args.workload == "closed":
+            from .runtime import BSE_CLOSED_LU
+            for _ in range(args.steps):
+                c.BSE(0, mode=BSE_CLOSED_LU)
+            print(f"closed: {args.steps} calls, flops/call {12.0 * c.nz ** 3 * 4 * c.nq:.4e}")
         elif args.workload == "closed_ir":
             from .runtime import BSE_CLOSED_IR
             for _ in range(args.steps):
