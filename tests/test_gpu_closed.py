@@ -29,7 +29,7 @@ def _say(*a):
     print(*a, flush=True, file=sys.stderr)
 
 
-@pytest.mark.parametrize("variant", [0, 1])  # panels of 16, blocks of 64
+@pytest.mark.parametrize("variant", [0, 1, 2])  # panels of 16, blocks of 64, blocks with register sub-panels
 @pytest.mark.parametrize(
     "nx,nf,nl,nq",
     [(4, 16, 1, 0),     # Nz=16: one panel
@@ -63,6 +63,40 @@ def test_closed_form_parity(nx, nf, nl, nq, variant):
         assert err < TOL
 
 
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("nx,nf,nl,nq", [(4, 8, 9, 2), (8, 64, 5, 1)])
+@pytest.mark.parametrize("refined", [False, True])
+def test_closed_form_parity_with_pivoting(nx, nf, nl, nq, variant, refined):
+    """Gamma without its diagonal and ten times larger: I - kappa Gamma chi is
+    far from diagonally dominant, so partial pivoting swaps rows at most
+    columns (226-295 of the 320 at D = 320, 65-67 of 72 at D = 72, counted with
+    LAPACK's getrf on the same matrices).  The condition numbers reach 3e3, so
+    the bar is 1e-10 against the oracle's LAPACK solve; a misplaced swap gives
+    errors of order one."""
+    from tups_amd import BSE_CLOSED_IR, BSE_CLOSED_LU, CHI, F, GAMMA
+    from tups_amd.runtime import K_BSE_CLOSED, K_BSE_IR
+
+    _say(f"closed parity, pivoting: nx={nx} nf={nf} nl={nl} nq={nq} variant={variant} refined={refined}")
+    rng = np.random.default_rng(nx * 31 + nf + nl)
+    lat = Lattice(nx, nf, nl)
+    with _ctx(nx=nx, nf=nf, nl=nl, nq=nq, beta=2.0, f_damping=0.5) as c:
+        c.set_variant(variant, K_BSE_IR if refined else K_BSE_CLOSED)
+        gam = random_vertices(rng, c.nz, c.nq, diag=0.0, scale=3.0)
+        chic = [random_chi_compact(rng, nl, nf, c.nq, 3.0) for _ in range(2)]
+        for r in range(4):
+            c.upload(GAMMA, r, gam[r])
+            c.upload(F, r, gam[r])
+        for p in range(2):
+            c.upload(CHI, p, chic[p])
+        c.BSE(0, mode=BSE_CLOSED_IR if refined else BSE_CLOSED_LU)
+        got = [c.download(GAMMA, r) for r in range(4)]
+    exp = B.bse_closed(lat, gam, [B.chi_compact_to_dense(lat, x) for x in chic], 2.0)
+    for r in range(4):
+        err = rel_fro(got[r], exp[r])
+        _say(f"  channel {r}: rel err {err:.2e}")
+        assert err < 1e-10
+
+
 def test_closed_form_is_fixed_point_of_gpu_ladder():
     from tups_amd import BSE_CLOSED_LU, CHI, F, GAMMA
 
@@ -86,12 +120,15 @@ def test_closed_form_is_fixed_point_of_gpu_ladder():
         assert rel_fro(again[r], phi[r]) < TOL, r
 
 
-def test_closed_form_reports_singular_system():
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_closed_form_reports_singular_system(variant):
     """Gamma_d = I and chi_ph = 1/kappa_d make I - kappa_d Gamma_d chi_ph = 0."""
     from tups_amd import BSE_CLOSED_LU, CHI, GAMMA, TupsError
+    from tups_amd.runtime import K_BSE_CLOSED
 
     nx, nf, nl = 4, 8, 1
     with _ctx(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as c:
+        c.set_variant(variant, K_BSE_CLOSED)
         D, nq = c.nz, c.nq
         eye = np.asfortranarray(np.repeat(np.eye(D, dtype=complex)[:, :, None], nq, axis=2))
         for r in range(4):
@@ -103,7 +140,7 @@ def test_closed_form_reports_singular_system():
             c.BSE(0, mode=BSE_CLOSED_LU)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize(
     "nx,nf,nl,nq",
     [(4, 16, 1, 0), (4, 8, 5, 0), (4, 6, 5, 3), (4, 8, 9, 2), (8, 64, 5, 2)],
@@ -151,11 +188,14 @@ def test_refined_closed_form_parity(nx, nf, nl, nq, variant):
         assert err < 1e-11
 
 
-def test_refined_closed_form_reports_singular_system():
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_refined_closed_form_reports_singular_system(variant):
     from tups_amd import BSE_CLOSED_IR, CHI, GAMMA, TupsError
+    from tups_amd.runtime import K_BSE_IR
 
     nx, nf, nl = 4, 8, 1
     with _ctx(nx=nx, nf=nf, nl=nl, beta=2.0, f_damping=0.5) as c:
+        c.set_variant(variant, K_BSE_IR)
         D, nq = c.nz, c.nq
         eye = np.asfortranarray(np.repeat(np.eye(D, dtype=complex)[:, :, None], nq, axis=2))
         for r in range(4):

@@ -99,6 +99,16 @@ __device__ __forceinline__ C cinv(C a) {
   return mkc(r / d, -1 / d);
 }
 
+// f(std::integral_constant<int, I>{}) for I = B .. E - 1, each I a compile-time
+// constant (register arrays indexed by it stay in registers)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 struct ClosedArgs {
   const double2* gam[4];
   double2* out[4];
@@ -487,6 +497,112 @@ __global__ __launch_bounds__(256, 2) void closed_solve(ClosedArgs a) {
   }
 }
 
+// X(dst[u], col) <- X(src[u], col) for u < n, every column col in [c0, c1)
+// outside [e0, e1), all with the values from before the call (the rows of a
+// composed sequence of row swaps).  Whole columns go at a time, PER items per
+// thread: every source of a column is read before any of its rows is written.
+template <class R, int PER>
+__device__ void move_rows(typename Prec<R>::C* X, int ld, int c0, int c1, int e0, int e1, const int* dst,
+                          const int* src, int n) {
+  using C = typename Prec<R>::C;
+  const int tid = threadIdx.x;
+  const int nlo = e0 - c0, ncol = (c1 - c0) - (e1 - e0);
+  if (n == 0 || ncol <= 0) return;
+  const int cols = max(1, 256 * PER / n);  // whole columns per round
+  for (int cb = 0; cb < ncol; cb += cols) {
+    const int items = n * min(cols, ncol - cb);
+    C val[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int it = k * 256 + tid;
+      if (it < items) {
+        const int cc = cb + it / n;
+        const int col = cc < nlo ? c0 + cc : e1 + (cc - nlo);
+        val[k] = X[(size_t)col * ld + src[it % n]];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int it = k * 256 + tid;
+      if (it < items) {
+        const int cc = cb + it / n;
+        const int col = cc < nlo ? c0 + cc : e1 + (cc - nlo);
+        X[(size_t)col * ld + dst[it % n]] = val[k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// As move_rows for n <= 128 and no excluded columns, without workgroup
+// barriers: wave w owns the columns c0 + G (w + 4 k) .. + G - 1 and lane l the
+// moved rows l and l + 64; a wave's loads of a group of columns have all
+// returned (vmcnt 0) before its stores to them are issued.
+template <class R, int G>
+__device__ void move_rows_waves(typename Prec<R>::C* X, int ld, int c0, int c1, const int* dst, const int* src,
+                                int n) {
+  using C = typename Prec<R>::C;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool h0 = lane < n, h1 = lane + 64 < n;
+  const int s0 = h0 ? src[lane] : 0, d0 = h0 ? dst[lane] : 0;
+  const int s1 = h1 ? src[lane + 64] : 0, d1 = h1 ? dst[lane + 64] : 0;
+  for (int cb = c0 + G * wave; cb < c1; cb += 4 * G) {
+    C x0[G], x1[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const size_t o = (size_t)min(cb + g, c1 - 1) * ld;
+      if (h0) x0[g] = X[o + s0];
+      if (h1) x1[g] = X[o + s1];
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (cb + g >= c1) break;
+      const size_t o = (size_t)(cb + g) * ld;
+      if (h0) X[o + d0] = x0[g];
+      if (h1) X[o + d1] = x1[g];
+    }
+  }
+}
+
+// The rows that the swaps (r0 + i, piv[i]), i < nr, move (piv[i] >= r0 + i),
+// for wave 0: map[x - lo] is the original row that row x holds after all the
+// swaps, and the rows with map[x - lo] != x are listed, those of r0 .. r0 + nr
+// - 1 first (in order), then those beyond, each once.  The count goes to *cnt.
+__device__ __forceinline__ void list_moved_rows(const int* piv, int r0, int nr, const int* map, int lo, int* dst,
+                                                int* src, int* cnt) {
+  const int lane = threadIdx.x & 63;
+  bool in = false, out = false;
+  int p = 0, sin = 0, sout = 0;
+  if (lane < nr) {
+    sin = map[r0 + lane - lo];
+    in = sin != r0 + lane;
+    p = piv[lane];
+    if (p >= r0 + nr) {
+      sout = map[p - lo];
+      out = sout != p;
+      for (int j = 0; j < lane; ++j)
+        if (piv[j] == p) out = false;
+    }
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mi = __ballot(in), mo = __ballot(out);
+  const int ni = __popcll(mi);
+  if (in) {
+    const int u = __popcll(mi & below);
+    dst[u] = r0 + lane;
+    src[u] = sin;
+  }
+  if (out) {
+    const int u = ni + __popcll(mo & below);
+    dst[u] = p;
+    src[u] = sout;
+  }
+  if (lane == 0) *cnt = ni + __popcll(mo);
+}
+
 // Block Gauss-Jordan with CL_BB-wide blocks, one workgroup per (q, r) matrix.
 // For the block column k0 .. k0 + nb:
 //  A. its rows k0 .. D - 1 are copied to S and factored there (LU with partial
@@ -499,7 +615,17 @@ __global__ __launch_bounds__(256, 2) void closed_solve(ClosedArgs a) {
 //  D. every other row r: Z(r, .) -= W(r, k0 .. k0 + nb) X (rank-nb MFMA updates).
 // Each block thus passes the later columns through memory once, where the
 // CL_NB-wide form (closed_solve) passes them once per CL_NB columns.
-template <class R, bool PHI>
+// RP > 0 (D <= 256 RP): a sub-panel's column steps run on registers (thread t
+// holds rows s0 + t + 256 i, i < RP, of the CL_NB columns).  One barrier per
+// step: before it each wave leaves its best pivot candidate (by a wave
+// reduction) and that candidate's whole row in LDS, and thread c - s0 its row
+// c; after it every thread picks the same winner, the two rows are exchanged
+// from LDS and the rank-1 update runs on registers.  The swaps are recorded
+// as a row map; the rows they move (none when no row is swapped) reach the
+// other columns of the block at the end of the sub-panel and W at the end of
+// A.  RP = 0: every column step works on S in memory and swaps the rows of W
+// at once.
+template <class R, bool PHI, int RP>
 __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
   using C = typename Prec<R>::C;
   __shared__ C As[CL_NB][CL_T + 1];
@@ -508,6 +634,16 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
   __shared__ C Li[CL_NB][CL_NB];
   __shared__ R red_v[256];
   __shared__ int red_i[256];
+  constexpr int RMAX = RP > 0 ? 256 * RP : 1;  // rows of the register path
+  constexpr int RQ = RP > 0 ? 1 : 0;
+  __shared__ R pv[2][4 * RQ + 1];              // per-wave pivot candidates, two buffers
+  __shared__ int pi[2][4 * RQ + 1];
+  __shared__ C cand[2][4 * RQ + 1][CL_NB * RQ + 1];  // each wave's candidate pivot row, two buffers
+  __shared__ C crow[2][CL_NB * RQ + 1];              // row c, which the pivot row replaces
+  __shared__ int piv[CL_BB * RQ + 1];          // pivot row of each column of the block
+  __shared__ int map_s[RMAX], map_b[RMAX];     // row sources after the sub-panel's / block's swaps
+  __shared__ int mv_dst[2 * CL_BB * RQ + 1], mv_src[2 * CL_BB * RQ + 1];
+  __shared__ int mv_n;
   const int mat = blockIdx.x;
   const int D = a.D, D2 = 2 * D;
   C* W = static_cast<C*>(a.W) + (size_t)mat * 2 * D * D;
@@ -527,9 +663,139 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
       const int col = k0 + u / (D - k0), row = k0 + u % (D - k0);
       Sk[(size_t)col * D + row] = W[(size_t)col * D + row];
     }
+    if constexpr (RP > 0)
+      for (int x = k0 + tid; x < D; x += 256) map_b[x - k0] = x;
     __syncthreads();
     for (int s0 = k0; s0 < k0 + nb; s0 += CL_NB) {
       const int sb = min(CL_NB, k0 + nb - s0);
+      if constexpr (RP > 0) {
+        const int lane = tid & 63, wave = tid >> 6;
+        for (int x = s0 + tid; x < D; x += 256) map_s[x - s0] = x;
+        C v[RP][CL_NB];
+#pragma unroll
+        for (int i = 0; i < RP; ++i) {
+          const int row = s0 + tid + 256 * i;
+#pragma unroll
+          for (int j = 0; j < CL_NB; ++j) v[i][j] = (row < D && j < sb) ? Sk[(size_t)(s0 + j) * D + row] : zero;
+        }
+        // the CL_NB steps unrolled: the register columns take static indices
+        static_for<0, CL_NB>([&](auto jc) {
+          constexpr int j = decltype(jc)::value, buf = j & 1;
+          if (j >= sb) return;
+          const int c = s0 + j;
+          C pc[RP];  // this thread's entries of column c
+#pragma unroll
+          for (int i = 0; i < RP; ++i) pc[i] = v[i][j];
+          R best = -1;
+          int bi = c;
+#pragma unroll
+          for (int i = 0; i < RP; ++i) {
+            const int row = s0 + tid + 256 * i;
+            if (row >= c && row < D) {
+              const R m = fabs(pc[i].x) + fabs(pc[i].y);
+              if (m > best) {
+                best = m;
+                bi = row;
+              }
+            }
+          }
+          for (int off = 32; off > 0; off >>= 1) {
+            const R ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ob > best || (ob == best && oi < bi)) {
+              best = ob;
+              bi = oi;
+            }
+          }
+          const int tb = (bi - s0) & 255, ib = (bi - s0) >> 8;
+          if (tid == tb) {
+#pragma unroll
+            for (int i = 0; i < RP; ++i)
+              if (i == ib) {
+#pragma unroll
+                for (int jj = 0; jj < CL_NB; ++jj) cand[buf][wave][jj] = v[i][jj];
+              }
+          }
+          if (tid == j) {
+#pragma unroll
+            for (int jj = 0; jj < CL_NB; ++jj) crow[buf][jj] = v[0][jj];
+          }
+          if (lane == 0) {
+            pv[buf][wave] = best;
+            pi[buf][wave] = bi;
+          }
+          __syncthreads();
+          R pmax = pv[buf][0];
+          int p = pi[buf][0], pw = 0;
+#pragma unroll
+          for (int w = 1; w < 4; ++w) {
+            const R ob = pv[buf][w];
+            const int oi = pi[buf][w];
+            if (ob > pmax || (ob == pmax && oi < p)) {
+              pmax = ob;
+              p = oi;
+              pw = w;
+            }
+          }
+          if (!(pmax > 0)) {
+            if (info == 0) info = c + 1;
+            if (tid == 0) piv[c - k0] = c;
+            return;
+          }
+          const C* prw = cand[buf][pw];
+          const int tp = (p - s0) & 255, ip = (p - s0) >> 8;
+          if (p != c) {
+            if (tid == j) {
+#pragma unroll
+              for (int jj = 0; jj < CL_NB; ++jj) v[0][jj] = prw[jj];
+            }
+            if (tid == tp) {
+#pragma unroll
+              for (int i = 0; i < RP; ++i)
+                if (i == ip) {
+                  pc[i] = crow[buf][j];
+#pragma unroll
+                  for (int jj = 0; jj < CL_NB; ++jj) v[i][jj] = crow[buf][jj];
+                }
+            }
+          }
+          if (tid == 0) {
+            piv[c - k0] = p;
+            if (p != c) {
+              const int t1 = map_s[c - s0];
+              map_s[c - s0] = map_s[p - s0];
+              map_s[p - s0] = t1;
+              const int t2 = map_b[c - k0];
+              map_b[c - k0] = map_b[p - k0];
+              map_b[p - k0] = t2;
+            }
+          }
+          const C inv = cinv(prw[j]);
+#pragma unroll
+          for (int i = 0; i < RP; ++i) {
+            const int row = s0 + tid + 256 * i;
+            if (row > c && row < D) {
+              const C l = cmul(pc[i], inv);
+              v[i][j] = l;
+#pragma unroll
+              for (int jj = j + 1; jj < CL_NB; ++jj) v[i][jj] = csub(v[i][jj], cmul(l, prw[jj]));
+            }
+          }
+        });
+#pragma unroll
+        for (int i = 0; i < RP; ++i) {
+          const int row = s0 + tid + 256 * i;
+          if (row < D)
+#pragma unroll
+            for (int j = 0; j < CL_NB; ++j)
+              if (j < sb) Sk[(size_t)(s0 + j) * D + row] = v[i][j];
+        }
+        // the sub-panel's swaps on the other columns of the block
+        if (wave == 0) list_moved_rows(piv + (s0 - k0), s0, sb, map_s, s0, mv_dst, mv_src, &mv_n);
+        __syncthreads();
+        move_rows<R, 8>(Sk, D, k0, k0 + nb, s0, s0 + sb, mv_dst, mv_src, mv_n);
+        __syncthreads();
+      } else
       for (int c = s0; c < s0 + sb; ++c) {
         R best = -1;
         int bi = c;
@@ -605,6 +871,11 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
         __syncthreads();
         closed_update<R>(Sk, D, s0 + sb, D, s0 + sb, k0 + nb, s0, sb, As, Bs);
       }
+    }
+    if constexpr (RP > 0) {  // the block's swaps on W from column k0 on
+      if (tid < 64) list_moved_rows(piv, k0, nb, map_b, k0, mv_dst, mv_src, &mv_n);
+      __syncthreads();
+      move_rows_waves<R, 8>(W, D, k0, D2, mv_dst, mv_src, mv_n);
     }
     // B. T = U11^-1 L11^-1 (identity-padded to CL_BB) by 16 x 16 blocks: the
     // inverses of the four diagonal blocks of L11 and U11 from copies in LDS
@@ -939,12 +1210,17 @@ static void closed_common(tups_ctx* c, ClosedArgs& a) {
 }
 
 // elimination variants: 0 = CL_NB-wide panels (closed_solve), 1 = CL_BB-wide
-// blocks (closed_solve_b64); -1 = the block form from D = 128 on
-static const char* const kClosedVariants[] = {"gj_panel16", "gj_block64"};
-int closed_variant_count() { return 2; }
-const char* closed_variant_name(int v) { return v >= 0 && v < 2 ? kClosedVariants[v] : ""; }
+// blocks (closed_solve_b64), 2 = the same with the sub-panels on registers
+// (D <= 512; above that it runs as 1); -1 = 2 from D = 128 on
+static const char* const kClosedVariants[] = {"gj_panel16", "gj_block64", "gj_block64_rp"};
+constexpr int kClosedRp = 2;  // rows per thread of the register sub-panel
+int closed_variant_count() { return 3; }
+const char* closed_variant_name(int v) { return v >= 0 && v < 3 ? kClosedVariants[v] : ""; }
 static bool closed_blocked(const tups_ctx* c) {
-  return c->closed_variant == 1 || (c->closed_variant < 0 && c->sz.nz >= 128);
+  return c->closed_variant >= 1 || (c->closed_variant < 0 && c->sz.nz >= 128);
+}
+static bool closed_regs(const tups_ctx* c) {
+  return closed_blocked(c) && c->closed_variant != 1 && c->sz.nz <= 256 * kClosedRp;
 }
 // scratch of closed_solve_b64 per matrix, in complex elements
 static size_t closed_b64_elems(size_t D) { return (size_t)CL_BB * D + 3 * (size_t)CL_BB * CL_BB; }
@@ -972,8 +1248,10 @@ int launch_bse_closed(tups_ctx* c) {
     if (int rc = timer_begin(c)) return rc;
     hipLaunchKernelGGL(closed_build, dim3((unsigned)(2 * D), mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
-    if (blk)
-      hipLaunchKernelGGL((closed_solve_b64<double, true>), dim3(mats), dim3(256), 0, c->stream, a);
+    if (blk && closed_regs(c))
+      hipLaunchKernelGGL((closed_solve_b64<double, true, kClosedRp>), dim3(mats), dim3(256), 0, c->stream, a);
+    else if (blk)
+      hipLaunchKernelGGL((closed_solve_b64<double, true, 0>), dim3(mats), dim3(256), 0, c->stream, a);
     else
       hipLaunchKernelGGL((closed_solve<double, true>), dim3(mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
@@ -1042,8 +1320,10 @@ int launch_bse_closed_ir(tups_ctx* c) {
     if (int rc = timer_begin(c)) return rc;
     hipLaunchKernelGGL(ir_build32, dim3((unsigned)(2 * D), mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
-    if (blk)
-      hipLaunchKernelGGL((closed_solve_b64<float, false>), dim3(mats), dim3(256), 0, c->stream, a);
+    if (blk && closed_regs(c))
+      hipLaunchKernelGGL((closed_solve_b64<float, false, kClosedRp>), dim3(mats), dim3(256), 0, c->stream, a);
+    else if (blk)
+      hipLaunchKernelGGL((closed_solve_b64<float, false, 0>), dim3(mats), dim3(256), 0, c->stream, a);
     else
       hipLaunchKernelGGL((closed_solve<float, false>), dim3(mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());

@@ -140,7 +140,7 @@ struct tups_ctx {
   int bse_variant = -1;          // kernel variant chosen by the autotuner (-1: default)
   bool three_mult = false;       // the autotuner may pick the three-multiplication product forms
   int pae_variant = -1;
-  int closed_variant = -1;       // closed-form elimination: 0 panels of 16, 1 blocks of 64 (-1: by D)
+  int closed_variant = -1;       // closed-form elimination: 0 panels of 16, 1 blocks of 64, 2 blocks with register sub-panels (-1: by D)
   int* pae_rows = nullptr;       // Nl > 1 crossing: the (l1, l2, Q) rows to form (null: all); F's other rows
                                  // are not written (set only by the uneven-box iteration, which reads no other)
   int pae_nrows = 0;
