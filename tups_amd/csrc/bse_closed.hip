@@ -33,6 +33,7 @@
 // 1e-12.  Work per (q, r): 12 D^3 FP32 (inverse) + 8 D^3 FP32 per product with
 // the inverse + 8 D^3 FP64 per residual.
 #include "internal.h"
+#include "lds_dma.h"
 
 #include <cmath>
 #include <cstring>
@@ -269,6 +270,138 @@ __device__ void closed_update(typename Prec<R>::C* W, int D, int r0, int r1, int
           }
     }
   __syncthreads();  // the updated Z is visible to the whole workgroup
+}
+
+// closed_update in FP64 with the inner index staged by direct global-to-LDS
+// loads into two stages, so chunk k + 1 is in flight while chunk k multiplies
+// and no registers hold it.  A stage is the A chunk sa[CL_NB][CL_T + 1] (rows
+// padded) and the B chunk in the first CL_T CL_NB slots of sb, column-major
+// with the inner index XOR-swizzled by the column (a fragment read is then
+// free of bank conflicts).  Rows beyond r1 and columns beyond c1 load clamped
+// addresses (their results are not written).  nb must be a multiple of
+// CL_NB, so every chunk issues the same 8 loads per thread.
+__device__ void closed_update_dma(double2* W, int D, int r0, int r1, int c0, int c1, int k0, int nb,
+                                  double2 (*s0a)[CL_T + 1], double2 (*s0b)[CL_T + 1], double2 (*s1a)[CL_T + 1],
+                                  double2 (*s1b)[CL_T + 1]) {
+  using P = Prec<double>;
+  using V = P::V;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const double2 zero = mkc(0.0, 0.0);
+  const int nch = nb / CL_NB;
+  for (int tr = r0; tr < r1; tr += CL_T)
+    for (int tc = c0; tc < c1; tc += CL_T) {
+      const int arow = min(tr + lane, D - 1);
+      auto issue = [&](int ch, double2 (*sa)[CL_T + 1], double2 (*sb)[CL_T + 1]) {
+        const int kb0 = k0 + ch * CL_NB;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {  // A: inner index ka, rows tr .. tr + 63
+          const int ka = wave + 4 * v;
+          dma16(W, (unsigned)(((size_t)(kb0 + ka) * D + arow) * sizeof(double2)), lds_addr(&sa[ka][0]));
+        }
+        const double2* sbf = &sb[0][0];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {  // B: slots 64 d .. 64 d + 63 = columns 4 d .. 4 d + 3
+          const int d = wave + 4 * v, slot = 64 * d + lane, col = slot >> 4;
+          const int kb = (slot & 15) ^ (col & 15);
+          const int gc = min(tc + col, c1 - 1);
+          dma16(W, (unsigned)(((size_t)gc * D + kb0 + kb) * sizeof(double2)), lds_addr(sbf + 64 * d));
+        }
+      };
+      V ar[2][2], ai[2][2];
+      auto compute = [&](double2 (*sa)[CL_T + 1], double2 (*sb)[CL_T + 1]) {
+        const double2* sbf = &sb[0][0];
+#pragma unroll
+        for (int kk = 0; kk < CL_NB; kk += 4) {
+          double2 fa[2], fb[2];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) fa[i] = sa[kk + lk][wr + i * 16 + li];
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const int col = wc + j * 16 + li;
+            fb[j] = sbf[col * CL_NB + ((kk + lk) ^ (col & 15))];
+          }
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              ar[i][j] = P::fma_sub(fa[i].x, fb[j].x, ar[i][j]);
+              ar[i][j] = P::fma_add(fa[i].y, fb[j].y, ar[i][j]);
+              ai[i][j] = P::fma_sub(fa[i].x, fb[j].y, ai[i][j]);
+              ai[i][j] = P::fma_sub(fa[i].y, fb[j].x, ai[i][j]);
+            }
+        }
+      };
+      issue(0, s0a, s0b);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
+            const double2 z = (row < r1 && col < c1) ? W[(size_t)col * D + row] : zero;
+            ar[i][j][e] = z.x;
+            ai[i][j][e] = z.y;
+          }
+      // the tile's values are waited for here, with chunk 0, and not at the
+      // first MFMA (where the wait would also cover chunk 1)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          asm volatile("" ::"v"(ar[i][j][0]), "v"(ar[i][j][1]), "v"(ar[i][j][2]), "v"(ar[i][j][3]),
+                       "v"(ai[i][j][0]), "v"(ai[i][j][1]), "v"(ai[i][j][2]), "v"(ai[i][j][3]));
+      for (int ch = 0; ch < nch; ch += 2) {
+        if (ch + 1 < nch) {
+          issue(ch + 1, s1a, s1b);
+          wait_vmcnt<8>();
+        } else {
+          wait_vmcnt<0>();
+        }
+        stage_barrier();
+        compute(s0a, s0b);
+        stage_barrier();
+        if (ch + 1 >= nch) break;
+        if (ch + 2 < nch) {
+          issue(ch + 2, s0a, s0b);
+          wait_vmcnt<8>();
+        } else {
+          wait_vmcnt<0>();
+        }
+        stage_barrier();
+        compute(s1a, s1b);
+        stage_barrier();
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
+            if (row < r1 && col < c1) W[(size_t)col * D + row] = mkc(ar[i][j][e], ai[i][j][e]);
+          }
+    }
+  __syncthreads();  // the updated Z is visible to the whole workgroup
+}
+
+// the update of the block elimination: in FP64 with the DMA stages when nb
+// allows, else (and in FP32) closed_update on the first stage
+template <class R>
+__device__ __forceinline__ void closed_update_b64(typename Prec<R>::C* W, int D, int r0, int r1, int c0, int c1,
+                                                  int k0, int nb, typename Prec<R>::C (*s0a)[CL_T + 1],
+                                                  typename Prec<R>::C (*s0b)[CL_T + 1],
+                                                  typename Prec<R>::C (*s1a)[CL_T + 1],
+                                                  typename Prec<R>::C (*s1b)[CL_T + 1]) {
+  if constexpr (std::is_same<R, double>::value) {
+    if (nb % CL_NB == 0) {
+      closed_update_dma(W, D, r0, r1, c0, c1, k0, nb, s0a, s0b, s1a, s1b);
+      return;
+    }
+  }
+  closed_update<R>(W, D, r0, r1, c0, c1, k0, nb, s0a, s0b);
 }
 
 // Z(k0 + i, c) <- sum_t T(i, t) Z(k0 + t, c) for i, t < nb (nb <= CL_BB) and every
@@ -628,14 +761,18 @@ __device__ __forceinline__ void list_moved_rows(const int* piv, int r0, int nr, 
 template <class R, bool PHI, int RP>
 __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
   using C = typename Prec<R>::C;
-  __shared__ C As[CL_NB][CL_T + 1];
-  __shared__ C Bs[CL_NB][CL_T + 1];
-  __shared__ C Ls[CL_NB][CL_NB];
-  __shared__ C Li[CL_NB][CL_NB];
-  __shared__ R red_v[256];
-  __shared__ int red_i[256];
   constexpr int RMAX = RP > 0 ? 256 * RP : 1;  // rows of the register path
   constexpr int RQ = RP > 0 ? 1 : 0;
+  // two stages of the update; the first is also the staging of the other
+  // products (As, Bs), the second holds L11 and its inverse of a sub-panel
+  __shared__ C As[CL_NB][CL_T + 1];
+  __shared__ C Bs[CL_NB][CL_T + 1];
+  __shared__ C As1[CL_NB][CL_T + 1];
+  __shared__ C Bs1[CL_NB][CL_T + 1];
+  C (*Ls)[CL_NB] = reinterpret_cast<C (*)[CL_NB]>(&As1[0][0]);
+  C (*Li)[CL_NB] = reinterpret_cast<C (*)[CL_NB]>(&Bs1[0][0]);
+  __shared__ R red_v[256 * (1 - RQ) + 1];
+  __shared__ int red_i[256 * (1 - RQ) + 1];
   __shared__ R pv[2][4 * RQ + 1];              // per-wave pivot candidates, two buffers
   __shared__ int pi[2][4 * RQ + 1];
   __shared__ C cand[2][4 * RQ + 1][CL_NB * RQ + 1];  // each wave's candidate pivot row, two buffers
@@ -869,7 +1006,7 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
         __syncthreads();
         panel_rows_apply<R>(Sk, D, s0, sb, s0 + sb, k0 + nb, Li);
         __syncthreads();
-        closed_update<R>(Sk, D, s0 + sb, D, s0 + sb, k0 + nb, s0, sb, As, Bs);
+        closed_update_b64<R>(Sk, D, s0 + sb, D, s0 + sb, k0 + nb, s0, sb, As, Bs, As1, Bs1);
       }
     }
     if constexpr (RP > 0) {  // the block's swaps on W from column k0 on
@@ -980,8 +1117,8 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
     // C. block rows of the later columns: X = T Z1
     closed_apply_T<R>(W, Linv, D, k0, nb, j0, D2, As, Bs);
     // D. rows below and above the block
-    closed_update<R>(W, D, k0 + nb, D, j0, D2, k0, nb, As, Bs);
-    closed_update<R>(W, D, 0, k0, j0, D2, k0, nb, As, Bs);
+    closed_update_b64<R>(W, D, k0 + nb, D, j0, D2, k0, nb, As, Bs, As1, Bs1);
+    closed_update_b64<R>(W, D, 0, k0, j0, D2, k0, nb, As, Bs, As1, Bs1);
   }
   if (info && tid == 0) atomicCAS(a.info, 0, info);
   if constexpr (PHI) {

@@ -32,6 +32,7 @@
 // row and column maxima: 12 moduli (ta + tb = 79 at D = 320) give ~2e-11
 // relative Frobenius error, 16 moduli reach FP64 rounding.
 #include "internal.h"
+#include "lds_dma.h"
 
 #include <algorithm>
 #include <cmath>
@@ -556,38 +557,8 @@ __global__ __launch_bounds__(256, 2) void crt_gemm(CrtArgs a) {
 // finished reading (its MFMAs consumed the fragments before the barrier).
 // The loads are issued from asm, so the compiler, which cannot tell the
 // ring's stages apart across loop iterations, adds no vmcnt(0) before the
-// fragment reads; its own waits for other loads only grow more conservative.
-typedef __attribute__((address_space(3))) int8_t lds_i8_t;
-
-// M0 carries the LDS base; nothing else in these kernels reads M0
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Winline-asm"
-// 16 bytes from gbase + voff (uniform 64-bit base, per-lane 32-bit offset) to
-// LDS at lds_base + 16 lane
-__device__ __forceinline__ void dma16(const int8_t* gbase, unsigned voff, unsigned lds_base) {
-  asm volatile(
-      "s_mov_b32 m0, %2\n\t"
-      "global_load_lds_dwordx4 %0, %1"
-      :
-      : "v"(voff), "s"(gbase), "s"(lds_base)
-      : "m0");
-}
-#pragma clang diagnostic pop
-
-template <int N>
-__device__ __forceinline__ void wait_vmcnt() {
-  static_assert(N >= 0 && N < 16, "vmcnt immediate");
-  __builtin_amdgcn_s_waitcnt(0x0F70 | N);  // vmcnt(N); expcnt, lgkmcnt not waited
-}
-
-// a workgroup barrier that orders the compiler's LDS accesses but adds no wait
-// for outstanding loads (a fence would wait for the LDS loads in flight)
-__device__ __forceinline__ void stage_barrier() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
+// fragment reads; its own waits for other loads only grow more conservative
+// (dma16, wait_vmcnt, stage_barrier: lds_dma.h).
 typedef int8_t CrtStage[4][CRT_TILE][64];
 
 template <int NS>
@@ -622,7 +593,7 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_dma(CrtArgs a) {
     const int md = step / nk, kc = step - md * nk;
     const size_t off = (size_t)md * 2 * plane + (size_t)kc * a.Dp * 64;
     // pieces A+, A-, B+, B- are 4 KiB apart in the stage
-    const unsigned lds0 = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(lds_i8_t*)&S[0][wave * 16][0]);
+    const unsigned lds0 = lds_addr(&S[0][wave * 16][0]);
     dma16(baseA + off, voff, lds0);
     dma16(baseA + off + plane, voff, lds0 + 4096);
     dma16(baseB + off, voff, lds0 + 8192);
