@@ -272,17 +272,24 @@ __device__ void closed_update(typename Prec<R>::C* W, int D, int r0, int r1, int
   __syncthreads();  // the updated Z is visible to the whole workgroup
 }
 
-// closed_update in FP64 with the inner index staged by direct global-to-LDS
-// loads into two stages, so chunk k + 1 is in flight while chunk k multiplies
-// and no registers hold it.  A stage is the A chunk sa[CL_NB][CL_T + 1] (rows
-// padded) and the B chunk in the first CL_T CL_NB slots of sb, column-major
-// with the inner index XOR-swizzled by the column (a fragment read is then
-// free of bank conflicts).  Rows beyond r1 and columns beyond c1 load clamped
-// addresses (their results are not written).  nb must be a multiple of
-// CL_NB, so every chunk issues the same 8 loads per thread.
-__device__ void closed_update_dma(double2* W, int D, int r0, int r1, int c0, int c1, int k0, int nb,
-                                  double2 (*s0a)[CL_T + 1], double2 (*s0b)[CL_T + 1], double2 (*s1a)[CL_T + 1],
-                                  double2 (*s1b)[CL_T + 1]) {
+// The FP64 products of the block elimination with the inner index staged by
+// direct global-to-LDS loads into two stages, so chunk k + 1 is in flight
+// while chunk k multiplies and no registers hold it:
+//   APPLY = false (closed_update): Z(r, c) -= sum_t W(r, k0 + t) W(k0 + t, c),
+//     r in [r0, r1), c in [c0, c1);
+//   APPLY = true (closed_apply_T): Z(k0 + i, c) <- sum_t T(i, t) Z(k0 + t, c),
+//     i < nb, T column-major with leading dimension CL_BB (each tile is read
+//     whole before it is written, so the product runs in place).
+// A stage is the A chunk sa[CL_NB][CL_T + 1] (rows padded) and the B chunk in
+// the first CL_T CL_NB slots of sb, column-major with the inner index
+// XOR-swizzled by the column (a fragment read is then free of bank
+// conflicts).  Rows and columns beyond the ranges load clamped addresses
+// (their results are not written).  nb must be a multiple of CL_NB, so every
+// chunk issues the same 8 loads per thread.
+template <bool APPLY>
+__device__ void closed_gemm_dma(double2* W, int D, const double2* T, int r0, int r1, int c0, int c1, int k0, int nb,
+                                double2 (*s0a)[CL_T + 1], double2 (*s0b)[CL_T + 1], double2 (*s1a)[CL_T + 1],
+                                double2 (*s1b)[CL_T + 1]) {
   using P = Prec<double>;
   using V = P::V;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -290,15 +297,22 @@ __device__ void closed_update_dma(double2* W, int D, int r0, int r1, int c0, int
   const int li = lane & 15, lk = lane >> 4;
   const double2 zero = mkc(0.0, 0.0);
   const int nch = nb / CL_NB;
+  // A(row, t) = Ab[(ka0 + t) lda + row]; the output row of A's row r is ro + r
+  const double2* Ab = APPLY ? T : W;
+  const int lda = APPLY ? CL_BB : D, ka0 = APPLY ? 0 : k0, amax = APPLY ? CL_BB : D, ro = APPLY ? k0 : 0;
+  if (APPLY) {
+    r0 = 0;
+    r1 = nb;
+  }
   for (int tr = r0; tr < r1; tr += CL_T)
     for (int tc = c0; tc < c1; tc += CL_T) {
-      const int arow = min(tr + lane, D - 1);
+      const int arow = min(tr + lane, amax - 1);
       auto issue = [&](int ch, double2 (*sa)[CL_T + 1], double2 (*sb)[CL_T + 1]) {
-        const int kb0 = k0 + ch * CL_NB;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {  // A: inner index ka, rows tr .. tr + 63
           const int ka = wave + 4 * v;
-          dma16(W, (unsigned)(((size_t)(kb0 + ka) * D + arow) * sizeof(double2)), lds_addr(&sa[ka][0]));
+          dma16(Ab, (unsigned)(((size_t)(ka0 + ch * CL_NB + ka) * lda + arow) * sizeof(double2)),
+                lds_addr(&sa[ka][0]));
         }
         const double2* sbf = &sb[0][0];
 #pragma unroll
@@ -306,7 +320,7 @@ __device__ void closed_update_dma(double2* W, int D, int r0, int r1, int c0, int
           const int d = wave + 4 * v, slot = 64 * d + lane, col = slot >> 4;
           const int kb = (slot & 15) ^ (col & 15);
           const int gc = min(tc + col, c1 - 1);
-          dma16(W, (unsigned)(((size_t)gc * D + kb0 + kb) * sizeof(double2)), lds_addr(sbf + 64 * d));
+          dma16(W, (unsigned)(((size_t)gc * D + k0 + ch * CL_NB + kb) * sizeof(double2)), lds_addr(sbf + 64 * d));
         }
       };
       V ar[2][2], ai[2][2];
@@ -326,33 +340,47 @@ __device__ void closed_update_dma(double2* W, int D, int r0, int r1, int c0, int
           for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-              ar[i][j] = P::fma_sub(fa[i].x, fb[j].x, ar[i][j]);
-              ar[i][j] = P::fma_add(fa[i].y, fb[j].y, ar[i][j]);
-              ai[i][j] = P::fma_sub(fa[i].x, fb[j].y, ai[i][j]);
-              ai[i][j] = P::fma_sub(fa[i].y, fb[j].x, ai[i][j]);
+              if constexpr (APPLY) {  // + A B
+                ar[i][j] = P::fma_add(fa[i].x, fb[j].x, ar[i][j]);
+                ar[i][j] = P::fma_sub(fa[i].y, fb[j].y, ar[i][j]);
+                ai[i][j] = P::fma_add(fa[i].x, fb[j].y, ai[i][j]);
+                ai[i][j] = P::fma_add(fa[i].y, fb[j].x, ai[i][j]);
+              } else {  // - A B
+                ar[i][j] = P::fma_sub(fa[i].x, fb[j].x, ar[i][j]);
+                ar[i][j] = P::fma_add(fa[i].y, fb[j].y, ar[i][j]);
+                ai[i][j] = P::fma_sub(fa[i].x, fb[j].y, ai[i][j]);
+                ai[i][j] = P::fma_sub(fa[i].y, fb[j].x, ai[i][j]);
+              }
             }
         }
       };
       issue(0, s0a, s0b);
+      if constexpr (APPLY) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+          for (int j = 0; j < 2; ++j) ar[i][j] = ai[i][j] = V{};
+      } else {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
-            const double2 z = (row < r1 && col < c1) ? W[(size_t)col * D + row] : zero;
-            ar[i][j][e] = z.x;
-            ai[i][j][e] = z.y;
-          }
-      // the tile's values are waited for here, with chunk 0, and not at the
-      // first MFMA (where the wait would also cover chunk 1)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-          asm volatile("" ::"v"(ar[i][j][0]), "v"(ar[i][j][1]), "v"(ar[i][j][2]), "v"(ar[i][j][3]),
-                       "v"(ai[i][j][0]), "v"(ai[i][j][1]), "v"(ai[i][j][2]), "v"(ai[i][j][3]));
+            for (int e = 0; e < 4; ++e) {
+              const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
+              const double2 z = (row < r1 && col < c1) ? W[(size_t)col * D + row] : zero;
+              ar[i][j][e] = z.x;
+              ai[i][j][e] = z.y;
+            }
+        // the tile's values are waited for here, with chunk 0, and not at the
+        // first MFMA (where the wait would also cover chunk 1)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            asm volatile("" ::"v"(ar[i][j][0]), "v"(ar[i][j][1]), "v"(ar[i][j][2]), "v"(ar[i][j][3]),
+                         "v"(ai[i][j][0]), "v"(ai[i][j][1]), "v"(ai[i][j][2]), "v"(ai[i][j][3]));
+      }
       for (int ch = 0; ch < nch; ch += 2) {
         if (ch + 1 < nch) {
           issue(ch + 1, s1a, s1b);
@@ -381,10 +409,10 @@ __device__ void closed_update_dma(double2* W, int D, int r0, int r1, int c0, int
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int row = tr + wr + i * 16 + P::orow(lk, e), col = tc + wc + j * 16 + li;
-            if (row < r1 && col < c1) W[(size_t)col * D + row] = mkc(ar[i][j][e], ai[i][j][e]);
+            if (row < r1 && col < c1) W[(size_t)col * D + ro + row] = mkc(ar[i][j][e], ai[i][j][e]);
           }
     }
-  __syncthreads();  // the updated Z is visible to the whole workgroup
+  __syncthreads();  // the result is visible to the whole workgroup
 }
 
 // the update of the block elimination: in FP64 with the DMA stages when nb
@@ -397,7 +425,7 @@ __device__ __forceinline__ void closed_update_b64(typename Prec<R>::C* W, int D,
                                                   typename Prec<R>::C (*s1b)[CL_T + 1]) {
   if constexpr (std::is_same<R, double>::value) {
     if (nb % CL_NB == 0) {
-      closed_update_dma(W, D, r0, r1, c0, c1, k0, nb, s0a, s0b, s1a, s1b);
+      closed_gemm_dma<false>(W, D, nullptr, r0, r1, c0, c1, k0, nb, s0a, s0b, s1a, s1b);
       return;
     }
   }
@@ -485,6 +513,23 @@ __device__ void closed_apply_T(typename Prec<R>::C* W, const typename Prec<R>::C
         }
   }
   __syncthreads();
+}
+
+// the T products of the block elimination: in FP64 with the DMA stages when
+// nb allows, else (and in FP32) closed_apply_T on the first stage
+template <class R>
+__device__ __forceinline__ void closed_apply_T_b64(typename Prec<R>::C* W, const typename Prec<R>::C* T, int D, int k0,
+                                                   int nb, int c0, int c1, typename Prec<R>::C (*s0a)[CL_T + 1],
+                                                   typename Prec<R>::C (*s0b)[CL_T + 1],
+                                                   typename Prec<R>::C (*s1a)[CL_T + 1],
+                                                   typename Prec<R>::C (*s1b)[CL_T + 1]) {
+  if constexpr (std::is_same<R, double>::value) {
+    if (nb % CL_NB == 0) {
+      closed_gemm_dma<true>(W, D, T, 0, nb, c0, c1, k0, nb, s0a, s0b, s1a, s1b);
+      return;
+    }
+  }
+  closed_apply_T<R>(W, T, D, k0, nb, c0, c1, s0a, s0b);
 }
 
 // Z(k0 + i, c) <- sum_t M[i][t] Z(k0 + t, c) for i, t < nb and every column c in
@@ -1111,11 +1156,11 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
         __syncthreads();
       }
       // T = Uinv Linv, in place in Linv
-      closed_apply_T<R>(Linv, Uinv, CL_BB, 0, CL_BB, 0, CL_BB, As, Bs);
+      closed_apply_T_b64<R>(Linv, Uinv, CL_BB, 0, CL_BB, 0, CL_BB, As, Bs, As1, Bs1);
     }
     const int j0 = k0 + nb;
     // C. block rows of the later columns: X = T Z1
-    closed_apply_T<R>(W, Linv, D, k0, nb, j0, D2, As, Bs);
+    closed_apply_T_b64<R>(W, Linv, D, k0, nb, j0, D2, As, Bs, As1, Bs1);
     // D. rows below and above the block
     closed_update_b64<R>(W, D, k0 + nb, D, j0, D2, k0, nb, As, Bs, As1, Bs1);
     closed_update_b64<R>(W, D, 0, k0, j0, D2, k0, nb, As, Bs, As1, Bs1);
