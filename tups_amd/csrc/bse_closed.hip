@@ -810,10 +810,11 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
   constexpr int RQ = RP > 0 ? 1 : 0;
   // two stages of the update; the first is also the staging of the other
   // products (As, Bs), the second holds L11 and its inverse of a sub-panel
-  __shared__ C As[CL_NB][CL_T + 1];
-  __shared__ C Bs[CL_NB][CL_T + 1];
-  __shared__ C As1[CL_NB][CL_T + 1];
-  __shared__ C Bs1[CL_NB][CL_T + 1];
+  __shared__ C stages[4][CL_NB][CL_T + 1];  // As, Bs, As1, Bs1, one object so phase B can use them whole
+  C (*As)[CL_T + 1] = stages[0];
+  C (*Bs)[CL_T + 1] = stages[1];
+  C (*As1)[CL_T + 1] = stages[2];
+  C (*Bs1)[CL_T + 1] = stages[3];
   C (*Ls)[CL_NB] = reinterpret_cast<C (*)[CL_NB]>(&As1[0][0]);
   C (*Li)[CL_NB] = reinterpret_cast<C (*)[CL_NB]>(&Bs1[0][0]);
   __shared__ R red_v[256 * (1 - RQ) + 1];
@@ -1111,43 +1112,77 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
         if (i / CL_NB > j / CL_NB) Uinv[u] = zero;
       }
       __syncthreads();
-      C* M = T;  // products of one distance: [L pairs][U pairs] x 16 x 16
+      // the blocks of one distance through LDS (the stages are free here):
+      // per pair P (L pairs, then U pairs) the block row X_P (16 x w, rows
+      // padded to w + 1) and block column Y_P (w x 16) of the product, w = 16 d,
+      // M_P = X_P Y_P kept in registers, then M_P and the diagonal block
+      // (transposed) of the triangle's inverse for -Dinv_aa M_P
+      C* stg = &As[0][0];
       for (int d = 1; d < NBB; ++d) {
-        const int np = NBB - d;  // pairs per triangle
-        for (int u = tid; u < 2 * np * CL_NB * CL_NB; u += 256) {
-          const int pr = u / (CL_NB * CL_NB), e = u % (CL_NB * CL_NB), i = e % CL_NB, j = e / CL_NB;
-          R xr = 0, xi = 0;
-          if (pr < np) {  // L: a = d + pr, b = pr
-            const int a2 = d + pr, b2 = pr;
-            for (int c2 = b2; c2 < a2; ++c2)
-              for (int k = 0; k < CL_NB; ++k) {
-                const C l = Lv(a2 * CL_NB + i, c2 * CL_NB + k);
-                const C y = Linv[(size_t)(b2 * CL_NB + j) * CL_BB + c2 * CL_NB + k];
-                xr = fma(l.x, y.x, fma(-l.y, y.y, xr));
-                xi = fma(l.x, y.y, fma(l.y, y.x, xi));
-              }
-          } else {  // U: a = pr - np, b = a + d
-            const int a2 = pr - np, b2 = a2 + d;
-            for (int c2 = a2 + 1; c2 <= b2; ++c2)
-              for (int k = 0; k < CL_NB; ++k) {
-                const C uu = Uv(a2 * CL_NB + i, c2 * CL_NB + k);
-                const C y = Uinv[(size_t)(b2 * CL_NB + j) * CL_BB + c2 * CL_NB + k];
-                xr = fma(uu.x, y.x, fma(-uu.y, y.y, xr));
-                xi = fma(uu.x, y.y, fma(uu.y, y.x, xi));
-              }
+        const int np = NBB - d, w = CL_NB * d, per = 2 * CL_NB * w + CL_NB;
+        for (int u = tid; u < 2 * np * CL_NB * w; u += 256) {
+          const int pr = u / (CL_NB * w), e = u % (CL_NB * w);
+          const bool upp = pr >= np;
+          const int a2 = upp ? pr - np : d + pr, b2 = upp ? a2 + d : pr;
+          const int c0b = upp ? a2 + 1 : b2;  // first block of the inner range
+          C* X = stg + pr * per;
+          C* Y = X + CL_NB * (w + 1);
+          {  // X(i, kk): row a2 * 16 + i of the triangle, column c0b * 16 + kk
+            const int i = e / w, kk = e % w;
+            X[i * (w + 1) + kk] = upp ? Uv(a2 * CL_NB + i, c0b * CL_NB + kk) : Lv(a2 * CL_NB + i, c0b * CL_NB + kk);
           }
-          M[u] = mkc(xr, xi);
+          {  // Y(kk, j): the inverse's row c0b * 16 + kk, column b2 * 16 + j
+            const int kk = e / CL_NB, j = e % CL_NB;
+            Y[kk * CL_NB + j] = (upp ? Uinv : Linv)[(size_t)(b2 * CL_NB + j) * CL_BB + c0b * CL_NB + kk];
+          }
+        }
+        __syncthreads();
+        constexpr int MP = (2 * (NBB - 1) * CL_NB * CL_NB + 255) / 256;  // pair elements per thread, at most
+        C mr[MP];
+#pragma unroll
+        for (int q = 0; q < MP; ++q) {
+          const int u = tid + 256 * q;
+          mr[q] = zero;
+          if (u < 2 * np * CL_NB * CL_NB) {
+            const int pr = u / (CL_NB * CL_NB), e = u % (CL_NB * CL_NB), i = e % CL_NB, j = e / CL_NB;
+            const C* X = stg + pr * per;
+            const C* Y = X + CL_NB * (w + 1);
+            R xr = 0, xi = 0;
+            for (int kk = 0; kk < w; ++kk) {
+              const C x = X[i * (w + 1) + kk], y = Y[kk * CL_NB + j];
+              xr = fma(x.x, y.x, fma(-x.y, y.y, xr));
+              xi = fma(x.x, y.y, fma(x.y, y.x, xi));
+            }
+            mr[q] = mkc(xr, xi);
+          }
+        }
+        __syncthreads();
+        // M_P at stg[P 512 ..], the diagonal block a of the inverse transposed after it
+        for (int u = tid; u < 2 * np * CL_NB * CL_NB; u += 256) {
+          const int pr = u / (CL_NB * CL_NB), e = u % (CL_NB * CL_NB), i = e % CL_NB, k = e / CL_NB;
+          const bool upp = pr >= np;
+          const int a2 = upp ? pr - np : d + pr;
+          const C* Dg = (upp ? Uinv : Linv) + (size_t)(a2 * CL_NB) * CL_BB + a2 * CL_NB;
+          stg[pr * 2 * CL_NB * CL_NB + CL_NB * CL_NB + k * CL_NB + i] = Dg[(size_t)k * CL_BB + i];
+        }
+#pragma unroll
+        for (int q = 0; q < MP; ++q) {
+          const int u = tid + 256 * q;
+          if (u < 2 * np * CL_NB * CL_NB) {
+            const int pr = u / (CL_NB * CL_NB), e = u % (CL_NB * CL_NB);
+            stg[pr * 2 * CL_NB * CL_NB + e] = mr[q];  // M_P(i, j) at e = j 16 + i
+          }
         }
         __syncthreads();
         for (int u = tid; u < 2 * np * CL_NB * CL_NB; u += 256) {
           const int pr = u / (CL_NB * CL_NB), e = u % (CL_NB * CL_NB), i = e % CL_NB, j = e / CL_NB;
           const bool upp = pr >= np;
           const int a2 = upp ? pr - np : d + pr, b2 = upp ? a2 + d : pr;
-          const C* Dg = (upp ? Uinv : Linv) + (size_t)(a2 * CL_NB) * CL_BB + a2 * CL_NB;  // diagonal block a
+          const C* Mp = stg + pr * 2 * CL_NB * CL_NB;
+          const C* Dt = Mp + CL_NB * CL_NB;
           R xr = 0, xi = 0;
           for (int k = 0; k < CL_NB; ++k) {
-            const C g = Dg[(size_t)k * CL_BB + i];
-            const C m = M[pr * CL_NB * CL_NB + j * CL_NB + k];
+            const C g = Dt[k * CL_NB + i], m = Mp[j * CL_NB + k];
             xr = fma(g.x, m.x, fma(-g.y, m.y, xr));
             xi = fma(g.x, m.y, fma(g.y, m.x, xi));
           }
