@@ -1091,19 +1091,28 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
         const int blk = (tid / CL_NB) % NBB, t = tid % CL_NB, off = blk * CL_NB;
         const C* Md = (up ? Ud : Ld) + blk * CL_NB * CL_NB;
         C* out = (up ? Uinv : Linv) + (size_t)(off + t) * CL_BB + off;  // column t of the diagonal block
+        C x[CL_NB];  // the column, in registers (static indices)
         if (!up) {
+#pragma unroll
           for (int i = 0; i < CL_NB; ++i) {
             C v = i == t ? one : zero;
-            for (int s2 = t; s2 < i; ++s2) v = csub(v, cmul(Md[s2 * CL_NB + i], out[s2]));
-            out[i] = v;
+#pragma unroll
+            for (int s2 = 0; s2 < i; ++s2)
+              if (s2 >= t) v = csub(v, cmul(Md[s2 * CL_NB + i], x[s2]));
+            x[i] = v;
           }
         } else {
+#pragma unroll
           for (int i = CL_NB - 1; i >= 0; --i) {
             C v = i == t ? one : zero;
-            for (int s2 = i + 1; s2 <= t; ++s2) v = csub(v, cmul(Md[s2 * CL_NB + i], out[s2]));
-            out[i] = cmul(v, cinv(Md[i * CL_NB + i]));
+#pragma unroll
+            for (int s2 = i + 1; s2 < CL_NB; ++s2)
+              if (s2 <= t) v = csub(v, cmul(Md[s2 * CL_NB + i], x[s2]));
+            x[i] = cmul(v, cinv(Md[i * CL_NB + i]));
           }
         }
+#pragma unroll
+        for (int i = 0; i < CL_NB; ++i) out[i] = x[i];
       }
       // zero blocks: above the diagonal of Linv, below it of Uinv
       for (int u = tid; u < CL_BB * CL_BB; u += 256) {
