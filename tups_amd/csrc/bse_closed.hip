@@ -1403,9 +1403,15 @@ __global__ __launch_bounds__(256) void ir_final(IrArgs a) {
 
 // ---- host -------------------------------------------------------------------------
 
+// Workspace of one pass: up to 16 GiB (1024 q at D = 320 in one launch, so
+// the workgroups' tail is paid once per call), at most half of the device
+// memory that is free or already held by this workspace, at least 1 GiB.
 static int closed_reserve(tups_ctx* c, size_t per_q, size_t extra, int* qbatch) {
   const tups_sizes& s = c->sz;
-  const size_t want = ((size_t)4 << 30) / per_q;  // about 4 GiB of workspace per pass
+  size_t budget = (size_t)16 << 30, fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess)
+    budget = std::min(budget, std::max((size_t)1 << 30, (fr + c->closed_ws_bytes) / 2));
+  const size_t want = budget / per_q;
   *qbatch = (int)std::max<size_t>(1, std::min<size_t>(s.nq, want));
   const size_t need = per_q * *qbatch + extra + 256;
   if (c->closed_ws_bytes < need) {
