@@ -91,13 +91,52 @@ __device__ __forceinline__ C csub(C a, C b) {
 }
 template <class C>
 __device__ __forceinline__ C cinv(C a) {
-  // 1 / a with the larger component scaled out (Smith's method)
+  // 1 / a with the larger component scaled out (Smith's method), two divisions
   if (fabs(a.x) >= fabs(a.y)) {
-    const auto r = a.y / a.x, d = a.x + a.y * r;
-    return mkc(1 / d, -r / d);
+    const auto r = a.y / a.x, id = 1 / (a.x + a.y * r);
+    return mkc(id, -r * id);
   }
-  const auto r = a.x / a.y, d = a.y + a.x * r;
-  return mkc(r / d, -1 / d);
+  const auto r = a.x / a.y, id = 1 / (a.y + a.x * r);
+  return mkc(r * id, -id);
+}
+
+// Largest 64-bit key over the wave (unsigned order), in every lane: DPP steps
+// leave the row maxima in every lane of a row, row_bcast carries them to lane
+// 63, which is read out.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long key) {
+  unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
+  auto step = [&](auto ctrl, auto rmask) {
+    constexpr int CT = decltype(ctrl)::value, RM = decltype(rmask)::value;
+    const unsigned nhi = (unsigned)__builtin_amdgcn_update_dpp((int)hi, (int)hi, CT, RM, 0xf, false);
+    const unsigned nlo = (unsigned)__builtin_amdgcn_update_dpp((int)lo, (int)lo, CT, RM, 0xf, false);
+    if (nhi > hi || (nhi == hi && nlo > lo)) {
+      hi = nhi;
+      lo = nlo;
+    }
+  };
+  step(std::integral_constant<int, 0xb1>{}, std::integral_constant<int, 0xf>{});   // quad_perm [1,0,3,2]
+  step(std::integral_constant<int, 0x4e>{}, std::integral_constant<int, 0xf>{});   // quad_perm [2,3,0,1]
+  step(std::integral_constant<int, 0x141>{}, std::integral_constant<int, 0xf>{});  // row_half_mirror
+  step(std::integral_constant<int, 0x140>{}, std::integral_constant<int, 0xf>{});  // row_mirror
+  step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xa>{});  // row_bcast:15 to rows 1, 3
+  step(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xc>{});  // row_bcast:31 to rows 2, 3
+  hi = (unsigned)__builtin_amdgcn_readlane((int)hi, 63);
+  lo = (unsigned)__builtin_amdgcn_readlane((int)lo, 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Pivot key of a candidate: the bits of |.|_1 (a non-negative double, so its
+// bits order as unsigned integers) with the low 10 bits replaced by 1023 -
+// off (off < 1024, the row's offset in the panel), so the largest key is the
+// largest magnitude (to 2^-42) and, among equal ones, the lowest row; 0 for
+// no candidate and for NaN.
+__device__ __forceinline__ unsigned long long pivot_key(double m, int off) {
+  if (!(m >= 0.0)) return 0ull;
+  return ((unsigned long long)__double_as_longlong(m) & ~0x3ffull) | (unsigned long long)(1023 - off);
+}
+__device__ __forceinline__ unsigned long long pivot_key(float m, int off) { return pivot_key((double)m, off); }
+__device__ __forceinline__ double pivot_mag(unsigned long long key) {
+  return __longlong_as_double((long long)(key & ~0x3ffull));
 }
 
 // f(std::integral_constant<int, I>{}) for I = B .. E - 1, each I a compile-time
@@ -819,8 +858,7 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
   C (*Li)[CL_NB] = reinterpret_cast<C (*)[CL_NB]>(&Bs1[0][0]);
   __shared__ R red_v[256 * (1 - RQ) + 1];
   __shared__ int red_i[256 * (1 - RQ) + 1];
-  __shared__ R pv[2][4 * RQ + 1];              // per-wave pivot candidates, two buffers
-  __shared__ int pi[2][4 * RQ + 1];
+  __shared__ unsigned long long pk[2][4 * RQ + 1];  // per-wave pivot keys, two buffers
   __shared__ C cand[2][4 * RQ + 1][CL_NB * RQ + 1];  // each wave's candidate pivot row, two buffers
   __shared__ C crow[2][CL_NB * RQ + 1];              // row c, which the pivot row replaces
   __shared__ int piv[CL_BB * RQ + 1];          // pivot row of each column of the block
@@ -869,27 +907,17 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
           C pc[RP];  // this thread's entries of column c
 #pragma unroll
           for (int i = 0; i < RP; ++i) pc[i] = v[i][j];
-          R best = -1;
-          int bi = c;
+          unsigned long long key = 0ull;
 #pragma unroll
           for (int i = 0; i < RP; ++i) {
             const int row = s0 + tid + 256 * i;
             if (row >= c && row < D) {
-              const R m = fabs(pc[i].x) + fabs(pc[i].y);
-              if (m > best) {
-                best = m;
-                bi = row;
-              }
+              const unsigned long long k2 = pivot_key(fabs(pc[i].x) + fabs(pc[i].y), row - s0);
+              if (k2 > key) key = k2;
             }
           }
-          for (int off = 32; off > 0; off >>= 1) {
-            const R ob = __shfl_xor(best, off);
-            const int oi = __shfl_xor(bi, off);
-            if (ob > best || (ob == best && oi < bi)) {
-              best = ob;
-              bi = oi;
-            }
-          }
+          key = wave_max_u64(key);  // the wave's candidate
+          const int bi = key ? s0 + 1023 - (int)(key & 0x3ff) : c;
           const int tb = (bi - s0) & 255, ib = (bi - s0) >> 8;
           if (tid == tb) {
 #pragma unroll
@@ -903,24 +931,20 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
 #pragma unroll
             for (int jj = 0; jj < CL_NB; ++jj) crow[buf][jj] = v[0][jj];
           }
-          if (lane == 0) {
-            pv[buf][wave] = best;
-            pi[buf][wave] = bi;
-          }
+          if (lane == 0) pk[buf][wave] = key;
           __syncthreads();
-          R pmax = pv[buf][0];
-          int p = pi[buf][0], pw = 0;
+          unsigned long long kmax = pk[buf][0];
+          int pw = 0;
 #pragma unroll
           for (int w = 1; w < 4; ++w) {
-            const R ob = pv[buf][w];
-            const int oi = pi[buf][w];
-            if (ob > pmax || (ob == pmax && oi < p)) {
-              pmax = ob;
-              p = oi;
+            const unsigned long long k2 = pk[buf][w];
+            if (k2 > kmax) {
+              kmax = k2;
               pw = w;
             }
           }
-          if (!(pmax > 0)) {
+          const int p = s0 + 1023 - (int)(kmax & 0x3ff);
+          if (!(pivot_mag(kmax) > 0)) {
             if (info == 0) info = c + 1;
             if (tid == 0) piv[c - k0] = c;
             return;
