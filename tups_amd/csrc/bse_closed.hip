@@ -163,57 +163,56 @@ struct ClosedArgs {
 
 __global__ void closed_clear(int* info) { *info = 0; }
 
-// (Gamma chi)(row, (l2, nu)) = sum_l1 Gamma(row, (l1, nu)) chi_nu(l1, l2)
-__device__ __forceinline__ double2 gamma_chi(const double2* G, const double2* cb, int D, int nf, int nl, int row,
-                                             int l2, int nu) {
-  double sr = 0.0, si = 0.0;
-  for (int l1 = 0; l1 < nl; ++l1) {
-    const double2 g = G[(size_t)(l1 * nf + nu) * D + row];
-    const double2 c = cb[l1 + nl * l2];
-    sr += g.x * c.x - g.y * c.y;
-    si += g.x * c.y + g.y * c.x;
-  }
-  return make_double2(sr, si);
-}
-
-// Direct mode: [I - kappa Gamma chi | Gamma]; column j of the left block is
-// (l2, nu) = (j / Nf, j % Nf).  grid: (2D, 4 * nqb); one thread per row.
-__global__ __launch_bounds__(256) void closed_build(ClosedArgs a) {
-  const int j = blockIdx.x, mat = blockIdx.y;
+// [I - kappa Gamma chi | Gamma] (direct mode) or [fp32(I - kappa Gamma chi) | I]
+// (refinement mode, A formed in FP64 and rounded).  One workgroup per (nu,
+// matrix) forms the Nl columns (l2, nu) of the left block from the Nl columns
+// (l1, nu) of Gamma, each read once: (Gamma chi)(row, (l2, nu)) =
+// sum_l1 Gamma(row, (l1, nu)) chi_nu(l1, l2); workgroup Nf + nu writes the
+// columns (l, nu) of the right block.  grid: (2 Nf, 4 nqb); one thread per row.
+constexpr int CL_NL_MAX = 9;
+template <class T, bool IR>
+__global__ __launch_bounds__(256) void closed_build_nu(ClosedArgs a) {
+  const int x = blockIdx.x, mat = blockIdx.y;
   const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
   const int D = a.D, nf = a.nf, nl = a.nl;
   const double2* G = a.gam[r] + (size_t)D * D * q;
-  double2* Wc = static_cast<double2*>(a.W) + (size_t)mat * 2 * D * D + (size_t)j * D;
-  if (j >= D) {
-    for (int row = threadIdx.x; row < D; row += 256) Wc[row] = G[(size_t)(j - D) * D + row];
+  T* W = static_cast<T*>(a.W) + (size_t)mat * 2 * D * D;
+  auto put = [](double re, double im) -> T {
+    if constexpr (IR) return make_float2((float)re, (float)im);
+    else return make_double2(re, im);
+  };
+  if (x >= nf) {
+    const int nu = x - nf;
+    for (int l = 0; l < nl; ++l) {
+      const int j = l * nf + nu;
+      T* Wc = W + (size_t)(D + j) * D;
+      for (int row = threadIdx.x; row < D; row += 256) {
+        if constexpr (IR) Wc[row] = put(row == j ? 1.0 : 0.0, 0.0);
+        else Wc[row] = G[(size_t)j * D + row];
+      }
+    }
     return;
   }
-  const int l2 = j / nf, nu = j - l2 * nf;
-  const double2* cb = a.chi[r >> 1] + (size_t)nf * nl * nl * q + (size_t)nu * nl * nl;
+  const int nu = x;
+  const double2* cb = a.chi[r >> 1] + (size_t)nf * nl * nl * q + (size_t)nu * nl * nl;  // chi_nu(l1, l2) at l1 + nl l2
   const double kap = a.kappa[r];
   for (int row = threadIdx.x; row < D; row += 256) {
-    const double2 s = gamma_chi(G, cb, D, nf, nl, row, l2, nu);
-    Wc[row] = make_double2((row == j ? 1.0 : 0.0) - kap * s.x, -kap * s.y);
-  }
-}
-
-// Refinement mode: [fp32(I - kappa Gamma chi) | I], A formed in FP64 and rounded.
-__global__ __launch_bounds__(256) void ir_build32(ClosedArgs a) {
-  const int j = blockIdx.x, mat = blockIdx.y;
-  const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
-  const int D = a.D, nf = a.nf, nl = a.nl;
-  const double2* G = a.gam[r] + (size_t)D * D * q;
-  float2* Wc = static_cast<float2*>(a.W) + (size_t)mat * 2 * D * D + (size_t)j * D;
-  if (j >= D) {
-    for (int row = threadIdx.x; row < D; row += 256) Wc[row] = make_float2(row == j - D ? 1.f : 0.f, 0.f);
-    return;
-  }
-  const int l2 = j / nf, nu = j - l2 * nf;
-  const double2* cb = a.chi[r >> 1] + (size_t)nf * nl * nl * q + (size_t)nu * nl * nl;
-  const double kap = a.kappa[r];
-  for (int row = threadIdx.x; row < D; row += 256) {
-    const double2 s = gamma_chi(G, cb, D, nf, nl, row, l2, nu);
-    Wc[row] = make_float2((float)((row == j ? 1.0 : 0.0) - kap * s.x), (float)(-kap * s.y));
+    double2 g[CL_NL_MAX];
+#pragma unroll
+    for (int l1 = 0; l1 < CL_NL_MAX; ++l1)
+      if (l1 < nl) g[l1] = G[(size_t)(l1 * nf + nu) * D + row];
+    for (int l2 = 0; l2 < nl; ++l2) {
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int l1 = 0; l1 < CL_NL_MAX; ++l1)
+        if (l1 < nl) {
+          const double2 c = cb[l1 + nl * l2];
+          sr += g[l1].x * c.x - g[l1].y * c.y;
+          si += g[l1].x * c.y + g[l1].y * c.x;
+        }
+      const int j = l2 * nf + nu;
+      W[(size_t)j * D + row] = put((row == j ? 1.0 : 0.0) - kap * sr, -kap * si);
+    }
   }
 }
 
@@ -1484,6 +1483,7 @@ static size_t closed_b64_elems(size_t D) { return (size_t)CL_BB * D + 3 * (size_
 int launch_bse_closed(tups_ctx* c) {
   const tups_sizes& s = c->sz;
   const size_t D = s.nz;
+  TUPS_CHECK(s.nl <= CL_NL_MAX, TUPS_EINVAL, "closed form: at most 9 form factors");
   const bool blk = closed_blocked(c);
   const size_t per_mat = (2 * D * D + (blk ? closed_b64_elems(D) : 0)) * sizeof(double2);
   int qbatch = 0;
@@ -1502,7 +1502,7 @@ int launch_bse_closed(tups_ctx* c) {
     a.nqb = std::min(qbatch, s.nq - q0);
     const unsigned mats = 4u * (unsigned)a.nqb;
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL(closed_build, dim3((unsigned)(2 * D), mats), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((closed_build_nu<double2, false>), dim3((unsigned)(2 * s.nf), mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (blk && closed_regs(c))
       hipLaunchKernelGGL((closed_solve_b64<double, true, kClosedRp>), dim3(mats), dim3(256), 0, c->stream, a);
@@ -1524,6 +1524,7 @@ int launch_bse_closed(tups_ctx* c) {
 
 int launch_bse_closed_ir(tups_ctx* c) {
   const tups_sizes& s = c->sz;
+  TUPS_CHECK(s.nl <= CL_NL_MAX, TUPS_EINVAL, "closed form: at most 9 form factors");
   const size_t D = s.nz, DD = D * D;
   const int tiles_n = (int)((D + CL_T - 1) / CL_T);
   // per matrix: [A32 | I] (2 D^2 float2), R and chi X (D^2 double2 each), tile partials, res
@@ -1574,7 +1575,7 @@ int launch_bse_closed_ir(tups_ctx* c) {
     const unsigned mats = 4u * (unsigned)a.nqb;
     const unsigned gblk = (unsigned)(tiles_n * tiles_n) * mats;
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL(ir_build32, dim3((unsigned)(2 * D), mats), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((closed_build_nu<float2, true>), dim3((unsigned)(2 * s.nf), mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (blk && closed_regs(c))
       hipLaunchKernelGGL((closed_solve_b64<float, false, kClosedRp>), dim3(mats), dim3(256), 0, c->stream, a);
