@@ -477,7 +477,7 @@ def main():
     closed = {"value": round(nq / (cl_step_ms / 1e3), 2), "unit": "q-points/s", "ms_per_call": round(cl_step_ms, 3),
               "achieved": round(cl_flops / (cl_step_ms / 1e3) / 1e12, 3), "peak": FP64_PEAK_TFLOPS,
               "unit_roofline": "TFLOP/s", "frac": round(cl_flops / (cl_step_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
-              "flops_per_call": cl_flops, "kernel": "closed_build + closed_solve_b64 (64-column block Gauss-Jordan)"}
+              "flops_per_call": cl_flops, "kernel": "closed_build_nu + closed_solve_b64 (64-column block Gauss-Jordan, register sub-panels)"}
     print(f"[bench] closed form: {cl_step_ms:.1f} ms per call", file=sys.stderr, flush=True)
     # --- closed form by FP32 LU (Gauss-Jordan inverse) + FP64 iterative refinement -------
     batched_bse_inputs(ctx, U=U, seed=0)
@@ -493,7 +493,7 @@ def main():
     refined = {"value": round(nq / (ir_step_ms / 1e3), 2), "unit": "q-points/s", "ms_per_call": round(ir_step_ms, 3),
                "refinement_steps": ir_steps, "residual": ir_res, "residual_bar": 1e-12,
                "vs_direct_fp64": round(cl_step_ms / ir_step_ms, 3),
-               "kernel": "ir_build32 + closed_solve_b64<float> (FP32 MFMA) + ir_gemm (FP32 / FP64 MFMA) + ir_chi "
+               "kernel": "closed_build_nu<float2> + closed_solve_b64<float> (FP32 MFMA) + ir_gemm (FP32 / FP64 MFMA) + ir_chi "
                          "+ ir_check",
                "flops_per_call": {"fp32": (12.0 + 8.0 * (1 + ir_steps)) * D ** 3 * 4 * nq,
                                   "fp64": 8.0 * (ir_steps + 1) * D ** 3 * 4 * nq}}
