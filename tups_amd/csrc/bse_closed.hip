@@ -1261,24 +1261,40 @@ struct IrArgs {
 
 __global__ void ir_clear(unsigned long long* rmax) { *rmax = 0ull; }
 
-// Y = chi X: Y((l1, nu), c) = sum_l2 chi_nu(l1, l2) X((l2, nu), c).  grid (D, 4 nqb)
+// Y = chi X: Y((l1, nu), c) = sum_l2 chi_nu(l1, l2) X((l2, nu), c), X the
+// solution in the output arrays.  One workgroup per (IR_CHI_COLS columns,
+// matrix); thread t takes the pairs u = t + 256 k of (nu, c), nu fastest,
+// and forms all Nl rows (l1, nu) of each from the Nl entries (l2, nu) of
+// column c, each read once.  grid: (ceil(D / IR_CHI_COLS), 4 nqb)
+constexpr int IR_CHI_COLS = 16;
 __global__ __launch_bounds__(256) void ir_chi(IrArgs a) {
-  const int col = blockIdx.x, mat = blockIdx.y;
+  const int mat = blockIdx.y, c0 = blockIdx.x * IR_CHI_COLS;
   const int r = mat / a.nqb, q = a.q0 + (mat - r * a.nqb);
   const int D = a.D, nf = a.nf, nl = a.nl;
-  const double2* X = a.x[r] + (size_t)D * D * q + (size_t)col * D;
-  double2* Y = a.y + (size_t)mat * D * D + (size_t)col * D;
+  const double2* X = a.x[r] + (size_t)D * D * q;
+  double2* Y = a.y + (size_t)mat * D * D;
   const double2* cq = a.chi[r >> 1] + (size_t)nf * nl * nl * q;
-  for (int row = threadIdx.x; row < D; row += 256) {
-    const int l1 = row / nf, nu = row - l1 * nf;
+  const int ncols = min(IR_CHI_COLS, D - c0);
+  for (int u = threadIdx.x; u < nf * ncols; u += 256) {
+    const int nu = u % nf, c = c0 + u / nf;
     const double2* cb = cq + (size_t)nu * nl * nl;
-    double sr = 0.0, si = 0.0;
-    for (int l2 = 0; l2 < nl; ++l2) {
-      const double2 c = cb[l1 + nl * l2], x = X[l2 * nf + nu];
-      sr += c.x * x.x - c.y * x.y;
-      si += c.x * x.y + c.y * x.x;
+    const double2* Xc = X + (size_t)c * D + nu;
+    double2* Yc = Y + (size_t)c * D + nu;
+    double2 x[CL_NL_MAX];
+#pragma unroll
+    for (int l2 = 0; l2 < CL_NL_MAX; ++l2)
+      if (l2 < nl) x[l2] = Xc[(size_t)l2 * nf];
+    for (int l1 = 0; l1 < nl; ++l1) {
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int l2 = 0; l2 < CL_NL_MAX; ++l2)
+        if (l2 < nl) {
+          const double2 cc = cb[l1 + nl * l2];
+          sr += cc.x * x[l2].x - cc.y * x[l2].y;
+          si += cc.x * x[l2].y + cc.y * x[l2].x;
+        }
+      Yc[(size_t)l1 * nf] = make_double2(sr, si);
     }
-    Y[row] = make_double2(sr, si);
   }
 }
 
@@ -1590,7 +1606,8 @@ int launch_bse_closed_ir(tups_ctx* c) {
     int steps = 0;
     double res = INFINITY;
     for (;;) {
-      hipLaunchKernelGGL(ir_chi, dim3((unsigned)D, mats), dim3(256), 0, c->stream, b);
+      hipLaunchKernelGGL(ir_chi, dim3((unsigned)((D + IR_CHI_COLS - 1) / IR_CHI_COLS), mats), dim3(256), 0,
+                         c->stream, b);
       TUPS_HIP(hipGetLastError());
       hipLaunchKernelGGL(ir_clear, dim3(1), dim3(1), 0, c->stream, b.rmax);
       TUPS_HIP(hipGetLastError());
