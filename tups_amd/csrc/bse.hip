@@ -182,6 +182,13 @@ __device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, cons
     }
   };
 
+  // 16 x 16 blocks of this wave that hold some row and column below D (uniform:
+  // a partial tile's blocks beyond D skip their MFMAs)
+  bool blk_ok[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) blk_ok[i][j] = row0 + wr + 16 * i < D && col0 + wc + 16 * j < D;
   auto mfma_chunk = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < KC / 4; ++ks) {
@@ -205,6 +212,7 @@ __device__ __forceinline__ void ladder_tile(const double2* __restrict__ Gm, cons
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
+          if (!blk_ok[i][j]) continue;
           if (K3) {
             acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc_re[i][j], 0, 0, 0);
             acc_p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acc_p2[i][j], 0, 0, 0);
