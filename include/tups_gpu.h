@@ -372,8 +372,10 @@ const char* tups_variant_name(int32_t kernel, int32_t variant);
 /* forces a variant id (must match the context's Nl); -1 restores the default.
  * TUPS_K_BSE_CLOSED and TUPS_K_BSE_IR choose the elimination of the closed
  * forms: 0 "gj_panel16" (16-column panels), 1 "gj_block64" (64-column blocks,
- * each passing the later columns through memory once; the default from
- * D = 128 on) */
+ * each passing the later columns through memory once), 2 "gj_block64_rp" (the
+ * same with each 16-column sub-panel factored on registers, D <= 512, above
+ * that as 1; the default from D = 128 on).  The closed forms work in passes
+ * whose workspace is at most 16 GiB and half of the free device memory. */
 int tups_set_variant(tups_ctx* ctx, int32_t kernel, int32_t variant);
 /* number of variant ids of `kernel` (ids 0 .. n-1; a ladder or crossing
  * variant serves one Nl, a split-GEMM or elimination variant every Nl) */
