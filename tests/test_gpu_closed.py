@@ -36,7 +36,8 @@ def _say(*a):
      (4, 8, 5, 0),      # Nz=40: partial last panel (8 columns)
      (4, 6, 5, 3),      # Nz=30: panel of 14, odd tile edges
      (4, 8, 9, 2),      # Nz=72: two update tiles, a block of 64 and one of 8
-     (8, 64, 5, 2)],    # Nz=320: the batched-BSE shape, five blocks
+     (8, 64, 5, 2),     # Nz=320: the batched-BSE shape, five blocks
+     (8, 64, 9, 1)],    # Nz=576: above the register sub-panel's 512 rows
 )
 def test_closed_form_parity(nx, nf, nl, nq, variant):
     from tups_amd import BSE_CLOSED_LU, CHI, F, GAMMA
@@ -143,7 +144,7 @@ def test_closed_form_reports_singular_system(variant):
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize(
     "nx,nf,nl,nq",
-    [(4, 16, 1, 0), (4, 8, 5, 0), (4, 6, 5, 3), (4, 8, 9, 2), (8, 64, 5, 2)],
+    [(4, 16, 1, 0), (4, 8, 5, 0), (4, 6, 5, 3), (4, 8, 9, 2), (8, 64, 5, 2), (8, 64, 9, 1)],
 )
 def test_refined_closed_form_parity(nx, nf, nl, nq, variant):
     """FP32 LU (Gauss-Jordan inverse) + FP64 iterative refinement: the residual
