@@ -155,7 +155,7 @@ struct ClosedArgs {
   const double2* chi[2];
   void* W;      // [mat][2D columns][D rows], column-major: double2 (direct) or float2 (inverse)
   void* S;      // closed_solve_b64: [mat][CL_BB columns][D rows], the block column being factored
-  void* T;      // closed_solve_b64: [mat][3][CL_BB][CL_BB], L11^-1, U11^-1 and their product
+  void* T;      // closed_solve_b64: [mat][2][CL_BB][CL_BB], L11^-1 (then T = U11^-1 L11^-1) and U11^-1
   int* info;    // first failing pivot (1-based column) over all matrices, 0 if none
   int D, nf, nl, q0, nqb;
   double kappa[4];
@@ -868,9 +868,8 @@ __global__ __launch_bounds__(256, 2) void closed_solve_b64(ClosedArgs a) {
   const int D = a.D, D2 = 2 * D;
   C* W = static_cast<C*>(a.W) + (size_t)mat * 2 * D * D;
   C* S = static_cast<C*>(a.S) + (size_t)mat * CL_BB * D;
-  C* Linv = static_cast<C*>(a.T) + (size_t)mat * 3 * CL_BB * CL_BB;  // column-major, leading dimension CL_BB
+  C* Linv = static_cast<C*>(a.T) + (size_t)mat * 2 * CL_BB * CL_BB;  // column-major, leading dimension CL_BB
   C* Uinv = Linv + CL_BB * CL_BB;
-  C* T = Uinv + CL_BB * CL_BB;
   const int tid = threadIdx.x;
   const C one = mkc((R)1, (R)0), zero = mkc((R)0, (R)0);
   int info = 0;
@@ -1494,7 +1493,7 @@ static bool closed_regs(const tups_ctx* c) {
   return closed_blocked(c) && c->closed_variant != 1 && c->sz.nz <= 256 * kClosedRp;
 }
 // scratch of closed_solve_b64 per matrix, in complex elements
-static size_t closed_b64_elems(size_t D) { return (size_t)CL_BB * D + 3 * (size_t)CL_BB * CL_BB; }
+static size_t closed_b64_elems(size_t D) { return (size_t)CL_BB * D + 2 * (size_t)CL_BB * CL_BB; }
 
 int launch_bse_closed(tups_ctx* c) {
   const tups_sizes& s = c->sz;
