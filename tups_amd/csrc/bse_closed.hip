@@ -19,7 +19,12 @@
 //   4. X = U11^-1 Y on the panel rows;
 //   5. rows above the panel: Z -= P_above X                (MFMA);
 // after the last panel the right-hand block holds the solution.  Work per
-// (q, r): 12 D^3 real flops.
+// (q, r): 12 D^3 real flops.  From D = 128 the block form (closed_solve_b64)
+// runs the same elimination with 64-column blocks: the block column factored
+// in a side buffer (on registers, 16 columns at a time, for D <= 512), T =
+// A11^-1 from blocked triangular inverses, the block rows multiplied by T and
+// every other row updated by rank-64 MFMA products whose operands stream
+// into LDS by direct global-to-LDS loads.
 //
 // Refinement mode (TUPS_BSE_CLOSED_IR): the same elimination in FP32 on
 // [fp32(A) | I] gives the inverse of the FP32-rounded A (its LU with partial
