@@ -561,7 +561,7 @@ def main():
     crossing = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4),
                 "traffic": profile_traffic("crossing_pae_nl1_nx16_nf64.md"),
-                "kernel": f"pae_stream_nl1 variant {cvar}" + (" (PhiR re-index folded in)" if cvar.startswith("nl1_f")
+                "kernel": f"pae_stream_nl1 variant {cvar}" + (" (PhiR re-index folded in)" if gat_n == 0
                                                                else " (+ pae_pregather_nl1)"),
                 "avg_launch_ms": round(pae_ms / pae_n, 4), "pregather_ms": round(gat_ms / max(1, gat_n), 4),
                 "bytes_per_call": cbytes,

@@ -326,6 +326,86 @@ __global__ __launch_bounds__(256) void pae_stream_nl1_fused(PaeNl1Args a) {
   }
 }
 
+// pae_stream_nl1_fused with one workgroup per (tile, pair, qw) over every
+// momentum: the Y entries are formed once per (nu1, nu2, qw) and the NIBZ
+// momenta stream through KC at a time, so few registers per thread (and more
+// waves per SIMD) carry the streams
+template <int KC, bool PF>
+__global__ __launch_bounds__(256, 8) void pae_stream_nl1_all(PaeNl1Args a) {
+  const int nf = a.nf, nw = a.nw;
+  const size_t nf2 = (size_t)nf * nf;
+  const int tx = (nf + 63) >> 6, ty = (nf + 3) >> 2;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = logical % (tx * ty);
+  const int rest = logical / (tx * ty);
+  const int pair = rest & 1;
+  const int w0 = rest >> 1;
+  const int nu1 = (tile % tx) * 64 + (threadIdx.x & 63);
+  const int nu2 = (tile / tx) * 4 + (threadIdx.x >> 6);
+  if (nu1 >= nf || nu2 >= nf) return;
+  const int ta = pair == 0 ? 0 : 2, tb = ta + 1;
+  const int ka = pair == 0 ? 0 : 4;
+  const size_t base = (size_t)nu1 + (size_t)nf * nu2 + nf2 * w0, mstride = nf2 * nw;
+  const double2 *phia = a.phi[ta] + base, *phib = a.phi[tb] + base;
+  const int nm = a.nibz;
+  // PF: the next chunk's loads are issued before the current chunk is
+  // combined and stored (the first chunk's before the Y entries are formed)
+  double2 fa[KC], fb[KC];
+  if (PF) {
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int m = min(k, nm - 1);
+      fa[k] = ld_stream(phia + mstride * m);
+      fb[k] = ld_stream(phib + mstride * m);
+    }
+  }
+  const double2 *pd = a.phir[0], *pm = a.phir[1], *ps = a.phir[2], *pt = a.phir[3];
+  const double2 ya0 = y_entry(ka + 0, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+  const double2 yb0 = y_entry(ka + 1, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+  const double2 ya1 = y_entry(ka + 2, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+  const double2 yb1 = y_entry(ka + 3, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+  const size_t li = (size_t)nu1 + (size_t)nf * ((size_t)nu2 + (size_t)nf * w0);
+  const double2 la = a.lam[ta][li], lb = a.lam[tb][li];
+  const double2* Ma = a.M[pair == 0 ? 0 : 2];
+  const double2* Mb = a.M[pair == 0 ? 1 : 3];
+  for (int mb = 0; mb < nm; mb += KC) {
+    double2 na[KC], nb[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int m = min(mb + k + (PF ? KC : 0), nm - 1);
+      na[k] = ld_stream(phia + mstride * m);
+      nb[k] = ld_stream(phib + mstride * m);
+    }
+    if (!PF) {
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        fa[k] = na[k];
+        fb[k] = nb[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int m = mb + k;
+      if (m >= nm) break;
+      const double2 m0 = Ma[m], m1 = Mb[m];
+      double2 oa = fa[k], ob = fb[k];
+      oa.x += la.x * a.nc_aux + (m0.x * ya0.x - m0.y * ya0.y + m1.x * ya1.x - m1.y * ya1.y) * a.inv_nx;
+      oa.y += la.y * a.nc_aux + (m0.x * ya0.y + m0.y * ya0.x + m1.x * ya1.y + m1.y * ya1.x) * a.inv_nx;
+      ob.x += lb.x * a.nc_aux + (m0.x * yb0.x - m0.y * yb0.y + m1.x * yb1.x - m1.y * yb1.y) * a.inv_nx;
+      ob.y += lb.y * a.nc_aux + (m0.x * yb0.y + m0.y * yb0.x + m1.x * yb1.y + m1.y * yb1.x) * a.inv_nx;
+      st_stream(a.F[ta] + base + mstride * m, oa);
+      st_stream(a.F[tb] + base + mstride * m, ob);
+    }
+    if (PF) {
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        fa[k] = na[k];
+        fb[k] = nb[k];
+      }
+    }
+  }
+}
+
 typedef void (*PaeKernel)(PaeNl1Args);
 struct PaeVariant {
   int cols;   // columns per thread (column kernels)
@@ -334,6 +414,7 @@ struct PaeVariant {
   PaeKernel fn;
   const char* name;
   bool fused = false;  // forms the re-indexed PhiR itself (no pre-pass)
+  bool all = false;    // one workgroup covers every momentum (km at a time)
 };
 static const PaeVariant kPaeVariants[] = {
     {2, 0, true, pae_stream_nl1<2, true>, "nl1_c2"},
@@ -347,6 +428,8 @@ static const PaeVariant kPaeVariants[] = {
     {1, 16, true, pae_stream_nl1_mom<16>, "nl1_m16"},
     {1, 8, true, pae_stream_nl1_fused<8>, "nl1_f8", true},
     {1, 16, true, pae_stream_nl1_fused<16>, "nl1_f16", true},
+    {1, 2, true, pae_stream_nl1_all<2, false>, "nl1_a2", true, true},
+    {1, 1, true, pae_stream_nl1_all<1, true>, "nl1_a1p", true, true},
 };
 constexpr int kNumPaeVariants = sizeof(kPaeVariants) / sizeof(kPaeVariants[0]);
 
@@ -361,7 +444,8 @@ static int pae_nl1_launch(tups_ctx* c, int v, const PaeNl1Args& b) {
   const tups_sizes& s = c->sz;
   const int cols = kPaeVariants[v].cols, km = kPaeVariants[v].km;
   const long long tiles = (long long)((s.nf + 63) / 64) * ((s.nf + 4 * cols - 1) / (4 * cols));
-  const long long nb1 = km ? tiles * ((s.nibz + km - 1) / km) * 2 * s.nw : tiles * 2 * s.nq;
+  const long long ngm = kPaeVariants[v].all ? 1 : (s.nibz + km - 1) / (km ? km : 1);
+  const long long nb1 = km ? tiles * ngm * 2 * s.nw : tiles * 2 * s.nq;
   TUPS_CHECK(nb1 < (1LL << 31), TUPS_EINVAL, "crossing grid too large");
   hipLaunchKernelGGL(kPaeVariants[v].fn, dim3((unsigned)nb1), dim3(256), 0, c->stream, b);
   TUPS_HIP(hipGetLastError());
