@@ -172,8 +172,8 @@ __global__ void closed_clear(int* info) { *info = 0; }
 // (refinement mode, A formed in FP64 and rounded).  One workgroup per (nu,
 // matrix) forms the Nl columns (l2, nu) of the left block from the Nl columns
 // (l1, nu) of Gamma, each read once: (Gamma chi)(row, (l2, nu)) =
-// sum_l1 Gamma(row, (l1, nu)) chi_nu(l1, l2); workgroup Nf + nu writes the
-// columns (l, nu) of the right block.  grid: (2 Nf, 4 nqb); one thread per row.
+// sum_l1 Gamma(row, (l1, nu)) chi_nu(l1, l2), and writes the columns (l1, nu)
+// of the right block from the same reads.  grid: (Nf, 4 nqb); one thread per row.
 constexpr int CL_NL_MAX = 9;
 template <class T, bool IR>
 __global__ __launch_bounds__(256) void closed_build_nu(ClosedArgs a) {
@@ -186,18 +186,6 @@ __global__ __launch_bounds__(256) void closed_build_nu(ClosedArgs a) {
     if constexpr (IR) return make_float2((float)re, (float)im);
     else return make_double2(re, im);
   };
-  if (x >= nf) {
-    const int nu = x - nf;
-    for (int l = 0; l < nl; ++l) {
-      const int j = l * nf + nu;
-      T* Wc = W + (size_t)(D + j) * D;
-      for (int row = threadIdx.x; row < D; row += 256) {
-        if constexpr (IR) Wc[row] = put(row == j ? 1.0 : 0.0, 0.0);
-        else Wc[row] = G[(size_t)j * D + row];
-      }
-    }
-    return;
-  }
   const int nu = x;
   const double2* cb = a.chi[r >> 1] + (size_t)nf * nl * nl * q + (size_t)nu * nl * nl;  // chi_nu(l1, l2) at l1 + nl l2
   const double kap = a.kappa[r];
@@ -205,7 +193,13 @@ __global__ __launch_bounds__(256) void closed_build_nu(ClosedArgs a) {
     double2 g[CL_NL_MAX];
 #pragma unroll
     for (int l1 = 0; l1 < CL_NL_MAX; ++l1)
-      if (l1 < nl) g[l1] = G[(size_t)(l1 * nf + nu) * D + row];
+      if (l1 < nl) {
+        const int j = l1 * nf + nu;
+        g[l1] = G[(size_t)j * D + row];
+        // the right-hand side: Gamma's column j (the identity's for the FP32 inverse)
+        if constexpr (IR) W[(size_t)(D + j) * D + row] = put(row == j ? 1.0 : 0.0, 0.0);
+        else W[(size_t)(D + j) * D + row] = g[l1];
+      }
     for (int l2 = 0; l2 < nl; ++l2) {
       double sr = 0.0, si = 0.0;
 #pragma unroll
@@ -1522,7 +1516,7 @@ int launch_bse_closed(tups_ctx* c) {
     a.nqb = std::min(qbatch, s.nq - q0);
     const unsigned mats = 4u * (unsigned)a.nqb;
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL((closed_build_nu<double2, false>), dim3((unsigned)(2 * s.nf), mats), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((closed_build_nu<double2, false>), dim3((unsigned)s.nf, mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (blk && closed_regs(c))
       hipLaunchKernelGGL((closed_solve_b64<double, true, kClosedRp>), dim3(mats), dim3(256), 0, c->stream, a);
@@ -1595,7 +1589,7 @@ int launch_bse_closed_ir(tups_ctx* c) {
     const unsigned mats = 4u * (unsigned)a.nqb;
     const unsigned gblk = (unsigned)(tiles_n * tiles_n) * mats;
     if (int rc = timer_begin(c)) return rc;
-    hipLaunchKernelGGL((closed_build_nu<float2, true>), dim3((unsigned)(2 * s.nf), mats), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((closed_build_nu<float2, true>), dim3((unsigned)s.nf, mats), dim3(256), 0, c->stream, a);
     TUPS_HIP(hipGetLastError());
     if (blk && closed_regs(c))
       hipLaunchKernelGGL((closed_solve_b64<float, false, kClosedRp>), dim3(mats), dim3(256), 0, c->stream, a);
