@@ -328,8 +328,9 @@ __global__ __launch_bounds__(256) void pae_stream_nl1_fused(PaeNl1Args a) {
 
 // pae_stream_nl1_fused with one workgroup per (tile, pair, qw) over every
 // momentum: the Y entries are formed once per (nu1, nu2, qw) and the NIBZ
-// momenta stream through KC at a time, so few registers per thread (and more
-// waves per SIMD) carry the streams
+// momenta stream through KC at a time, so few registers per thread carry the
+// streams (eight waves per SIMD, at most 64 VGPRs: at Nx = 16, Nf = 64 the
+// 2048 workgroups run as one round)
 template <int KC, bool PF>
 __global__ __launch_bounds__(256, 8) void pae_stream_nl1_all(PaeNl1Args a) {
   const int nf = a.nf, nw = a.nw;
