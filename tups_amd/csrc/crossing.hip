@@ -407,6 +407,84 @@ __global__ __launch_bounds__(256, 8) void pae_stream_nl1_all(PaeNl1Args a) {
   }
 }
 
+// pae_stream_nl1_all<1, true> with two entries per thread, (nu1, nu2) and
+// (nu1, nu2 + 4) of a 64 x 8 frequency tile: twice the loads in flight per
+// thread at four waves per SIMD (at most 128 VGPRs), the same waves in all
+template <int EPT>
+__global__ __launch_bounds__(256, 8 / EPT) void pae_stream_nl1_multi(PaeNl1Args a) {
+  const int nf = a.nf, nw = a.nw;
+  const size_t nf2 = (size_t)nf * nf;
+  const int tx = (nf + 63) >> 6, ty = (nf + 4 * EPT - 1) / (4 * EPT);
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = logical % (tx * ty);
+  const int rest = logical / (tx * ty);
+  const int pair = rest & 1;
+  const int w0 = rest >> 1;
+  const int nu1 = (tile % tx) * 64 + (threadIdx.x & 63);
+  const int nu2b = (tile / tx) * (4 * EPT) + (threadIdx.x >> 6);
+  if (nu1 >= nf || nu2b >= nf) return;
+  const int ta = pair == 0 ? 0 : 2, tb = ta + 1;
+  const int ka = pair == 0 ? 0 : 4;
+  const size_t mstride = nf2 * nw;
+  const int nm = a.nibz;
+  bool ok[EPT];
+  size_t base[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int nu2 = nu2b + 4 * e;
+    ok[e] = nu2 < nf;
+    base[e] = (size_t)nu1 + (size_t)nf * (ok[e] ? nu2 : nu2b) + nf2 * w0;
+  }
+  // the first momentum's loads before the Y entries are formed
+  double2 fa[EPT], fb[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    fa[e] = ld_stream(a.phi[ta] + base[e]);
+    fb[e] = ld_stream(a.phi[tb] + base[e]);
+  }
+  const double2 *pd = a.phir[0], *pm = a.phir[1], *ps = a.phir[2], *pt = a.phir[3];
+  double2 ya0[EPT], yb0[EPT], ya1[EPT], yb1[EPT], la[EPT], lb[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int nu2 = ok[e] ? nu2b + 4 * e : nu2b;
+    ya0[e] = y_entry(ka + 0, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+    yb0[e] = y_entry(ka + 1, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+    ya1[e] = y_entry(ka + 2, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+    yb1[e] = y_entry(ka + 3, nu1 + 1, nu2 + 1, w0 + 1, 0, nf, nw, pd, pm, ps, pt);
+    const size_t li = (size_t)nu1 + (size_t)nf * ((size_t)nu2 + (size_t)nf * w0);
+    la[e] = a.lam[ta][li];
+    lb[e] = a.lam[tb][li];
+  }
+  const double2* Ma = a.M[pair == 0 ? 0 : 2];
+  const double2* Mb = a.M[pair == 0 ? 1 : 3];
+  for (int m = 0; m < nm; ++m) {
+    const int mn = min(m + 1, nm - 1);
+    double2 na[EPT], nb[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      na[e] = ld_stream(a.phi[ta] + base[e] + mstride * mn);
+      nb[e] = ld_stream(a.phi[tb] + base[e] + mstride * mn);
+    }
+    const double2 m0 = Ma[m], m1 = Mb[m];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      if (!ok[e]) continue;
+      double2 oa = fa[e], ob = fb[e];
+      oa.x += la[e].x * a.nc_aux + (m0.x * ya0[e].x - m0.y * ya0[e].y + m1.x * ya1[e].x - m1.y * ya1[e].y) * a.inv_nx;
+      oa.y += la[e].y * a.nc_aux + (m0.x * ya0[e].y + m0.y * ya0[e].x + m1.x * ya1[e].y + m1.y * ya1[e].x) * a.inv_nx;
+      ob.x += lb[e].x * a.nc_aux + (m0.x * yb0[e].x - m0.y * yb0[e].y + m1.x * yb1[e].x - m1.y * yb1[e].y) * a.inv_nx;
+      ob.y += lb[e].y * a.nc_aux + (m0.x * yb0[e].y + m0.y * yb0[e].x + m1.x * yb1[e].y + m1.y * yb1[e].x) * a.inv_nx;
+      st_stream(a.F[ta] + base[e] + mstride * m, oa);
+      st_stream(a.F[tb] + base[e] + mstride * m, ob);
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      fa[e] = na[e];
+      fb[e] = nb[e];
+    }
+  }
+}
+
 typedef void (*PaeKernel)(PaeNl1Args);
 struct PaeVariant {
   int cols;   // columns per thread (column kernels)
@@ -431,6 +509,8 @@ static const PaeVariant kPaeVariants[] = {
     {1, 16, true, pae_stream_nl1_fused<16>, "nl1_f16", true},
     {1, 2, true, pae_stream_nl1_all<2, false>, "nl1_a2", true, true},
     {1, 1, true, pae_stream_nl1_all<1, true>, "nl1_a1p", true, true},
+    {2, 1, true, pae_stream_nl1_multi<2>, "nl1_e2p", true, true},
+    {4, 1, true, pae_stream_nl1_multi<4>, "nl1_e4p", true, true},
 };
 constexpr int kNumPaeVariants = sizeof(kPaeVariants) / sizeof(kPaeVariants[0]);
 
