@@ -407,9 +407,10 @@ __global__ __launch_bounds__(256, 8) void pae_stream_nl1_all(PaeNl1Args a) {
   }
 }
 
-// pae_stream_nl1_all<1, true> with two entries per thread, (nu1, nu2) and
-// (nu1, nu2 + 4) of a 64 x 8 frequency tile: twice the loads in flight per
-// thread at four waves per SIMD (at most 128 VGPRs), the same waves in all
+// pae_stream_nl1_all<1, true> with EPT entries per thread, (nu1, nu2 + 4 e)
+// of a 64 x 4 EPT frequency tile: EPT times the loads in flight per thread at
+// 8 / EPT waves per SIMD (at most 512 EPT / 8 VGPRs); at Nx = 16, Nf = 64 the
+// grid is still one round
 template <int EPT>
 __global__ __launch_bounds__(256, 8 / EPT) void pae_stream_nl1_multi(PaeNl1Args a) {
   const int nf = a.nf, nw = a.nw;
